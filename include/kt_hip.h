@@ -1,0 +1,158 @@
+/*
+ * kt_hip.h -- C ABI of the MI355X (gfx950) HIP backend for the KNdS geodesic tracer.
+ *
+ * Plain C: POD structs, pointers and sizes; no C++ or torch types cross this boundary.
+ * Every entry point returns KT_OK (0) or a negative KT_E* code; the message of the last
+ * failure on a context is available from kt_hip_last_error().
+ *
+ * Reference interface being replaced (KerrTraceCpp2 tree):
+ *   cuda_render(KNdSParams_CUDA, CameraParams_CUDA, bool require_fp64)
+ *       -> std::vector<uint32_t> ABGR                         gpu/cuda/tracer.cuh:11-26
+ *   called from render_image()'s USE_CUDA branch                main.cpp:2413-2466
+ *   GeoPixel (28 B) / .kgeo Phase-1 output                      main.cpp:84-137
+ *   colorize_buffer() Phase-2 shading                           main.cpp:1817-1866
+ */
+#ifndef KT_HIP_H
+#define KT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KT_HIP_ABI_VERSION 1
+
+/* status codes */
+#define KT_OK            0
+#define KT_E_ARG        -1   /* invalid argument (null pointer, bad size, bad enum) */
+#define KT_E_DEVICE     -2   /* no gfx950 device / HIP runtime error */
+#define KT_E_NOFP64     -3   /* --gpu-fp64 strict mode requested but the device lacks FP64 */
+#define KT_E_ALLOC      -4   /* device allocation failed */
+#define KT_E_LAUNCH     -5   /* kernel launch or execution failed */
+#define KT_E_UNSUPPORTED -6  /* option combination not provided by this backend */
+
+/* enums (int32 in the structs) */
+#define KT_CHART_BL 0          /* Boyer-Lindquist (r, theta, phi)                                */
+#define KT_CHART_KS 1          /* ingoing Kerr-Schild; any Lambda (reference: Lambda == 0 only)  */
+#define KT_INTEGRATOR_RK4 0    /* adaptive RK4 step doubling  (geodesic.hpp:204-257)             */
+#define KT_INTEGRATOR_DOPRI5 1 /* Dormand-Prince 5(4) with FSAL (geodesic.hpp:312-414)           */
+#define KT_PRECISION_FP64 0    /* every operation in binary64                                     */
+#define KT_PRECISION_MIXED 1   /* binary32 stage derivatives, binary64 state/error control/islands */
+
+/* reference behaviours that a drop-in must reproduce (all set by default, KT_QUIRKS_REFERENCE) */
+#define KT_QUIRK_DOPRI5_PHI_FROZEN   (1u << 0) /* DOPRI5 never advances phi (geodesic.hpp:300-310)        */
+#define KT_QUIRK_ESCAPE_POSITIONAL   (1u << 1) /* escape fills {phi_disk,theta_esc,phi_esc} =
+                                                  {theta,phi,0} (main.cpp:360,428-433,1572,1650,1659)     */
+#define KT_QUIRK_KS_INIT_NO_SHIFT    (1u << 2) /* KS initial covector keeps the BL spatial components
+                                                  (main.cpp:1373-1430)                                    */
+#define KT_QUIRK_BUNDLE_HORIZON_ESC  (1u << 3) /* bundle horizon/reject-cap -> outcome 0, angles 0
+                                                  (ray_bundle.hpp:426-428)                                */
+#define KT_QUIRK_BUNDLE_ROOT_SKIP    (1u << 4) /* bundle step whose Hermite root search fails skips the
+                                                  horizon/escape tests (ray_bundle.hpp:357-365)           */
+#define KT_QUIRK_KS_LAMBDA_TO_BL     (1u << 5) /* KS with Lambda != 0 runs in BL (main.cpp:2034-2041);
+                                                  NOT in KT_QUIRKS_REFERENCE: this backend has a KNdS KS  */
+#define KT_QUIRKS_REFERENCE 0x1Fu
+
+/* Per-pixel Phase-1 record, byte-identical to GeoPixel (main.cpp:84-94). */
+typedef struct kt_geopixel {
+    uint8_t outcome;   /* 0 escaped, 1 disk, 2 horizon */
+    uint8_t pad[3];
+    float r, redshift, magnif, phi_disk, theta_esc, phi_esc;
+} kt_geopixel;
+
+/* One frame: physics + camera.  Angles in DEGREES, as on the reference CLI (main.cpp:2548-2684). */
+typedef struct kt_frame {
+    double M, a, Q, Lambda;
+    double r_obs, theta_deg, phi_deg, fov_deg;
+    double disk_out;             /* outer disk radius; inner radius is r_isco (main.cpp:2080-2081) */
+    double offset_x, offset_y;   /* sub-pixel camera offsets (camera.hpp:38-47)                    */
+    int32_t width, height;
+} kt_frame;
+
+/* Integration / precision / shading options shared by all frames of a call. */
+typedef struct kt_opts {
+    int32_t chart;          /* KT_CHART_*                                                   */
+    int32_t integrator;     /* KT_INTEGRATOR_*                                              */
+    int32_t precision;      /* KT_PRECISION_*; --gpu-fp64 selects FP64                        */
+    int32_t bundles;        /* 1: Jacobi ray bundles (ray_bundle.hpp), BL chart              */
+    int32_t max_steps;      /* accepted-step cap (main.cpp:194)                              */
+    uint32_t quirks;        /* KT_QUIRK_* mask                                              */
+    double step_init;       /* initial affine step (main.cpp:195)                            */
+    double rtol, atol;      /* RK4: err < rtol (absolute, geodesic.hpp:234);
+                               DOPRI5: scale = atol + rtol*|y0| (reference: atol = rtol = tol) */
+    double exposure, gamma, temp_scale;   /* ColorParams (main.cpp:72-81)                   */
+    int32_t tuning_waves_per_cu;          /* 0: autotuned / default                        */
+    int32_t tuning_block;                 /* 0: default (256)                              */
+} kt_opts;
+
+/* Background: RGB8 equirectangular image (BackgroundImage, main.cpp:140-171). */
+typedef struct kt_background {
+    const uint8_t* rgb;     /* host pointer, w*h*3 bytes, may be NULL (black sky) */
+    int32_t width, height;
+} kt_background;
+
+/* Per-frame scalars as computed on the device (KGeoMeta fields + horizon). */
+typedef struct kt_frame_info {
+    double r_horizon, r_isco, r_disk_in, r_disk_out, r_escape;
+    int32_t eff_chart, eff_integrator;
+} kt_frame_info;
+
+/* Counters gathered on the device during a render. */
+typedef struct kt_stats {
+    double trace_ms;          /* kernel time of the geodesic launch (HIP events)           */
+    double total_ms;          /* setup + trace + shade, device time                        */
+    uint64_t rays;            /* primary rays traced                                       */
+    uint64_t steps_accepted;  /* accepted integrator steps                                  */
+    uint64_t steps_rejected;  /* rejected attempts                                          */
+    uint64_t rhs_fp32;        /* right-hand-side evaluations in binary32                    */
+    uint64_t rhs_fp64;        /* right-hand-side evaluations in binary64                    */
+    uint64_t events;          /* Hermite disk searches run                                  */
+} kt_stats;
+
+typedef struct kt_hip_ctx kt_hip_ctx;
+
+/* Library / context life cycle. */
+int  kt_hip_abi_version(void);
+void kt_hip_default_opts(kt_opts* opts);  /* reference defaults: BL, RK4, tol 1e-7, step 1, 500000 */
+int  kt_hip_create(int device, kt_hip_ctx** out);
+void kt_hip_destroy(kt_hip_ctx* ctx);
+const char* kt_hip_last_error(const kt_hip_ctx* ctx);
+/* device name, CU count, FP64 capability; name_out may be NULL */
+int  kt_hip_device_info(kt_hip_ctx* ctx, char* name_out, size_t name_len, int* n_cu, int* has_fp64);
+
+/* Set the background used by escaped rays (copied to the device). */
+int kt_hip_set_background(kt_hip_ctx* ctx, const kt_background* bg);
+
+/* Render n_frames frames in one persistent launch (host buffers).
+ * rgba_out:  sum(W*H) uint32 ABGR (R in bits 0-7, alpha 0xFF; tracer.cu:482), may be NULL
+ * geo_out:   sum(W*H) GeoPixel records, may be NULL
+ * info_out:  n_frames kt_frame_info, may be NULL
+ * stats:     may be NULL
+ * Frames are laid out back to back in frame order.                                            */
+int kt_hip_render(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const kt_opts* opts,
+                  uint32_t* rgba_out, kt_geopixel* geo_out, kt_frame_info* info_out, kt_stats* stats);
+
+/* Same as kt_hip_render, but outputs stay in device memory owned by the context
+ * (valid until the next call on the context); the timed path of the benchmark.
+ * d_rgba / d_geo receive device pointers (may be NULL).                                      */
+int kt_hip_render_device(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const kt_opts* opts,
+                         void** d_rgba, void** d_geo, kt_stats* stats);
+
+/* Phase 2 on the device: GeoPixel records -> ABGR (colorize_buffer, main.cpp:1817-1866). */
+int kt_hip_colorize(kt_hip_ctx* ctx, const kt_geopixel* geo, int64_t n, double M, double r_isco,
+                    const kt_opts* opts, uint32_t* rgba_out);
+
+/* Per-frame scalars only (setup kernel): horizon, ISCO, escape radius. */
+int kt_hip_frame_info(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const kt_opts* opts,
+                      kt_frame_info* info_out);
+
+/* Procedural background: cells_x x cells_y checkerboard, colours from a seeded generator. */
+int kt_make_checker_background(uint64_t seed, int32_t width, int32_t height, int32_t cells_x,
+                               int32_t cells_y, uint8_t* rgb_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KT_HIP_H */

@@ -1,0 +1,143 @@
+"""kerrtracecpp2_amd -- MI355X (gfx950) HIP backend for the KerrTraceCpp2 geodesic hot path.
+
+The compute path is libkt_hip.so (hand-written HIP kernels behind the C ABI of
+include/kt_hip.h).  This package is a thin Python host over that ABI: it loads
+the in-tree library and raises if it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import (GEOPIXEL_DTYPE, KT_CHART_BL, KT_CHART_KS, KT_INTEGRATOR_DOPRI5, KT_INTEGRATOR_RK4,  # noqa: F401
+                   KT_PRECISION_FP64, KT_PRECISION_MIXED, KT_QUIRKS_REFERENCE, KtFrame, KtOpts, default_opts,
+                   make_frame)
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libkt_hip.so")
+
+_lib = None
+
+
+class KtError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Load libkt_hip.so from the package directory (raises KtError if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise KtError(f"HIP library not built: {LIB_PATH} missing (run __graft_entry__.build())")
+        _lib = _abi.bind(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Context:
+    """One device context (kt_hip_ctx): owns device buffers, the stream and the background."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.kt_hip_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise KtError(f"kt_hip_create({device}) failed with status {rc}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.kt_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.kt_hip_last_error(self.h)
+            raise KtError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def device_info(self):
+        name = ctypes.create_string_buffer(256)
+        ncu = ctypes.c_int()
+        f64 = ctypes.c_int()
+        self._check(self.lib.kt_hip_device_info(self.h, name, 256, ctypes.byref(ncu), ctypes.byref(f64)),
+                    "kt_hip_device_info")
+        return {"name": name.value.decode(), "n_cu": ncu.value, "fp64": bool(f64.value)}
+
+    def set_background(self, rgb):
+        """rgb: (h, w, 3) uint8 equirectangular image, or None."""
+        if rgb is None:
+            self._check(self.lib.kt_hip_set_background(self.h, None), "kt_hip_set_background")
+            return
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        bg = _abi.KtBackground(rgb.ctypes.data, rgb.shape[1], rgb.shape[0])
+        self._check(self.lib.kt_hip_set_background(self.h, ctypes.byref(bg)), "kt_hip_set_background")
+
+    def render(self, frames, opts, want_rgba=True, want_geo=True):
+        """Render frames (list of KtFrame) -> dict(rgba, geo, info, stats), frames concatenated."""
+        frames = list(frames)
+        arr = (_abi.KtFrame * len(frames))(*frames)
+        npix = sum(f.width * f.height for f in frames)
+        rgba = np.zeros(npix, np.uint32) if want_rgba else None
+        geo = np.zeros(npix, GEOPIXEL_DTYPE) if want_geo else None
+        info = (_abi.KtFrameInfo * len(frames))()
+        st = _abi.KtStats()
+        self._check(self.lib.kt_hip_render(self.h, arr, len(frames), ctypes.byref(opts), _ptr(rgba), _ptr(geo),
+                                           info, ctypes.byref(st)), "kt_hip_render")
+        return {"rgba": rgba, "geo": geo, "info": [_info_dict(i) for i in info], "stats": st.as_dict()}
+
+    def render_device(self, frames, opts, shade=True):
+        """Render with outputs left in device memory; returns stats (the benchmark's timed call)."""
+        frames = list(frames)
+        arr = (_abi.KtFrame * len(frames))(*frames)
+        d_rgba = ctypes.c_void_p()
+        d_geo = ctypes.c_void_p()
+        st = _abi.KtStats()
+        self._check(self.lib.kt_hip_render_device(self.h, arr, len(frames), ctypes.byref(opts),
+                                                  ctypes.byref(d_rgba) if shade else None, ctypes.byref(d_geo),
+                                                  ctypes.byref(st)), "kt_hip_render_device")
+        return st.as_dict()
+
+    def frame_info(self, frames, opts):
+        frames = list(frames)
+        arr = (_abi.KtFrame * len(frames))(*frames)
+        info = (_abi.KtFrameInfo * len(frames))()
+        self._check(self.lib.kt_hip_frame_info(self.h, arr, len(frames), ctypes.byref(opts), info),
+                    "kt_hip_frame_info")
+        return [_info_dict(i) for i in info]
+
+    def colorize(self, geo, M, r_isco, opts):
+        geo = np.ascontiguousarray(geo, dtype=GEOPIXEL_DTYPE)
+        rgba = np.zeros(geo.size, np.uint32)
+        self._check(self.lib.kt_hip_colorize(self.h, _ptr(geo), geo.size, float(M), float(r_isco),
+                                             ctypes.byref(opts), _ptr(rgba)), "kt_hip_colorize")
+        return rgba
+
+
+def _info_dict(i):
+    return {k: getattr(i, k) for k, _ in i._fields_}
+
+
+def checker_background(seed=1234, width=512, height=256, cells_x=32, cells_y=16):
+    """Seeded cells_x x cells_y checkerboard on the celestial sphere (RGB8, equirectangular)."""
+    lib = load_library()
+    out = np.zeros((height, width, 3), np.uint8)
+    rc = lib.kt_make_checker_background(int(seed), int(width), int(height), int(cells_x), int(cells_y), _ptr(out))
+    if rc != 0:
+        raise KtError("kt_make_checker_background failed")
+    return out
+
+
+def rgba_to_rgb(rgba):
+    """ABGR uint32 (R in bits 0-7) -> (n, 3) uint8, as main.cpp:2438-2443 unpacks it."""
+    rgba = np.asarray(rgba, dtype=np.uint32)
+    return np.stack([(rgba & 0xFF), (rgba >> 8) & 0xFF, (rgba >> 16) & 0xFF], axis=-1).astype(np.uint8)

@@ -1,0 +1,230 @@
+// ============================================================================
+//  kt_device.hpp -- per-ray physics of the KNdS geodesic tracer (HIP, gfx950).
+//
+//  Everything here is templated on the arithmetic type T (double or float) so
+//  the binary64 and the mixed-precision kernels share one source.  Functions
+//  are __host__ __device__ so the identical code can also be compiled for the
+//  host by the test-only CPU check library.
+//
+//  Reference algorithm anchors (KerrTraceCpp2 tree):
+//    BL metric / Hamiltonian ........ knds_metric.hpp:45-132
+//    RHS (dH/dp, -dH/dq) ............ geodesic.hpp:143-166 (finite differences there;
+//                                     analytic here, same Hamiltonian)
+//    RK4 step doubling .............. geodesic.hpp:169-257
+//    DOPRI5 + FSAL .................. geodesic.hpp:261-414
+//    Hermite event location ......... geodesic.hpp:49-140
+//    disk redshift .................. main.cpp:329-341
+//    camera tetrad .................. camera.hpp:38-145
+// ============================================================================
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define KT_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define KT_HD inline
+namespace kt {
+using std::isfinite; using std::fma; using std::sqrt; using std::fabs; using std::fmin; using std::fmax;
+using std::floor; using std::frexp; using std::ldexp; using std::cbrt; using std::exp; using std::log;
+using std::log10; using std::pow; using std::fmod; using std::fmaf; using std::atan2;
+}
+#endif
+
+#include <stdint.h>
+
+namespace kt {
+
+static constexpr double kPi = 3.14159265358979323846;
+// cos(M_PI/2) and sin(M_PI/2) as evaluated by the reference (glibc, binary64)
+static constexpr double kCosHalfPi = 6.123233995736766e-17;
+
+// ---------------------------------------------------------------------------
+//  small math layer
+// ---------------------------------------------------------------------------
+KT_HD double kt_sqrt(double x) { return sqrt(x); }
+KT_HD float kt_sqrt(float x) { return sqrtf(x); }
+KT_HD double kt_abs(double x) { return fabs(x); }
+KT_HD float kt_abs(float x) { return fabsf(x); }
+KT_HD void kt_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+KT_HD void kt_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+KT_HD bool kt_finite(double x) { return isfinite(x); }
+KT_HD bool kt_finite(float x) { return isfinite(x); }
+
+// reciprocal: hardware estimate refined by Newton steps (full precision for
+// the normal-range operands that occur here; no denormal/inf special cases)
+KT_HD double kt_rcp(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double x = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, x, 1.0);
+    x = fma(x, e, x);
+    e = fma(-d, x, 1.0);
+    x = fma(x, e, x);
+    return x;
+#else
+    return 1.0 / d;
+#endif
+}
+KT_HD float kt_rcp(float d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(d);
+#else
+    return 1.0f / d;
+#endif
+}
+
+// (sin, cos) of theta0 + delta from (sin theta0, cos theta0) and a short
+// Taylor series in delta.  |delta| <= 0.5: truncation below 3e-20 (binary64).
+template <class T> struct SinCos { T s, c; };
+
+KT_HD SinCos<double> rotate(const SinCos<double>& b, double d) {
+    const double d2 = d*d;
+    // sin d = d (1 - d2/(2.3) (1 - d2/(4.5) (1 - ... (1 - d2/(14.15)))))
+    double ps = fma(d2, -1.0/210.0, 1.0);
+    ps = fma(d2 * (-1.0/156.0), ps, 1.0);
+    ps = fma(d2 * (-1.0/110.0), ps, 1.0);
+    ps = fma(d2 * (-1.0/72.0), ps, 1.0);
+    ps = fma(d2 * (-1.0/42.0), ps, 1.0);
+    ps = fma(d2 * (-1.0/20.0), ps, 1.0);
+    ps = fma(d2 * (-1.0/6.0), ps, 1.0);
+    const double sd = d * ps;
+    // 1 - cos d = d2/2 (1 - d2/(3.4) (1 - ... (1 - d2/(15.16))))
+    double pc = fma(d2, -1.0/240.0, 1.0);
+    pc = fma(d2 * (-1.0/182.0), pc, 1.0);
+    pc = fma(d2 * (-1.0/132.0), pc, 1.0);
+    pc = fma(d2 * (-1.0/90.0), pc, 1.0);
+    pc = fma(d2 * (-1.0/56.0), pc, 1.0);
+    pc = fma(d2 * (-1.0/30.0), pc, 1.0);
+    pc = fma(d2 * (-1.0/12.0), pc, 1.0);
+    const double omc = 0.5 * d2 * pc;                 // 1 - cos d, no cancellation
+    SinCos<double> o;
+    o.s = fma(b.c, sd, fma(-b.s, omc, b.s));           // s cos d + c sin d
+    o.c = fma(-b.s, sd, fma(-b.c, omc, b.c));          // c cos d - s sin d
+    return o;
+}
+KT_HD SinCos<float> rotate(const SinCos<float>& b, float d) {
+    const float d2 = d*d;
+    float ps = fmaf(d2, -1.0f/110.0f, 1.0f);
+    ps = fmaf(d2 * (-1.0f/72.0f), ps, 1.0f);
+    ps = fmaf(d2 * (-1.0f/42.0f), ps, 1.0f);
+    ps = fmaf(d2 * (-1.0f/20.0f), ps, 1.0f);
+    ps = fmaf(d2 * (-1.0f/6.0f), ps, 1.0f);
+    const float sd = d * ps;
+    float pc = fmaf(d2, -1.0f/132.0f, 1.0f);
+    pc = fmaf(d2 * (-1.0f/90.0f), pc, 1.0f);
+    pc = fmaf(d2 * (-1.0f/56.0f), pc, 1.0f);
+    pc = fmaf(d2 * (-1.0f/30.0f), pc, 1.0f);
+    pc = fmaf(d2 * (-1.0f/12.0f), pc, 1.0f);
+    const float omc = 0.5f * d2 * pc;
+    SinCos<float> o;
+    o.s = fmaf(b.c, sd, fmaf(-b.s, omc, b.s));
+    o.c = fmaf(-b.s, sd, fmaf(-b.c, omc, b.c));
+    return o;
+}
+static constexpr double kRotateMax = 0.5;
+
+// ---------------------------------------------------------------------------
+//  per-frame physics constants, in the arithmetic type of the RHS
+// ---------------------------------------------------------------------------
+template <class T> struct Knds {
+    T a, a2, M, M2, Q2, lam3, lam3a2, xi, xi2;   // M2 = 2M, lam3 = Lambda/3, lam3a2 = Lambda a^2/3
+};
+
+template <class T, class S> KT_HD Knds<T> knds_cast(const Knds<S>& k) {
+    Knds<T> o;
+    o.a = (T)k.a; o.a2 = (T)k.a2; o.M = (T)k.M; o.M2 = (T)k.M2; o.Q2 = (T)k.Q2;
+    o.lam3 = (T)k.lam3; o.lam3a2 = (T)k.lam3a2; o.xi = (T)k.xi; o.xi2 = (T)k.xi2;
+    return o;
+}
+
+// Right-hand side of Hamilton's equations for H = 1/2 g^{mu nu} p_mu p_nu.
+//   out[0] = dr/dl, out[1] = dtheta/dl, out[2] = dp_r/dl, out[3] = dp_theta/dl, out[4] = dphi/dl
+//
+// BL chart (knds_metric.hpp:99-132), written in the separable form
+//   2 Sigma H = Dr pr^2 + Dt pth^2 - Xi^2 K^2 / Dr + Xi^2 L^2 / (Dt s^2)
+//   K = (r^2+a^2) pt + a pphi,   L = a s^2 pt + pphi
+// which expands to exactly the reference's g^{tt}, g^{t phi}, g^{phi phi}.
+//
+// KS chart: ingoing Kerr(-Schild) coordinates (v, r, theta, phi~) with
+//   dv = dt + Xi (r^2+a^2)/Dr dr, dphi~ = dphi + Xi a / Dr dr, for which
+//   2 Sigma H = Dr pr^2 + 2 Xi pr K + Dt pth^2 + Xi^2 L^2 / (Dt s^2),
+// regular through the horizon for every Lambda; for Lambda = 0 this is the
+// reference's Cartesian Kerr-Schild chart (knds_metric.hpp:135-230) with
+// X + iY = (r + i a) sin(theta) e^{i phi~}, Z = r cos(theta), T = v - r.
+//
+// Arithmetic: one reciprocal of Sigma * Dr * (Dt s^2) (BL) / Sigma * Dt s^2 (KS).
+enum { CHART_BL = 0, CHART_KS = 1 };
+
+template <int CHART, class T>
+KT_HD void rhs(const Knds<T>& P, T r, const SinCos<T>& sc, T pr, T pth, T pt, T pp, T out[5]) {
+    const T s = sc.s, c = sc.c;
+    const T s2 = s*s, c2 = c*c, sc2 = s*c;
+    const T r2 = r*r;
+    const T ra = r2 + P.a2;
+    const T sig = fma(P.a2, c2, r2);
+    const T one_m = fma(-P.lam3, r2, (T)1);                 // 1 - Lambda r^2/3
+    const T dR = fma(ra, one_m, fma(-P.M2, r, P.Q2));       // Delta_r
+    // dDelta_r/dr = 2r(1 - L r^2/3) - (r^2+a^2) 2 L r/3 - 2M
+    const T dR_r = fma((T)2*r, one_m, fma(-(T)2*P.lam3*r, ra, -P.M2));
+    const T dT = fma(P.lam3a2, c2, (T)1);                   // Delta_theta
+    const T dT_th = -(T)2*P.lam3a2*sc2;                      // dDelta_theta/dtheta
+    const T K = fma(ra, pt, P.a*pp);
+    const T L = fma(P.a*s2, pt, pp);
+    const T Kr = (T)2*r*pt;                                  // dK/dr
+    const T Lth = (T)2*P.a*sc2*pt;                           // dL/dtheta
+    const T dTs2 = dT*s2;
+    if (CHART == CHART_BL) {
+        const T p12 = sig*dR;
+        const T inv = kt_rcp(p12*dTs2);
+        const T inv_sig = inv*dR*dTs2;
+        const T inv_dR = inv*sig*dTs2;
+        const T inv_ts = inv*p12;
+        const T A1 = P.xi2*K*inv_dR;                         // Xi^2 K / Dr
+        const T B1 = P.xi2*L*inv_ts;                         // Xi^2 L / (Dt s^2)
+        const T N = fma(dR*pr, pr, fma(dT*pth, pth, fma(-A1, K, B1*L)));
+        out[0] = dR*pr*inv_sig;
+        out[1] = dT*pth*inv_sig;
+        out[4] = fma(-A1, P.a, B1)*inv_sig;
+        // dN/dr = dR_r pr^2 - 2 A1 Kr + A1 K dR_r / Dr
+        const T dNr = fma(dR_r*pr, pr, fma(-(T)2*A1, Kr, A1*K*inv_dR*dR_r));
+        // dN/dth = dT_th pth^2 + 2 B1 Lth - B1 L (dT_th s^2 + 2 Dt s c)/(Dt s^2)
+        const T dNth = fma(dT_th*pth, pth, fma((T)2*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, (T)2*dT*sc2)));
+        const T hs = (T)0.5*inv_sig;
+        out[2] = -fma(-N*(T)2*r, inv_sig, dNr)*hs;
+        out[3] = -fma(N*(T)2*P.a2*sc2, inv_sig, dNth)*hs;
+    } else {
+        const T inv = kt_rcp(sig*dTs2);
+        const T inv_sig = inv*dTs2;
+        const T inv_ts = inv*sig;
+        const T B1 = P.xi2*L*inv_ts;                         // Xi^2 L / (Dt s^2)
+        const T XK = P.xi*K;
+        const T N = fma(dR*pr, pr, fma((T)2*XK, pr, fma(dT*pth, pth, B1*L)));
+        out[0] = fma(dR, pr, XK)*inv_sig;
+        out[1] = dT*pth*inv_sig;
+        out[4] = fma(P.xi*P.a, pr, B1)*inv_sig;
+        // dN/dr = dR_r pr^2 + 2 Xi pr Kr
+        const T dNr = fma(dR_r*pr, pr, (T)2*P.xi*pr*Kr);
+        const T dNth = fma(dT_th*pth, pth, fma((T)2*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, (T)2*dT*sc2)));
+        const T hs = (T)0.5*inv_sig;
+        out[2] = -fma(-N*(T)2*r, inv_sig, dNr)*hs;
+        out[3] = -fma(N*(T)2*P.a2*sc2, inv_sig, dNth)*hs;
+    }
+}
+
+// 2 Sigma H (the numerator N above), for constraint monitoring / renormalisation
+template <int CHART, class T>
+KT_HD T hamiltonian_num(const Knds<T>& P, T r, const SinCos<T>& sc, T pr, T pth, T pt, T pp, T* sig_out) {
+    const T s2 = sc.s*sc.s, c2 = sc.c*sc.c;
+    const T r2 = r*r, ra = r2 + P.a2;
+    const T sig = fma(P.a2, c2, r2);
+    const T dR = fma(ra, fma(-P.lam3, r2, (T)1), fma(-P.M2, r, P.Q2));
+    const T dT = fma(P.lam3a2, c2, (T)1);
+    const T K = fma(ra, pt, P.a*pp);
+    const T L = fma(P.a*s2, pt, pp);
+    if (sig_out) *sig_out = sig;
+    if (CHART == CHART_BL)
+        return dR*pr*pr + dT*pth*pth - P.xi2*K*K/dR + P.xi2*L*L/(dT*s2);
+    return dR*pr*pr + (T)2*P.xi*pr*K + dT*pth*pth + P.xi2*L*L/(dT*s2);
+}
+
+}  // namespace kt

@@ -1,0 +1,815 @@
+// ============================================================================
+//  kt_hip.hip -- HIP kernels (gfx950) and the C ABI of include/kt_hip.h.
+//
+//  Kernels:
+//    kt_setup_kernel    one workgroup per frame: horizon (parallel Delta_r scan +
+//                       bisection), ISCO (exact parallel scan), camera constants
+//    kt_trace_kernel    persistent, work-queue driven: each wave64 pulls 64-ray
+//                       chunks (8x8 pixel tiles) from a device counter and refills
+//                       lanes whose ray finished, so uneven step counts do not idle
+//                       the SIMD; per ray: camera -> adaptive steps -> events -> GeoPixel
+//    kt_shade_kernel    GeoPixel -> ABGR (colorize_buffer)
+//    kt_fill_kernel     zeroes the queue/statistics words (no memset nodes)
+//
+//  Reference interface replaced: cuda_render() (gpu/cuda/tracer.cuh:23-26,
+//  gpu/cuda/tracer.cu:558-610) and its caller render_image() (main.cpp:2413-2466).
+// ============================================================================
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kt_hip.h"
+#include "kt_setup.hpp"
+#include "kt_shade.hpp"
+#include "kt_trace.hpp"
+
+using namespace kt;
+
+static_assert(sizeof(kt_geopixel) == 28, "GeoPixel layout (main.cpp:84-94)");
+
+namespace {
+
+// ---------------------------------------------------------------------------
+//  device-side statistics / queue words
+// ---------------------------------------------------------------------------
+enum {
+    W_QUEUE = 0,       // next global ray index handed out
+    W_ACC, W_REJ, W_RHS32, W_RHS64, W_HERM, W_RAYS,
+    W_COUNT = 8
+};
+
+struct WaveVote {
+    KT_HD static bool any(bool b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __any(b) != 0;
+#else
+        return b;
+#endif
+    }
+};
+
+__global__ void kt_fill_kernel(unsigned long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+//  setup: one workgroup (256 threads) per frame
+// ---------------------------------------------------------------------------
+constexpr int kSetupThreads = 256;
+
+__device__ double bisect_ref(const FrameDev& F, double lo, double hi) {
+    double flo = delta_r_ref(F, lo), fhi = delta_r_ref(F, hi);
+    if (!isfinite(flo) || !isfinite(fhi)) return NAN;
+    for (int i = 0; i < 160; ++i) {
+        const double mid = 0.5*(lo + hi), fm = delta_r_ref(F, mid);
+        if (!isfinite(fm)) break;
+        if ((flo > 0.0 && fm < 0.0) || (flo < 0.0 && fm > 0.0)) { hi = mid; fhi = fm; }
+        else { lo = mid; flo = fm; }
+    }
+    (void)fhi;
+    return 0.5*(lo + hi);
+}
+
+__global__ __launch_bounds__(kSetupThreads) void kt_setup_kernel(FrameDev* frames) {
+    FrameDev& F = frames[blockIdx.x];
+    const int t = threadIdx.x;
+    __shared__ double s_rh;
+    __shared__ int s_nc;
+    __shared__ double s_cr[64];
+    __shared__ int s_ck[64];
+    __shared__ ScanSeg s_seg[kMaxSeg];
+    __shared__ int s_ns;
+    __shared__ unsigned long long s_best;
+    if (t == 0) { s_nc = 0; frame_knds(F); frame_camera_consts(F); }
+    __syncthreads();
+
+    // ---- horizon (knds_metric.hpp:237-315) ----
+    if (fabs(F.Lambda) < 1e-15) {
+        if (t == 0) {
+            const double disc = F.M*F.M - F.a*F.a - F.Q*F.Q;
+            s_rh = F.M + sqrt(disc > 0.0 ? disc : 0.0);
+        }
+    } else {
+        const double scale = fabs(F.M) > 1.0 ? fabs(F.M) : 1.0;
+        const double rmin = (1e-6*scale > 1e-9) ? 1e-6*scale : 1e-9;
+        double rmax;
+        if (F.Lambda > 0.0) { const double q = 1.25*sqrt(3.0/F.Lambda); rmax = 20.0*scale > q ? 20.0*scale : q; }
+        else rmax = 200.0*scale + 20.0*fabs(F.a) + 20.0*fabs(F.Q);
+        if (!(rmax > rmin)) rmax = rmin * 10.0;
+        const int N = 20000;
+        const double span = log(rmax / rmin);
+        const int per = (N + kSetupThreads - 1) / kSetupThreads;
+        const int i0 = 1 + t*per;
+        const int i1 = min(N, i0 + per - 1);
+        if (i0 <= N) {
+            double r0 = (i0 - 1 == 0) ? rmin : rmin * exp(span * (double(i0 - 1) / double(N)));
+            double f0 = delta_r_ref(F, r0);
+            for (int i = i0; i <= i1; ++i) {
+                const double r1 = rmin * exp(span * (double(i) / double(N)));
+                const double f1 = delta_r_ref(F, r1);
+                if (isfinite(f0) && isfinite(f1)) {
+                    if (f0 == 0.0) { const int j = atomicAdd(&s_nc, 1); if (j < 64) { s_cr[j] = r0; s_ck[j] = 3*i; } }
+                    if (f1 == 0.0) { const int j = atomicAdd(&s_nc, 1); if (j < 64) { s_cr[j] = r1; s_ck[j] = 3*i + 1; } }
+                    if ((f0 > 0.0 && f1 < 0.0) || (f0 < 0.0 && f1 > 0.0)) {
+                        const double rr = bisect_ref(F, r0, r1);
+                        const int j = atomicAdd(&s_nc, 1);
+                        if (j < 64) { s_cr[j] = rr; s_ck[j] = 3*i + 2; }
+                    }
+                }
+                r0 = r1; f0 = f1;
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            const int nc = s_nc < 64 ? s_nc : 64;
+            for (int i = 1; i < nc; ++i)     // scan order
+                for (int j = i; j > 0 && s_ck[j-1] > s_ck[j]; --j) {
+                    const int tk = s_ck[j]; s_ck[j] = s_ck[j-1]; s_ck[j-1] = tk;
+                    const double tr = s_cr[j]; s_cr[j] = s_cr[j-1]; s_cr[j-1] = tr;
+                }
+            double roots[16];
+            int nr = 0;
+            for (int c = 0; c < nc; ++c) {
+                const double rr = s_cr[c];
+                if (!isfinite(rr) || rr <= 0.0) continue;
+                bool dup = false;
+                for (int i = 0; i < nr; ++i) {
+                    double rel = 1.0;
+                    if (fabs(roots[i]) > rel) rel = fabs(roots[i]);
+                    if (fabs(rr) > rel) rel = fabs(rr);
+                    if (fabs(roots[i] - rr) <= 1e-8 * rel) { dup = true; break; }
+                }
+                if (!dup && nr < 16) roots[nr++] = rr;
+            }
+            double rh;
+            if (nr == 0) {
+                const double disc = F.M*F.M - F.a*F.a - F.Q*F.Q;
+                rh = F.M + sqrt(disc > 0.0 ? disc : 0.0);
+            } else {
+                for (int i = 1; i < nr; ++i)
+                    for (int j = i; j > 0 && roots[j-1] > roots[j]; --j) {
+                        const double tr = roots[j]; roots[j] = roots[j-1]; roots[j-1] = tr;
+                    }
+                rh = (F.Lambda > 0.0 && nr >= 2) ? roots[nr - 2] : roots[nr - 1];
+            }
+            s_rh = rh;
+        }
+    }
+    __syncthreads();
+    const double rh = s_rh;
+
+    // ---- ISCO (knds_metric.hpp:318-349) ----
+    double risco = 0.0;
+    if (fabs(F.Lambda) < 1e-15 && fabs(F.Q) < 1e-15) {
+        risco = isco_bardeen(F.M, F.a);
+    } else {
+        const double rmax = 30.0*F.M;
+        if (t == 0) s_ns = isco_segments(rh * 1.05, 1e-4*F.M, rmax, s_seg);
+        __syncthreads();
+        const int ns = s_ns;
+        if (ns < 0) {
+            if (t == 0) s_best = 0;   // fallback: sequential loop in thread 0
+            if (t == 0) {
+                double r = rh * 1.05;
+                const double dr = 1e-4*F.M;
+                double Lp = isco_L(F, r);
+                bool done = false;
+                for (; r < rmax; r += dr) {
+                    const double L = isco_L(F, r + dr);
+                    if (L < Lp && Lp > 0.0) { done = true; break; }
+                    Lp = L;
+                }
+                (void)done;
+                s_rh = r;   // reuse as the result slot
+            }
+            __syncthreads();
+            risco = s_rh;
+        } else {
+            const int per = 32;
+            const int64_t round_len = (int64_t)kSetupThreads * per;
+            for (int64_t base = 0;; base += round_len) {
+                if (t == 0) s_best = ~0ull;
+                __syncthreads();
+                const int64_t k0 = base + (int64_t)t * per;
+                double rk = isco_abscissa(s_seg, ns, k0);
+                double Lk = isco_L(F, rk);
+                for (int j = 0; j < per; ++j) {
+                    const int64_t k = k0 + j;
+                    if (!(rk < rmax)) { atomicMin(&s_best, (unsigned long long)k); break; }
+                    const double rn = isco_abscissa(s_seg, ns, k + 1);
+                    const double Ln = isco_L(F, rn);
+                    if (Ln < Lk && Lk > 0.0) { atomicMin(&s_best, (unsigned long long)k); break; }
+                    rk = rn; Lk = Ln;
+                }
+                __syncthreads();
+                const unsigned long long b = s_best;
+                __syncthreads();
+                if (b != ~0ull) { risco = isco_abscissa(s_seg, ns, (int64_t)b); break; }
+            }
+        }
+    }
+    if (t == 0) {
+        F.r_h = rh;
+        F.r_isco = risco;
+        F.r_in = risco;
+        F.r_out = F.disk_out;
+        F.r_esc = F.r_obs * 1.05;
+        F.r_cut = rh * 1.03;
+    }
+}
+
+// ---------------------------------------------------------------------------
+//  persistent trace kernel
+// ---------------------------------------------------------------------------
+struct TraceArgs {
+    const FrameDev* frames;
+    int32_t n_frames;
+    int32_t single;          // all rays in frame 0
+    int64_t total_rays;      // tile-padded ray count over all frames
+    uint32_t* geo;           // 7 dwords per pixel
+    unsigned long long* words;
+    RunOpts O;
+};
+
+constexpr int kBlock = 256;
+constexpr int kChunk = 64;
+
+__device__ __forceinline__ int find_frame(const FrameDev* fr, int n, int64_t g) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (fr[mid].ray_base <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int CHART, int METHOD, int PREC>
+__global__ __launch_bounds__(kBlock) void kt_trace_kernel(TraceArgs A) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    RayState S;
+    RayOut o;
+    bool active = false;
+    int frame = 0;
+    int px = 0, py = 0;
+    Knds<double> P = A.frames[0].P;
+    Knds<float> Pf = knds_cast<float>(P);
+    int64_t pool_next = 0, pool_end = 0;
+    bool exhausted = false;
+    unsigned long long c_acc = 0, c_rej = 0, c_r32 = 0, c_r64 = 0, c_herm = 0, c_rays = 0;
+
+    for (;;) {
+        // ---- refill idle lanes from the wave's pool / the global queue ----
+        const unsigned long long idle = __ballot(!active);
+        if (idle != 0ull && !(exhausted && pool_next >= pool_end)) {
+            const int need = __popcll(idle);
+            const int rank = __popcll(idle & lt_mask);
+            int64_t mine = -1;
+            int64_t avail = pool_end - pool_next;
+            int taken = (int)(avail < need ? avail : need);
+            if (!active && rank < taken) mine = pool_next + rank;
+            pool_next += taken;
+            if (taken < need && !exhausted) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&A.words[W_QUEUE], (unsigned long long)kChunk);
+                base = __shfl(base, 0, 64);
+                if ((int64_t)base >= A.total_rays) {
+                    exhausted = true;
+                } else {
+                    pool_next = (int64_t)base;
+                    pool_end = (int64_t)base + kChunk < A.total_rays ? (int64_t)base + kChunk : A.total_rays;
+                    const int avail2 = (int)(pool_end - pool_next);
+                    const int rest = need - taken;
+                    const int take2 = avail2 < rest ? avail2 : rest;
+                    if (!active && rank >= taken && rank - taken < take2) mine = pool_next + (rank - taken);
+                    pool_next += take2;
+                }
+            }
+            if (mine >= 0) {
+                frame = A.single ? 0 : find_frame(A.frames, A.n_frames, mine);
+                const FrameDev& F = A.frames[frame];
+                const int64_t loc = mine - F.ray_base;
+                const int64_t tile = loc >> 6;
+                const int w = (int)(loc & 63);
+                px = (int)(tile % F.tiles_x) * 8 + (w & 7);
+                py = (int)(tile / F.tiles_x) * 8 + (w >> 3);
+                if (px < F.W && py < F.H) {
+                    if (!A.single) { P = F.P; Pf = knds_cast<float>(P); }
+                    ray_init<CHART, PREC>(F, A.O, px, py, S);
+                    active = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted && pool_next >= pool_end) break;
+            continue;
+        }
+        if (active) {
+            const FrameDev& F = A.frames[A.single ? 0 : frame];
+            bool use64 = true;
+            if (PREC == PREC_MIXED) use64 = WaveVote::any(wants_fp64(F, A.O, S));
+            const bool done = ray_advance<CHART, METHOD, PREC, WaveVote>(F, P, Pf, A.O, use64, S, o);
+            if (done) {
+                uint32_t wv[7];
+                pack_geopixel(o, wv);
+                uint32_t* dst = A.geo + 7 * (F.pix_base + (int64_t)py * F.W + px);
+                for (int i = 0; i < 7; ++i) dst[i] = wv[i];
+                c_acc += (unsigned long long)S.n_acc;
+                c_rej += (unsigned long long)S.rej_total;
+                c_r32 += (unsigned long long)S.rhs32;
+                c_r64 += (unsigned long long)S.rhs64;
+                c_herm += (unsigned long long)S.hermites;
+                c_rays += 1ull;
+                active = false;
+            }
+        }
+    }
+    c_acc = wave_sum(c_acc); c_rej = wave_sum(c_rej); c_r32 = wave_sum(c_r32);
+    c_r64 = wave_sum(c_r64); c_herm = wave_sum(c_herm); c_rays = wave_sum(c_rays);
+    if (lane == 0) {
+        atomicAdd(&A.words[W_ACC], c_acc);
+        atomicAdd(&A.words[W_REJ], c_rej);
+        atomicAdd(&A.words[W_RHS32], c_r32);
+        atomicAdd(&A.words[W_RHS64], c_r64);
+        atomicAdd(&A.words[W_HERM], c_herm);
+        atomicAdd(&A.words[W_RAYS], c_rays);
+    }
+}
+
+// ---------------------------------------------------------------------------
+//  shading
+// ---------------------------------------------------------------------------
+struct ShadeArgs {
+    const FrameDev* frames;
+    int32_t n_frames;
+    int64_t n_pix;
+    const uint32_t* geo;
+    uint32_t* rgba;
+    const uint8_t* bg;
+    int32_t bw, bh;
+    double exposure, gamma, temp_scale;
+    double M_override, risco_override;   // used when frames == nullptr (colour-only)
+};
+
+__global__ __launch_bounds__(256) void kt_shade_kernel(ShadeArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_pix) return;
+    ShadeParams p;
+    if (A.frames) {
+        int lo = 0, hi = A.n_frames - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (A.frames[mid].pix_base <= i) lo = mid; else hi = mid - 1;
+        }
+        p.M = A.frames[lo].M;
+        p.r_isco = A.frames[lo].r_isco;
+    } else {
+        p.M = A.M_override;
+        p.r_isco = A.risco_override;
+    }
+    p.exposure = A.exposure; p.gamma = A.gamma; p.temp_scale = A.temp_scale;
+    const uint32_t* g = A.geo + 7*i;
+    union { uint32_t u; float f; } r, red, mag, th, ph;
+    r.u = g[1]; red.u = g[2]; mag.u = g[3]; th.u = g[5]; ph.u = g[6];
+    A.rgba[i] = shade_pixel((uint8_t)(g[0] & 0xFF), r.f, red.f, mag.f, th.f, ph.f, p, A.bg, A.bw, A.bh);
+}
+
+typedef void (*TraceKernelFn)(TraceArgs);
+
+TraceKernelFn pick_kernel(int chart, int method, int prec) {
+    const int key = chart*4 + method*2 + prec;
+    switch (key) {
+        case 0: return kt_trace_kernel<CHART_BL, METHOD_RK4, PREC_FP64>;
+        case 1: return kt_trace_kernel<CHART_BL, METHOD_RK4, PREC_MIXED>;
+        case 2: return kt_trace_kernel<CHART_BL, METHOD_DP5, PREC_FP64>;
+        case 3: return kt_trace_kernel<CHART_BL, METHOD_DP5, PREC_MIXED>;
+        case 4: return kt_trace_kernel<CHART_KS, METHOD_RK4, PREC_FP64>;
+        case 5: return kt_trace_kernel<CHART_KS, METHOD_RK4, PREC_MIXED>;
+        case 6: return kt_trace_kernel<CHART_KS, METHOD_DP5, PREC_FP64>;
+        case 7: return kt_trace_kernel<CHART_KS, METHOD_DP5, PREC_MIXED>;
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+// ============================================================================
+//  context + C ABI
+// ============================================================================
+struct kt_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    char name[256] = {0};
+    int n_cu = 0;
+    int has_fp64 = 1;
+    FrameDev* d_frames = nullptr; size_t cap_frames = 0;
+    uint32_t* d_geo = nullptr; size_t cap_pix = 0;
+    uint32_t* d_rgba = nullptr;
+    uint8_t* d_bg = nullptr; int bg_w = 0, bg_h = 0; size_t cap_bg = 0;
+    unsigned long long* d_words = nullptr;
+    std::vector<FrameDev> h_frames;
+};
+
+static int fail(kt_hip_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define KT_HIP(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), KT_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+extern "C" {
+
+int kt_hip_abi_version(void) { return KT_HIP_ABI_VERSION; }
+
+void kt_hip_default_opts(kt_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->chart = KT_CHART_BL;
+    o->integrator = KT_INTEGRATOR_RK4;
+    o->precision = KT_PRECISION_FP64;
+    o->bundles = 0;
+    o->max_steps = 500000;          // main.cpp:194
+    o->quirks = KT_QUIRKS_REFERENCE;
+    o->step_init = 1.0;             // main.cpp:195
+    o->rtol = 1e-7;                 // main.cpp:196
+    o->atol = 1e-7;
+    o->exposure = 1.0;              // main.cpp:73-75
+    o->gamma = 2.2;
+    o->temp_scale = 1.0;
+}
+
+int kt_hip_create(int device, kt_hip_ctx** out) {
+    if (!out) return KT_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return KT_E_DEVICE;
+    if (device < 0 || device >= n) return KT_E_ARG;
+    kt_hip_ctx* c = new kt_hip_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return KT_E_DEVICE;
+    }
+    std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    c->n_cu = prop.multiProcessorCount;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        delete c;
+        return KT_E_DEVICE;   // this library carries gfx950 code objects only
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return KT_E_DEVICE; }
+    for (int i = 0; i < 4; ++i)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
+    if (hipMalloc(&c->d_words, W_COUNT * sizeof(unsigned long long)) != hipSuccess) {
+        kt_hip_destroy(c);
+        return KT_E_ALLOC;
+    }
+    *out = c;
+    return KT_OK;
+}
+
+void kt_hip_destroy(kt_hip_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_frames);
+    (void)hipFree(c->d_geo);
+    (void)hipFree(c->d_rgba);
+    (void)hipFree(c->d_bg);
+    (void)hipFree(c->d_words);
+    for (int i = 0; i < 4; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* kt_hip_last_error(const kt_hip_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int kt_hip_device_info(kt_hip_ctx* c, char* name_out, size_t name_len, int* n_cu, int* has_fp64) {
+    if (!c) return KT_E_ARG;
+    if (name_out && name_len) { std::strncpy(name_out, c->name, name_len - 1); name_out[name_len - 1] = 0; }
+    if (n_cu) *n_cu = c->n_cu;
+    if (has_fp64) *has_fp64 = c->has_fp64;
+    return KT_OK;
+}
+
+int kt_hip_set_background(kt_hip_ctx* c, const kt_background* bg) {
+    if (!c) return KT_E_ARG;
+    KT_HIP(c, hipSetDevice(c->device));
+    if (!bg || !bg->rgb || bg->width <= 0 || bg->height <= 0) { c->bg_w = c->bg_h = 0; return KT_OK; }
+    const size_t bytes = (size_t)bg->width * bg->height * 3;
+    if (bytes > c->cap_bg) {
+        (void)hipFree(c->d_bg);
+        c->d_bg = nullptr;
+        c->cap_bg = 0;
+        if (hipMalloc(&c->d_bg, bytes) != hipSuccess) return fail(c, KT_E_ALLOC, "background allocation failed");
+        c->cap_bg = bytes;
+    }
+    KT_HIP(c, hipMemcpy(c->d_bg, bg->rgb, bytes, hipMemcpyHostToDevice));
+    c->bg_w = bg->width;
+    c->bg_h = bg->height;
+    return KT_OK;
+}
+
+static int check_opts(kt_hip_ctx* c, const kt_opts* o) {
+    if (!o) return fail(c, KT_E_ARG, "opts is NULL");
+    if (o->chart != KT_CHART_BL && o->chart != KT_CHART_KS) return fail(c, KT_E_ARG, "bad chart %d", o->chart);
+    if (o->integrator != KT_INTEGRATOR_RK4 && o->integrator != KT_INTEGRATOR_DOPRI5)
+        return fail(c, KT_E_ARG, "bad integrator %d", o->integrator);
+    if (o->precision != KT_PRECISION_FP64 && o->precision != KT_PRECISION_MIXED)
+        return fail(c, KT_E_ARG, "bad precision %d", o->precision);
+    if (o->bundles) return fail(c, KT_E_UNSUPPORTED, "ray bundles are provided by kt_hip_render with bundles=1 "
+                                                     "only in builds with the bundle kernel");
+    if (!(o->rtol > 0.0) || !(o->atol > 0.0)) return fail(c, KT_E_ARG, "tolerances must be positive");
+    return KT_OK;
+}
+
+// Effective chart of a frame: the reference runs KS only for Lambda == 0
+// (main.cpp:2034-2041); KT_QUIRK_KS_LAMBDA_TO_BL reproduces that fallback.
+static int frame_chart(const kt_opts* o, const kt_frame& f) {
+    if (o->chart == KT_CHART_KS && (o->quirks & KT_QUIRK_KS_LAMBDA_TO_BL) && std::fabs(f.Lambda) > 1e-15)
+        return KT_CHART_BL;
+    return o->chart;
+}
+
+static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o,
+                         int64_t* total_rays, int64_t* total_pix, int* chart_out) {
+    if (!frames || n_frames <= 0) return fail(c, KT_E_ARG, "no frames");
+    c->h_frames.assign(n_frames, FrameDev());
+    int64_t rays = 0, pix = 0;
+    const double d2r = kPi / 180.0;
+    int chart = -1;
+    for (int i = 0; i < n_frames; ++i) {
+        const kt_frame& f = frames[i];
+        if (f.width <= 0 || f.height <= 0 || f.width > 65536 || f.height > 65536)
+            return fail(c, KT_E_ARG, "frame %d: bad size %dx%d", i, f.width, f.height);
+        const int ch = frame_chart(o, f);
+        if (chart < 0) chart = ch;
+        else if (chart != ch) return fail(c, KT_E_UNSUPPORTED, "frames of one call must share the effective chart");
+        FrameDev& F = c->h_frames[i];
+        std::memset(&F, 0, sizeof(F));
+        F.M = f.M; F.a = f.a; F.Q = f.Q; F.Lambda = f.Lambda;
+        F.r_obs = f.r_obs;
+        F.th_obs = f.theta_deg * d2r;
+        F.ph_obs = f.phi_deg * d2r;
+        F.fov = f.fov_deg * d2r;
+        F.disk_out = f.disk_out;
+        F.off_x = f.offset_x; F.off_y = f.offset_y;
+        F.W = f.width; F.H = f.height;
+        F.tiles_x = (f.width + 7) / 8;
+        F.tiles_y = (f.height + 7) / 8;
+        F.ray_base = rays;
+        F.pix_base = pix;
+        F.chart = ch;
+        F.ks_ref_init = (ch == KT_CHART_KS && (o->quirks & KT_QUIRK_KS_INIT_NO_SHIFT) && std::fabs(f.Lambda) <= 1e-15)
+                            ? 1 : 0;
+        rays += (int64_t)F.tiles_x * F.tiles_y * 64;
+        pix += (int64_t)f.width * f.height;
+    }
+    if ((size_t)n_frames > c->cap_frames) {
+        (void)hipFree(c->d_frames);
+        c->d_frames = nullptr;
+        c->cap_frames = 0;
+        if (hipMalloc(&c->d_frames, n_frames * sizeof(FrameDev)) != hipSuccess)
+            return fail(c, KT_E_ALLOC, "frame table allocation failed");
+        c->cap_frames = n_frames;
+    }
+    if ((size_t)pix > c->cap_pix) {
+        (void)(void)hipFree(c->d_geo); (void)hipFree(c->d_rgba);
+        c->d_geo = nullptr; c->d_rgba = nullptr; c->cap_pix = 0;
+        if (hipMalloc(&c->d_geo, (size_t)pix * 28) != hipSuccess ||
+            hipMalloc(&c->d_rgba, (size_t)pix * 4) != hipSuccess)
+            return fail(c, KT_E_ALLOC, "output allocation (%lld pixels) failed", (long long)pix);
+        c->cap_pix = pix;
+    }
+    KT_HIP(c, hipMemcpyAsync(c->d_frames, c->h_frames.data(), n_frames * sizeof(FrameDev), hipMemcpyHostToDevice,
+                             c->stream));
+    *total_rays = rays;
+    *total_pix = pix;
+    *chart_out = chart;
+    return KT_OK;
+}
+
+static RunOpts run_opts(const kt_opts* o) {
+    RunOpts R;
+    R.max_steps = o->max_steps < 1 ? 1 : o->max_steps;
+    R.quirks = o->quirks;
+    R.step_init = o->step_init < 1e-10 ? 1e-10 : o->step_init;
+    R.rtol = o->rtol < 1e-10 && o->integrator == KT_INTEGRATOR_RK4 ? 1e-10 : o->rtol;
+    R.atol = o->atol;
+    R.isl_noise = 3e-8;
+    R.isl_r = 2.5;
+    return R;
+}
+
+// setup + trace + shade on the context stream; outputs stay on the device
+static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o, bool shade,
+                            kt_stats* st, int64_t* n_pix_out) {
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    KT_HIP(c, hipSetDevice(c->device));
+    int64_t rays = 0, pix = 0;
+    int chart = 0;
+    rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
+    if (rc) return rc;
+    TraceKernelFn fn = pick_kernel(chart, o->integrator, o->precision);
+    if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");
+
+    // persistent grid: as many resident workgroups as the kernel's register use allows
+    int per_cu = 0;
+    KT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kBlock, 0));
+    if (per_cu < 1) per_cu = 1;
+    if (o->tuning_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, o->tuning_waves_per_cu / (kBlock / 64)));
+    int64_t grid = (int64_t)c->n_cu * per_cu;
+    const int64_t max_useful = (rays + kBlock - 1) / kBlock;
+    if (grid > max_useful) grid = max_useful;
+    if (grid < 1) grid = 1;
+
+    TraceArgs A;
+    A.frames = c->d_frames;
+    A.n_frames = n_frames;
+    A.single = (n_frames == 1) ? 1 : 0;
+    A.total_rays = rays;
+    A.geo = c->d_geo;
+    A.words = c->d_words;
+    A.O = run_opts(o);
+
+    hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
+    KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(kt_setup_kernel, dim3(n_frames), dim3(kSetupThreads), 0, c->stream, c->d_frames);
+    KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(kBlock), 0, c->stream, A);
+    KT_HIP(c, hipGetLastError());
+    KT_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    if (shade) {
+        ShadeArgs S;
+        S.frames = c->d_frames; S.n_frames = n_frames; S.n_pix = pix;
+        S.geo = c->d_geo; S.rgba = c->d_rgba;
+        S.bg = c->bg_w > 0 ? c->d_bg : nullptr; S.bw = c->bg_w; S.bh = c->bg_h;
+        S.exposure = o->exposure; S.gamma = o->gamma; S.temp_scale = o->temp_scale;
+        S.M_override = 1.0; S.risco_override = 6.0;
+        hipLaunchKernelGGL(kt_shade_kernel, dim3((unsigned)((pix + 255) / 256)), dim3(256), 0, c->stream, S);
+        KT_HIP(c, hipGetLastError());
+    }
+    KT_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    if (st) {
+        unsigned long long w[W_COUNT];
+        KT_HIP(c, hipMemcpy(w, c->d_words, sizeof(w), hipMemcpyDeviceToHost));
+        float t_trace = 0.f, t_total = 0.f;
+        (void)hipEventElapsedTime(&t_trace, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&t_total, c->ev[0], c->ev[3]);
+        st->trace_ms = t_trace;
+        st->total_ms = t_total;
+        st->rays = w[W_RAYS];
+        st->steps_accepted = w[W_ACC];
+        st->steps_rejected = w[W_REJ];
+        st->rhs_fp32 = w[W_RHS32];
+        st->rhs_fp64 = w[W_RHS64];
+        st->events = w[W_HERM];
+    }
+    *n_pix_out = pix;
+    return KT_OK;
+}
+
+int kt_hip_render_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o, void** d_rgba,
+                         void** d_geo, kt_stats* st) {
+    if (!c) return KT_E_ARG;
+    int64_t pix = 0;
+    const int rc = render_on_device(c, frames, n_frames, o, d_rgba != nullptr, st, &pix);
+    if (rc) return rc;
+    if (d_rgba) *d_rgba = c->d_rgba;
+    if (d_geo) *d_geo = c->d_geo;
+    return KT_OK;
+}
+
+int kt_hip_render(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o, uint32_t* rgba_out,
+                  kt_geopixel* geo_out, kt_frame_info* info_out, kt_stats* st) {
+    if (!c) return KT_E_ARG;
+    int64_t pix = 0;
+    const int rc = render_on_device(c, frames, n_frames, o, rgba_out != nullptr, st, &pix);
+    if (rc) return rc;
+    if (rgba_out) KT_HIP(c, hipMemcpy(rgba_out, c->d_rgba, (size_t)pix * 4, hipMemcpyDeviceToHost));
+    if (geo_out) KT_HIP(c, hipMemcpy(geo_out, c->d_geo, (size_t)pix * 28, hipMemcpyDeviceToHost));
+    if (info_out) {
+        std::vector<FrameDev> fr(n_frames);
+        KT_HIP(c, hipMemcpy(fr.data(), c->d_frames, n_frames * sizeof(FrameDev), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n_frames; ++i) {
+            info_out[i].r_horizon = fr[i].r_h;
+            info_out[i].r_isco = fr[i].r_isco;
+            info_out[i].r_disk_in = fr[i].r_in;
+            info_out[i].r_disk_out = fr[i].r_out;
+            info_out[i].r_escape = fr[i].r_esc;
+            info_out[i].eff_chart = fr[i].chart;
+            info_out[i].eff_integrator = o->integrator;
+        }
+    }
+    return KT_OK;
+}
+
+int kt_hip_frame_info(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o,
+                      kt_frame_info* info_out) {
+    if (!c || !info_out) return KT_E_ARG;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    KT_HIP(c, hipSetDevice(c->device));
+    int64_t rays = 0, pix = 0;
+    int chart = 0;
+    rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kt_setup_kernel, dim3(n_frames), dim3(kSetupThreads), 0, c->stream, c->d_frames);
+    KT_HIP(c, hipGetLastError());
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<FrameDev> fr(n_frames);
+    KT_HIP(c, hipMemcpy(fr.data(), c->d_frames, n_frames * sizeof(FrameDev), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n_frames; ++i) {
+        info_out[i].r_horizon = fr[i].r_h;
+        info_out[i].r_isco = fr[i].r_isco;
+        info_out[i].r_disk_in = fr[i].r_in;
+        info_out[i].r_disk_out = fr[i].r_out;
+        info_out[i].r_escape = fr[i].r_esc;
+        info_out[i].eff_chart = fr[i].chart;
+        info_out[i].eff_integrator = o->integrator;
+    }
+    return KT_OK;
+}
+
+int kt_hip_colorize(kt_hip_ctx* c, const kt_geopixel* geo, int64_t n, double M, double r_isco, const kt_opts* o,
+                    uint32_t* rgba_out) {
+    if (!c || !geo || !rgba_out || n <= 0 || !o) return KT_E_ARG;
+    KT_HIP(c, hipSetDevice(c->device));
+    if ((size_t)n > c->cap_pix) {
+        (void)(void)hipFree(c->d_geo); (void)hipFree(c->d_rgba);
+        c->d_geo = nullptr; c->d_rgba = nullptr; c->cap_pix = 0;
+        if (hipMalloc(&c->d_geo, (size_t)n * 28) != hipSuccess || hipMalloc(&c->d_rgba, (size_t)n * 4) != hipSuccess)
+            return fail(c, KT_E_ALLOC, "colorize allocation failed");
+        c->cap_pix = n;
+    }
+    KT_HIP(c, hipMemcpyAsync(c->d_geo, geo, (size_t)n * 28, hipMemcpyHostToDevice, c->stream));
+    ShadeArgs S;
+    S.frames = nullptr; S.n_frames = 0; S.n_pix = n;
+    S.geo = c->d_geo; S.rgba = c->d_rgba;
+    S.bg = c->bg_w > 0 ? c->d_bg : nullptr; S.bw = c->bg_w; S.bh = c->bg_h;
+    S.exposure = o->exposure; S.gamma = o->gamma; S.temp_scale = o->temp_scale;
+    S.M_override = M; S.risco_override = r_isco;
+    hipLaunchKernelGGL(kt_shade_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S);
+    KT_HIP(c, hipGetLastError());
+    KT_HIP(c, hipMemcpyAsync(rgba_out, c->d_rgba, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    return KT_OK;
+}
+
+// splitmix64 (Steele, Lea, Flood 2014)
+static uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int kt_make_checker_background(uint64_t seed, int32_t w, int32_t h, int32_t cx, int32_t cy, uint8_t* out) {
+    if (!out || w <= 0 || h <= 0 || cx <= 0 || cy <= 0) return KT_E_ARG;
+    std::vector<uint8_t> pal((size_t)cx * cy * 3);
+    uint64_t s = seed;
+    for (int j = 0; j < cy; ++j)
+        for (int i = 0; i < cx; ++i) {
+            const uint64_t v = splitmix64(s);
+            const bool dark = ((i + j) & 1) != 0;
+            const int lo = dark ? 16 : 128, span = dark ? 64 : 128;
+            for (int ch = 0; ch < 3; ++ch)
+                pal[((size_t)j * cx + i) * 3 + ch] = (uint8_t)(lo + ((v >> (8 * ch)) & 0xFF) * span / 256);
+        }
+    for (int y = 0; y < h; ++y) {
+        const int j = (int)((int64_t)y * cy / h);
+        for (int x = 0; x < w; ++x) {
+            const int i = (int)((int64_t)x * cx / w);
+            for (int ch = 0; ch < 3; ++ch)
+                out[((size_t)y * w + x) * 3 + ch] = pal[((size_t)j * cx + i) * 3 + ch];
+        }
+    }
+    return KT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,653 @@
+// ============================================================================
+//  kt_trace.hpp -- per-ray tracer state machine (camera -> adaptive steps ->
+//  disk / horizon / escape events -> GeoPixel), shared by the persistent HIP
+//  kernel and the host-compiled check build.
+//
+//  Reference anchors (KerrTraceCpp2 tree):
+//    camera ............ camera.hpp:38-145
+//    BL tracer loop .... main.cpp:343-434 (reject cap 64, max_steps, event arbitration)
+//    KS tracer loop .... main.cpp:1530-1660 (initial covector :1404-1430)
+//    RK4 doubling ...... geodesic.hpp:204-257
+//    DOPRI5 ............ geodesic.hpp:312-414
+//    events ............ geodesic.hpp:49-140, main.cpp:363-431
+//    disk redshift ..... main.cpp:329-341
+// ============================================================================
+#pragma once
+#include "kt_device.hpp"
+
+namespace kt {
+
+enum { METHOD_RK4 = 0, METHOD_DP5 = 1 };
+enum { PREC_FP64 = 0, PREC_MIXED = 1 };
+
+// quirk bits (same values as the KT_QUIRK_* macros of include/kt_hip.h)
+enum : uint32_t {
+    Q_DP5_PHI_FROZEN = 1u << 0,
+    Q_ESC_POSITIONAL = 1u << 1,
+    Q_KS_INIT_NO_SHIFT = 1u << 2,
+    Q_BUNDLE_HORIZON_ESC = 1u << 3,
+    Q_BUNDLE_ROOT_SKIP = 1u << 4,
+    Q_KS_LAMBDA_TO_BL = 1u << 5,
+};
+
+// Per-frame constants, produced by the setup kernel (device memory, 64-bit aligned).
+struct FrameDev {
+    // inputs
+    double M, a, Q, Lambda;
+    double r_obs, th_obs, ph_obs, fov;     // radians
+    double disk_out, off_x, off_y;
+    int32_t W, H, tiles_x, tiles_y;
+    int64_t ray_base;                      // first global ray index (8x8 tile padded)
+    int64_t pix_base;                      // first output pixel
+    // derived
+    double r_h, r_isco, r_in, r_out, r_esc, r_cut;
+    Knds<double> P;
+    int32_t chart, ks_ref_init;            // effective chart; KS covector per reference (Lambda == 0)
+    // camera constants at the observer (camera.hpp:66-142, pixel independent)
+    double gtt, gtp, grr, gth, gpp, sq_rr, sq_th, dphi;
+    double u0, ep, et;
+    int32_t frame_ok, pad0;
+    double U00, U03, U11, U22, U33;
+};
+
+struct RunOpts {
+    int32_t max_steps;
+    uint32_t quirks;
+    double step_init, rtol, atol;
+    double isl_noise;      // mixed: FP32 stage noise coefficient for the island test
+    double isl_r;          // mixed: binary64 below r < isl_r * r_horizon
+};
+
+// result fields of one ray before GeoPixel narrowing (TraceResult, main.cpp:187-191)
+struct RayOut {
+    int32_t fate;          // 0 escaped, 1 disk, 2 horizon
+    double r, g, ph_disk, th_esc, ph_esc;
+};
+
+// ---------------------------------------------------------------------------
+//  camera: pixel -> BL state with p_t re-solved from H = 0   camera.hpp:38-145
+// ---------------------------------------------------------------------------
+KT_HD void camera_ray(const FrameDev& F, double px, double py, double st[7]) {
+    const int span = (F.W > 1) ? (F.W - 1) : 1;
+    const double al = F.fov * (px + F.off_x - 0.5*(F.W-1)) / span;
+    const double be = F.fov * (0.5*(F.H-1) - (py + F.off_y)) / span;
+    double sa, ca, sb, cb;
+    kt_sincos(al, &sa, &ca);
+    kt_sincos(be, &sb, &cb);
+    double ut, ur, uth, uph;
+    if (F.frame_ok) {
+        const double nr = -ca*cb, nt = -sb, np = -sa*cb;
+        ut = F.u0 + np*F.et;
+        ur = nr / F.sq_rr;
+        uth = nt / F.sq_th;
+        uph = np*F.ep;
+    } else {
+        ut = 1.0;
+        ur = -ca*cb / fmax(F.sq_rr, 1e-14);
+        uth = -sb / fmax(F.sq_th, 1e-14);
+        uph = -sa*cb / sqrt(fmax(fabs(F.gpp), 1e-14));
+    }
+    double pt = F.gtt*ut + F.gtp*uph;
+    const double pr = F.grr*ur;
+    const double pth = F.gth*uth;
+    const double pp = F.gtp*ut + F.gpp*uph;
+    const double A = F.U00, B = 2.0*F.U03*pp;
+    const double C = F.U11*pr*pr + F.U22*pth*pth + F.U33*pp*pp;
+    const double disc = B*B - 4.0*A*C;
+    if (disc >= 0.0 && fabs(A) > 1e-15) {
+        const double sq = sqrt(disc);
+        const double p1 = (-B - sq) / (2.0*A), p2 = (-B + sq) / (2.0*A);
+        pt = (p1 < 0.0) ? p1 : p2;
+        if (pt > 0.0) pt = fmin(p1, p2);
+    }
+    st[0] = F.r_obs; st[1] = F.th_obs; st[2] = F.ph_obs;
+    st[3] = pr; st[4] = pth; st[5] = pt; st[6] = pp;
+}
+
+// BL state -> ingoing Kerr-Schild covector (v, r, theta, phi~), same event point.
+//  reference form (Lambda == 0): spatial covector kept, p_T re-solved from H = 0
+//  (main.cpp:1404-1430); p_r(v chart) = p_r(T chart) - p_T.
+//  exact form: p_v = p_t, p_r(v) = p_r - Xi K / Delta_r.
+KT_HD void bl_to_ks_covector(const FrameDev& F, double st[7]) {
+    const Knds<double>& P = F.P;
+    const double r = st[0], th = st[1];
+    double s, c;
+    kt_sincos(th, &s, &c);
+    const double s2 = s*s, c2 = c*c, ra = r*r + P.a2;
+    const double dR = ra*(1.0 - P.lam3*r*r) - P.M2*r + P.Q2;
+    const double dT = 1.0 + P.lam3a2*c2;
+    const double pr = st[3], pth = st[4], pt = st[5], pp = st[6];
+    if (F.ks_ref_init) {
+        const double Pr = pr;
+        const double A = dR - 2.0*P.xi*ra + P.xi2*P.a2*s2/dT;
+        const double B = -2.0*dR*Pr + 2.0*P.xi*Pr*ra - 2.0*P.xi*P.a*pp + 2.0*P.xi2*P.a*pp/dT;
+        const double C = dR*Pr*Pr + 2.0*P.xi*Pr*P.a*pp + dT*pth*pth + P.xi2*pp*pp/(dT*s2);
+        const double disc = B*B - 4.0*A*C;
+        if (disc >= 0.0 && fabs(A) > 1e-15) {
+            const double sq = sqrt(disc);
+            const double p1 = (-B - sq) / (2.0*A), p2 = (-B + sq) / (2.0*A);
+            double pT = (p1 < 0.0) ? p1 : p2;
+            if (pT > 0.0) pT = fmin(p1, p2);
+            st[5] = pT;
+            st[3] = Pr - pT;
+            return;
+        }
+    }
+    const double K = ra*pt + P.a*pp;
+    st[3] = pr - P.xi*K/dR;
+}
+
+// ---------------------------------------------------------------------------
+//  event helpers (geodesic.hpp:49-140)
+// ---------------------------------------------------------------------------
+KT_HD double hermite(double y0, double y1, double d0, double d1, double h, double al) {
+    al = al < 0.0 ? 0.0 : (al > 1.0 ? 1.0 : al);
+    const double a2 = al*al, a3 = a2*al;
+    const double h00 = 2.0*a3 - 3.0*a2 + 1.0;
+    const double h10 = a3 - 2.0*a2 + al;
+    const double h01 = -2.0*a3 + 3.0*a2;
+    const double h11 = a3 - a2;
+    return h00*y0 + h10*(h*d0) + h01*y1 + h11*(h*d1);
+}
+KT_HD bool crosses(double f0, double f1) { return (f0 <= 0.0 && f1 >= 0.0) || (f0 >= 0.0 && f1 <= 0.0); }
+
+// Can the Hermite cubic minus target come within 1e-12 of zero on [0,1]?  Exact
+// bound from the endpoint values and the interior critical points; a cheap
+// pre-test before the 8-bin / 8-bisection search of the reference.
+KT_HD bool hermite_may_touch(double y0, double y1, double d0, double d1, double h, double tgt) {
+    const double f0 = y0 - tgt, f1 = y1 - tgt;
+    if (crosses(f0, f1)) return true;
+    // cubic in alpha: f(al) = c3 al^3 + c2 al^2 + c1 al + c0
+    const double m0 = h*d0, m1 = h*d1;
+    const double c3 = 2.0*f0 + m0 - 2.0*f1 + m1;
+    const double c2 = -3.0*f0 - 2.0*m0 + 3.0*f1 - m1;
+    const double c1 = m0;
+    double lo = fmin(f0, f1), hi = fmax(f0, f1);
+    // roots of 3 c3 al^2 + 2 c2 al + c1
+    const double qa = 3.0*c3, qb = 2.0*c2, qc = c1;
+    double ex[2];
+    int n = 0;
+    if (fabs(qa) > 1e-300) {
+        const double disc = qb*qb - 4.0*qa*qc;
+        if (disc >= 0.0) {
+            const double sq = sqrt(disc);
+            ex[n++] = (-qb - sq) / (2.0*qa);
+            ex[n++] = (-qb + sq) / (2.0*qa);
+        }
+    } else if (fabs(qb) > 1e-300) {
+        ex[n++] = -qc / qb;
+    }
+    for (int i = 0; i < n; ++i) {
+        const double al = ex[i];
+        if (al > 0.0 && al < 1.0) {
+            const double v = ((c3*al + c2)*al + c1)*al + f0;
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+    }
+    // generous margin for the rounding of the monomial form
+    const double margin = 1e-12 + 1e-12*(fabs(f0) + fabs(f1) + fabs(m0) + fabs(m1));
+    return lo <= margin && hi >= -margin;
+}
+
+// first root of the Hermite cubic on [0,1]: 8 bins, 8 bisections, |f| <= 1e-12 short cut
+KT_HD bool hermite_first_root(double y0, double y1, double d0, double d1, double h, double tgt, double& out) {
+    const double feps = 1e-12;
+    double a0 = 0.0, f0 = y0 - tgt;
+    if (fabs(f0) <= feps) { out = 0.0; return true; }
+    for (int i = 1; i <= 8; ++i) {
+        const double a1 = double(i) / 8.0;
+        const double f1 = (i == 8) ? (y1 - tgt) : (hermite(y0, y1, d0, d1, h, a1) - tgt);
+        if (fabs(f1) <= feps) { out = a1; return true; }
+        if (crosses(f0, f1)) {
+            double lo = a0, hi = a1;
+            for (int k = 0; k < 8; ++k) {
+                const double mid = 0.5*(lo + hi);
+                const double fm = hermite(y0, y1, d0, d1, h, mid) - tgt;
+                if (crosses(f0, fm)) hi = mid; else lo = mid;
+            }
+            out = 0.5*(lo + hi);
+            return true;
+        }
+        a0 = a1;
+        f0 = f1;
+    }
+    return false;
+}
+
+// Keplerian angular velocity, reference sign convention   knds_metric.hpp:354-363
+KT_HD double omega_k(const FrameDev& F, double r) {
+    const double s = (F.a < 0.0) ? 1.0 : -1.0;
+    const double Me = F.M - F.Q*F.Q/(2.0*r) + F.Lambda*F.a*r*r/3.0;
+    const double sq = sqrt(fmax(Me, 0.0));
+    const double den = r*sqrt(r) + s*F.a*sq;
+    if (fabs(den) < 1e-14) return 0.0;
+    return s * sq / den;
+}
+
+// g_tt, g_tphi, g_phiphi of BL at theta = pi/2 (knds_metric.hpp:71-88 with the
+// reference's cos(M_PI/2), sin(M_PI/2))
+KT_HD void lower_equator(const FrameDev& F, double r, double& gtt, double& gtp, double& gpp) {
+    const double a = F.a, ct = kCosHalfPi;
+    const double sg = r*r + a*a*ct*ct;
+    const double dr = (r*r + a*a) * (1.0 - F.Lambda*r*r/3.0) - 2.0*F.M*r + F.Q*F.Q;
+    const double dt = 1.0 + F.Lambda*a*a*ct*ct/3.0;
+    const double xi = 1.0 + F.Lambda*a*a/3.0;
+    const double x2 = xi*xi, s2 = 1.0, ra = r*r + a*a, den = sg*x2;
+    gtt = (-dr + dt*a*a*s2) / den;
+    gtp = a*s2*(dr - dt*ra) / den;
+    gpp = s2*(dt*ra*ra - dr*a*a*s2) / den;
+}
+
+// redshift of the thin disk at r (main.cpp:329-341), clamped to [0,20] (main.cpp:390)
+KT_HD double disk_redshift(const FrameDev& F, double r, double pt, double pp) {
+    const double Om = omega_k(F, r);
+    const double b = -pp / (-pt);
+    double gtt, gtp, gpp;
+    lower_equator(F, r, gtt, gtp, gpp);
+    const double d2 = -(gtt + 2.0*gtp*Om + gpp*Om*Om);
+    double red;
+    if (d2 <= 0.0) red = 1.0;
+    else {
+        const double dv = fabs(1.0 - Om*b);
+        if (dv < 1e-8) red = 20.0;
+        else {
+            red = sqrt(d2) / dv;
+            if (!isfinite(red)) red = 1.0;
+        }
+    }
+    return red < 0.0 ? 0.0 : (red > 20.0 ? 20.0 : red);
+}
+
+// ---------------------------------------------------------------------------
+//  DOPRI5 tableau (Dormand & Prince 1980; geodesic.hpp:261-274)
+// ---------------------------------------------------------------------------
+struct DP5 {
+    static constexpr double a21 = 1.0/5.0;
+    static constexpr double a31 = 3.0/40.0, a32 = 9.0/40.0;
+    static constexpr double a41 = 44.0/45.0, a42 = -56.0/15.0, a43 = 32.0/9.0;
+    static constexpr double a51 = 19372.0/6561.0, a52 = -25360.0/2187.0, a53 = 64448.0/6561.0,
+                            a54 = -212.0/729.0;
+    static constexpr double a61 = 9017.0/3168.0, a62 = -355.0/33.0, a63 = 46732.0/5247.0,
+                            a64 = 49.0/176.0, a65 = -5103.0/18656.0;
+    static constexpr double b1 = 35.0/384.0, b3 = 500.0/1113.0, b4 = 125.0/192.0,
+                            b5 = -2187.0/6784.0, b6 = 11.0/84.0;
+    // b5 - b4 (error weights; b4 of geodesic.hpp:337-340)
+    static constexpr double e1 = 35.0/384.0 - 5179.0/57600.0;
+    static constexpr double e3 = 500.0/1113.0 - 7571.0/16695.0;
+    static constexpr double e4 = 125.0/192.0 - 393.0/640.0;
+    static constexpr double e5 = -2187.0/6784.0 + 92097.0/339200.0;
+    static constexpr double e6 = 11.0/84.0 - 187.0/2100.0;
+    static constexpr double e7 = -1.0/40.0;
+};
+
+// ---------------------------------------------------------------------------
+//  one ray
+// ---------------------------------------------------------------------------
+struct RayState {
+    double y[4];           // r, theta, p_r, p_theta
+    double ph, pt, pp;     // phi (phi~ in KS), conserved p_t (p_v), p_phi
+    double h;
+    double k1[5];          // RHS at y (FSAL / first stage)
+    SinCos<double> sc;     // sin, cos of y[1]
+    int32_t n_acc, n_rej;
+    int32_t rhs32, rhs64, hermites, rej_total;
+};
+
+// Wave-level vote policy: the kernel passes one backed by __ballot, the host check
+// build evaluates lanes one at a time.
+struct LaneVote {
+    KT_HD static bool any(bool b) { return b; }
+};
+
+template <int CHART, class Vote>
+KT_HD void stage_f64(const Knds<double>& P, const double y0[4], const double d[4], const SinCos<double>& sc0,
+                     double pt, double pp, double k[5], SinCos<double>& sc) {
+    const double th = y0[1] + d[1];
+    if (Vote::any(!(fabs(d[1]) <= kRotateMax))) kt_sincos(th, &sc.s, &sc.c);
+    else sc = rotate(sc0, d[1]);
+    rhs<CHART>(P, y0[0] + d[0], sc, y0[2] + d[2], y0[3] + d[3], pt, pp, k);
+}
+
+template <int CHART, class Vote>
+KT_HD void stage_f32(const Knds<float>& Pf, const double y0[4], const double d[4], const SinCos<double>& sc0,
+                     double pt, double pp, double k[5]) {
+    SinCos<float> scf;
+    const float d1 = (float)d[1];
+    if (Vote::any(!(fabs(d[1]) <= kRotateMax))) kt_sincos((float)(y0[1] + d[1]), &scf.s, &scf.c);
+    else scf = rotate(SinCos<float>{(float)sc0.s, (float)sc0.c}, d1);
+    float kf[5];
+    rhs<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
+               (float)pt, (float)pp, kf);
+    for (int i = 0; i < 5; ++i) k[i] = (double)kf[i];
+}
+
+template <int CHART, int PREC, class Vote>
+KT_HD void stage(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4], const double d[4],
+                 const SinCos<double>& sc0, double pt, double pp, double k[5], RayState& S) {
+    if (PREC == PREC_FP64 || use64) {
+        SinCos<double> sc;
+        stage_f64<CHART, Vote>(P, y0, d, sc0, pt, pp, k, sc);
+        S.rhs64++;
+    } else {
+        stage_f32<CHART, Vote>(Pf, y0, d, sc0, pt, pp, k);
+        S.rhs32++;
+    }
+}
+
+// step-size factor err^(-q) in binary32 (the controller needs ~3 digits)
+KT_HD double pow_neg(double err, float q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float e = (float)err;
+    return (double)__builtin_amdgcn_exp2f(-q * __builtin_amdgcn_logf(e));
+#else
+    return (double)std::exp2(-q * std::log2((float)err));
+#endif
+}
+
+// DOPRI5 attempt.  Returns true if accepted; on acceptance y_prev/k_prev keep
+// the start of the step (for events) and S.k1 holds f(y_new) (FSAL).
+template <int CHART, int PREC, class Vote>
+KT_HD bool dp5_attempt(const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64, bool phi,
+                       RayState& S, double y_prev[4], double& ph_prev, double k_prev[5], double& h_used) {
+    typedef DP5 C;
+    const double h = S.h;
+    h_used = h;
+    const double* y = S.y;
+    double k2[5], k3[5], k4[5], k5[5], k6[5], k7[5];
+    const double* k1 = S.k1;
+    double d[4];
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
+    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2, S);
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
+    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3, S);
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
+    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4, S);
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
+    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5, S);
+    for (int i = 0; i < 4; ++i)
+        d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
+    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6, S);
+    double d5[4];
+    for (int i = 0; i < 4; ++i)
+        d5[i] = h*(C::b1*k1[i] + C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i] + C::b6*k6[i]);
+    // FSAL stage at y5: always in the stage precision; its sin/cos becomes the next base
+    SinCos<double> sc7;
+    if (PREC == PREC_FP64 || use64) {
+        stage_f64<CHART, Vote>(P, y, d5, S.sc, S.pt, S.pp, k7, sc7);
+        S.rhs64++;
+    } else {
+        stage_f32<CHART, Vote>(Pf, y, d5, S.sc, S.pt, S.pp, k7);
+        S.rhs32++;
+        if (Vote::any(!(fabs(d5[1]) <= kRotateMax))) kt_sincos(y[1] + d5[1], &sc7.s, &sc7.c);
+        else sc7 = rotate(S.sc, d5[1]);
+    }
+    double e2 = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        const double ev = h*(C::e1*k1[i] + C::e3*k3[i] + C::e4*k4[i] + C::e5*k5[i] + C::e6*k6[i] + C::e7*k7[i]);
+        const double q = ev / (O.atol + O.rtol*fabs(y[i]));
+        e2 += q*q;
+    }
+    const double err = sqrt(e2 * 0.25);
+    if (!isfinite(err)) {
+        double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
+        S.h = hn < 1e-10 ? 1e-10 : hn;
+        return false;
+    }
+    if (err <= 1.0 || h < 1e-10) {
+        for (int i = 0; i < 4; ++i) { y_prev[i] = S.y[i]; k_prev[i] = S.k1[i]; }
+        k_prev[4] = S.k1[4];
+        ph_prev = S.ph;
+        if (phi) S.ph += h*(C::b1*k1[4] + C::b3*k3[4] + C::b4*k4[4] + C::b5*k5[4] + C::b6*k6[4]);
+        for (int i = 0; i < 4; ++i) S.y[i] += d5[i];
+        for (int i = 0; i < 5; ++i) S.k1[i] = k7[i];
+        S.sc = sc7;
+        double hn = h * ((err > 1e-12) ? 0.9*pow_neg(err, 0.2f) : 4.0);
+        if (!isfinite(hn)) hn = h;
+        S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
+        return true;
+    }
+    double hn = h * 0.9 * pow_neg(err, 0.25f);
+    if (!isfinite(hn)) hn = h*0.5;
+    S.h = hn < 1e-10 ? 1e-10 : hn;
+    return false;
+}
+
+// Classic RK4 over [0, h] from y0 with f(y0) = k0; returns the increment (5 comps,
+// phi last) and, through scy, the sin/cos at the end point.
+template <int CHART, int PREC, class Vote>
+KT_HD void rk4_incr(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
+                    const SinCos<double>& sc0, const double k0[5], double h, double pt, double pp,
+                    double inc[5], RayState& S) {
+    double k2[5], k3[5], k4[5], d[4];
+    for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k0[i];
+    stage<CHART, PREC, Vote>(P, Pf, use64, y0, d, sc0, pt, pp, k2, S);
+    for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k2[i];
+    stage<CHART, PREC, Vote>(P, Pf, use64, y0, d, sc0, pt, pp, k3, S);
+    for (int i = 0; i < 4; ++i) d[i] = h*k3[i];
+    stage<CHART, PREC, Vote>(P, Pf, use64, y0, d, sc0, pt, pp, k4, S);
+    for (int i = 0; i < 5; ++i) inc[i] = h/6.0*(k0[i] + 2.0*k2[i] + 2.0*k3[i] + k4[i]);
+}
+
+template <int CHART, int PREC, class Vote>
+KT_HD SinCos<double> sc_at(const SinCos<double>& sc0, double th0, double dth) {
+    SinCos<double> o;
+    if (Vote::any(!(fabs(dth) <= kRotateMax))) kt_sincos(th0 + dth, &o.s, &o.c);
+    else o = rotate(sc0, dth);
+    return o;
+}
+
+// RK4 step doubling attempt (geodesic.hpp:204-257).  On acceptance S.k1 is
+// refreshed to f(y_new) (needed by the events and the next attempt).
+template <int CHART, int PREC, class Vote>
+KT_HD bool rk4_attempt(const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
+                       RayState& S, double y_prev[4], double& ph_prev, double k_prev[5], double& h_used) {
+    const double h = S.h;
+    h_used = h;
+    const double tol = O.rtol;
+    double incA[5], incB1[5], incB2[5];
+    rk4_incr<CHART, PREC, Vote>(P, Pf, use64, S.y, S.sc, S.k1, h, S.pt, S.pp, incA, S);
+    rk4_incr<CHART, PREC, Vote>(P, Pf, use64, S.y, S.sc, S.k1, 0.5*h, S.pt, S.pp, incB1, S);
+    double yB1[4];
+    for (int i = 0; i < 4; ++i) yB1[i] = S.y[i] + incB1[i];
+    const SinCos<double> scB1 = sc_at<CHART, PREC, Vote>(S.sc, S.y[1], incB1[1]);
+    double kB1[5];
+    {
+        const double z[4] = {0.0, 0.0, 0.0, 0.0};
+        stage<CHART, PREC, Vote>(P, Pf, use64, yB1, z, scB1, S.pt, S.pp, kB1, S);
+    }
+    rk4_incr<CHART, PREC, Vote>(P, Pf, use64, yB1, scB1, kB1, 0.5*h, S.pt, S.pp, incB2, S);
+    double yA[4], yB[4];
+    for (int i = 0; i < 4; ++i) { yA[i] = S.y[i] + incA[i]; yB[i] = yB1[i] + incB2[i]; }
+    double e2 = 0.0;
+    for (int i = 0; i < 4; ++i) { const double t = yA[i] - yB[i]; e2 += t*t; }
+    const double err = sqrt(e2) / 15.0;
+    if (!isfinite(err)) {
+        double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
+        S.h = hn < 1e-10 ? 1e-10 : hn;
+        return false;
+    }
+    if (err < tol || h < 1e-10) {
+        for (int i = 0; i < 4; ++i) { y_prev[i] = S.y[i]; k_prev[i] = S.k1[i]; }
+        k_prev[4] = S.k1[4];
+        ph_prev = S.ph;
+        S.ph = S.ph + incB1[4] + incB2[4];
+        const double dth = (yB1[1] - S.y[1]) + incB2[1];
+        for (int i = 0; i < 4; ++i) S.y[i] = yB[i];
+        S.sc = sc_at<CHART, PREC, Vote>(S.sc, y_prev[1], dth);
+        double hn = h * ((err > 1e-14) ? 0.9*pow_neg(err / tol, 0.2f) : 4.0);
+        if (!isfinite(hn)) hn = h;
+        S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
+        // f(y_new), always binary64 in FP64 mode; stage precision otherwise
+        const double z[4] = {0.0, 0.0, 0.0, 0.0};
+        stage<CHART, PREC, Vote>(P, Pf, use64, S.y, z, S.sc, S.pt, S.pp, S.k1, S);
+        return true;
+    }
+    const double half = h*0.5;
+    double hn = h * 0.9 * pow_neg(err / tol, 0.25f);
+    if (!isfinite(hn)) hn = half;
+    if (hn > half) hn = half;
+    S.h = hn < 1e-10 ? 1e-10 : hn;
+    return false;
+}
+
+// (p_r, p_theta) -> kappa (p_r, p_theta) with H = 0 exactly (binary64); used by
+// the mixed-precision path after accepted steps.
+template <int CHART>
+KT_HD void renormalise(const Knds<double>& P, RayState& S) {
+    const double r = S.y[0], s2 = S.sc.s*S.sc.s, c2 = S.sc.c*S.sc.c;
+    const double ra = r*r + P.a2;
+    const double dR = ra*(1.0 - P.lam3*r*r) - P.M2*r + P.Q2;
+    const double dT = 1.0 + P.lam3a2*c2;
+    const double K = ra*S.pt + P.a*S.pp;
+    const double L = P.a*s2*S.pt + S.pp;
+    const double A = dR*S.y[2]*S.y[2] + dT*S.y[3]*S.y[3];
+    double kap = 1.0;
+    if (CHART == CHART_BL) {
+        const double C = -P.xi2*K*K/dR + P.xi2*L*L/(dT*s2);
+        if (A > 0.0 && -C > 0.0) kap = sqrt(-C / A);
+    } else {
+        const double B = 2.0*P.xi*S.y[2]*K;
+        const double C = P.xi2*L*L/(dT*s2);
+        const double disc = B*B - 4.0*A*C;
+        if (fabs(A) > 1e-300 && disc >= 0.0) {
+            const double sq = sqrt(disc);
+            const double k1 = (-B + sq) / (2.0*A), k2 = (-B - sq) / (2.0*A);
+            kap = (fabs(k1 - 1.0) < fabs(k2 - 1.0)) ? k1 : k2;
+        }
+    }
+    if (isfinite(kap) && fabs(kap - 1.0) < 1e-4) { S.y[2] *= kap; S.y[3] *= kap; }
+}
+
+// Event test after an accepted step (main.cpp:363-431).  Returns true if the ray
+// is finished (o filled).
+// The reference's KS tracer reports azimuths through atan2 (KS_to_BL_spatial,
+// knds_metric.hpp:389-411), i.e. in (-pi, pi]; BL reports the integrated phi.
+template <int CHART> KT_HD double out_phi(double ph) {
+    if (CHART == CHART_KS) { double s, c; kt_sincos(ph, &s, &c); return atan2(s, c); }
+    return ph;
+}
+
+template <int CHART>
+KT_HD bool step_events(const FrameDev& F, const RayState& S, const double yp[4], double php,
+                       const double kp[5], double hu, uint32_t quirks, RayOut& o, int& hermites) {
+    const double halfpi = kPi / 2.0;
+    double best = 2.0;
+    int ev = 0;
+    double d_r = 0.0, d_g = 1.0, d_ph = 0.0;
+    const double q0 = yp[1] - halfpi, q1 = S.y[1] - halfpi;
+    const bool near_eq = crosses(q0, q1) || (fmin(fabs(q0), fabs(q1)) < 0.35);
+    if (near_eq && hermite_may_touch(yp[1], S.y[1], kp[1], S.k1[1], hu, halfpi)) {
+        hermites++;
+        double al = 0.0;
+        if (hermite_first_root(yp[1], S.y[1], kp[1], S.k1[1], hu, halfpi, al)) {
+            const double rh = hermite(yp[0], S.y[0], kp[0], S.k1[0], hu, al);
+            if (rh >= F.r_in && rh <= F.r_out) {
+                d_r = rh;
+                d_g = disk_redshift(F, rh, S.pt, S.pp);
+                d_ph = out_phi<CHART>(php + al*(S.ph - php));
+                best = al;
+                ev = 1;
+            }
+        }
+    }
+    const double r = S.y[0];
+    if (((yp[0] > F.r_cut) && (r <= F.r_cut)) || (r <= F.r_cut)) {
+        const double dn = yp[0] - r;
+        double ah = (fabs(dn) > 1e-12) ? ((yp[0] - F.r_cut) / dn) : 0.0;
+        ah = ah < 0.0 ? 0.0 : (ah > 1.0 ? 1.0 : ah);
+        if (ah < best) { best = ah; ev = 2; }
+    }
+    if (((yp[0] < F.r_esc) && (r >= F.r_esc)) || (r >= F.r_esc)) {
+        const double dn = r - yp[0];
+        double ae = (fabs(dn) > 1e-12) ? ((F.r_esc - yp[0]) / dn) : 1.0;
+        ae = ae < 0.0 ? 0.0 : (ae > 1.0 ? 1.0 : ae);
+        if (ae < best) { best = ae; ev = 3; }
+    }
+    if (ev == 0) return false;
+    o.ph_disk = o.th_esc = o.ph_esc = 0.0;
+    if (ev == 1) { o.fate = 1; o.r = d_r; o.g = d_g; o.ph_disk = d_ph; }
+    else if (ev == 2) { o.fate = 2; o.r = yp[0] + best*(r - yp[0]); o.g = 0.0; }
+    else {
+        o.fate = 0; o.r = F.r_esc; o.g = 1.0;
+        const double te = yp[1] + best*(S.y[1] - yp[1]);
+        const double pe = out_phi<CHART>(php + best*(S.ph - php));
+        if (quirks & Q_ESC_POSITIONAL) { o.ph_disk = te; o.th_esc = pe; o.ph_esc = 0.0; }
+        else { o.th_esc = te; o.ph_esc = pe; }
+    }
+    return true;
+}
+
+// result of a ray stopped by the reject cap or max_steps (main.cpp:360,433)
+template <int CHART>
+KT_HD void stopped_result(const RayState& S, uint32_t quirks, RayOut& o) {
+    o.fate = 0; o.r = S.y[0]; o.g = 1.0;
+    const double ph = out_phi<CHART>(S.ph);
+    if (quirks & Q_ESC_POSITIONAL) { o.ph_disk = S.y[1]; o.th_esc = ph; o.ph_esc = 0.0; }
+    else { o.ph_disk = 0.0; o.th_esc = S.y[1]; o.ph_esc = ph; }
+}
+
+// Initialise a ray at pixel (px, py) of frame F.
+template <int CHART, int PREC>
+KT_HD void ray_init(const FrameDev& F, const RunOpts& O, int px, int py, RayState& S) {
+    double st[7];
+    camera_ray(F, (double)px, (double)py, st);
+    if (CHART == CHART_KS) bl_to_ks_covector(F, st);
+    S.y[0] = st[0]; S.y[1] = st[1]; S.y[2] = st[3]; S.y[3] = st[4];
+    S.ph = st[2]; S.pt = st[5]; S.pp = st[6];
+    S.h = O.step_init < 1e-10 ? 1e-10 : O.step_init;
+    kt_sincos(S.y[1], &S.sc.s, &S.sc.c);
+    rhs<CHART>(F.P, S.y[0], S.sc, S.y[2], S.y[3], S.pt, S.pp, S.k1);
+    S.n_acc = 0; S.n_rej = 0;
+    S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0;
+}
+
+// Does this lane need a binary64 step in the mixed-precision path?  The FP32
+// stage noise of the embedded error estimate, ~isl_noise * h |k|, must stay
+// well below the error scale atol + rtol |y|; near the horizon and the polar
+// axis the FP32 Hamiltonian loses too many digits.
+KT_HD bool wants_fp64(const FrameDev& F, const RunOpts& O, const RayState& S) {
+    if (S.y[0] < O.isl_r * F.r_h) return true;
+    if (S.sc.s*S.sc.s < 1e-3) return true;
+    for (int i = 0; i < 4; ++i) {
+        const double sc = O.atol + O.rtol*fabs(S.y[i]);
+        if (O.isl_noise * S.h * fabs(S.k1[i]) > 0.25*sc) return true;
+    }
+    return false;
+}
+
+// Pack a result into the 28-byte GeoPixel (main.cpp:2130-2147).
+KT_HD void pack_geopixel(const RayOut& o, uint32_t w[7]) {
+    union { float f; uint32_t u; } cv;
+    w[0] = (uint32_t)(o.fate & 0xFF);
+    cv.f = (float)o.r; w[1] = cv.u;
+    cv.f = (float)o.g; w[2] = cv.u;
+    cv.f = 1.0f; w[3] = cv.u;
+    cv.f = (float)o.ph_disk; w[4] = cv.u;
+    cv.f = (float)o.th_esc; w[5] = cv.u;
+    cv.f = (float)o.ph_esc; w[6] = cv.u;
+}
+
+// One attempt + bookkeeping for an active ray; returns true when the ray is done.
+template <int CHART, int METHOD, int PREC, class Vote>
+KT_HD bool ray_advance(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O,
+                       bool use64, RayState& S, RayOut& o) {
+    double yp[4], kp[5], php, hu;
+    const bool phi = !(METHOD == METHOD_DP5 && (O.quirks & Q_DP5_PHI_FROZEN));
+    bool ok;
+    if (METHOD == METHOD_DP5) ok = dp5_attempt<CHART, PREC, Vote>(P, Pf, O, use64, phi, S, yp, php, kp, hu);
+    else ok = rk4_attempt<CHART, PREC, Vote>(P, Pf, O, use64, S, yp, php, kp, hu);
+    if (!ok) {
+        S.rej_total++;
+        if (!isfinite(S.h) || ++S.n_rej > 64) { stopped_result<CHART>(S, O.quirks, o); return true; }
+        return false;
+    }
+    S.n_rej = 0;
+    S.n_acc++;
+    if (PREC == PREC_MIXED) renormalise<CHART>(P, S);
+    if (step_events<CHART>(F, S, yp, php, kp, hu, O.quirks, o, S.hermites)) return true;
+    if (S.n_acc >= O.max_steps) { stopped_result<CHART>(S, O.quirks, o); return true; }
+    return false;
+}
+
+}  // namespace kt
