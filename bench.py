@@ -1,0 +1,165 @@
+"""Benchmark: rays/s of the KNdS geodesic hot path on one MI355X.
+
+Workload: Kerr-Newman-de Sitter a=0.9, Q=0.3, Lambda=1e-5,
+r_obs=40 M, theta_obs=85 deg, fov 30 deg (CLI default), disk r_isco..12 M, 1280x720,
+ingoing Kerr-Schild chart, DOPRI5 rtol=1e-9 atol=1e-12.  A "step" is one full render of
+the frame (device setup + persistent trace + shading) with its parameters already on the
+device and the outputs (GeoPixel + RGBA) left in HBM.  `value` is the mixed-precision
+rays/s; the FP64 path is timed the same way and reported beside it.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rays/sec @1280x720 a=0.9 KS DOPRI5: mixed vs FP64 GPU at max|Δg|≤1e-4"
+W, H = 1280, 720
+FRAME = dict(a=0.9, Q=0.3, Lambda=1e-5, theta=85.0, r_obs=40.0, fov=30.0, disk_out=12.0)
+RTOL, ATOL = 1e-9, 1e-12
+
+# Algorithmic floating-point work per unit (add/sub/mul 1, fma 2, div/sqrt/rcp 1):
+#   right-hand side, counted by tests/hostcheck/flopcount.cpp on the kernel source,
+#   incremental sin/cos rotation per stage (binary64 / binary32 form),
+#   DOPRI5 stage combination + embedded error + update per attempt,
+#   Hermite 8-bin/8-bisection disk search.
+F_RHS = {"bl": 98, "ks": 93}
+F_ROT = {64: 39, 32: 31}
+F_DP5_COMB = 256
+F_HERMITE = 370
+PEAK_VALU_TFLOPS = 78.6   # MI355X vector FP64; FP32 without packed math issues at the same rate
+
+
+def flops_of(st, chart="ks"):
+    attempts = st["steps_accepted"] + st["steps_rejected"]
+    rays = st["rays"]
+    stage64 = max(st["rhs_fp64"] - rays, 0)          # the first RHS of a ray uses a full sincos
+    stage32 = st["rhs_fp32"]
+    return (st["rhs_fp64"] * F_RHS[chart] + st["rhs_fp32"] * F_RHS[chart] + stage64 * F_ROT[64]
+            + stage32 * F_ROT[32] + attempts * F_DP5_COMB + st["events"] * F_HERMITE)
+
+
+def gpu_run(ctx, kt, prec, steps, warmup):
+    fr = kt.make_frame(W, H, **FRAME)
+    o = kt.default_opts()
+    o.chart, o.integrator, o.precision = kt.KT_CHART_KS, kt.KT_INTEGRATOR_DOPRI5, prec
+    o.rtol, o.atol = RTOL, ATOL
+    for _ in range(warmup):
+        ctx.render_device([fr], o)
+    t0 = time.perf_counter()
+    stats = [ctx.render_device([fr], o) for _ in range(steps)]
+    wall = time.perf_counter() - t0
+    out = ctx.render([fr], o, want_rgba=False, want_geo=True)
+    return wall, stats, out["geo"], out["info"][0]
+
+
+def accuracy(geo, ref, px, py):
+    from tests._util import fate_boundary_band
+    band = fate_boundary_band(geo["outcome"], W, H).reshape(H, W)[py, px]
+    sub = geo.reshape(H, W)[py, px]
+    keep = ~band
+    d = keep & (sub["outcome"] == 1) & (ref["outcome"] == 1)
+    dg = np.abs(sub["redshift"][d] - ref["redshift"][d])
+    dr = np.abs(sub["r"][d] - ref["r"][d])
+    return {"pixels_compared": int(px.size), "fate_agreement_off_band": float((sub["outcome"] == ref["outcome"])[keep].mean()),
+            "max_abs_dg_off_band": float(dg.max()) if d.any() else 0.0,
+            "max_abs_dr_off_band": float(dr.max()) if d.any() else 0.0,
+            "fate_agreement_all": float((sub["outcome"] == ref["outcome"]).mean())}
+
+
+def cpu_baseline(budget_s):
+    """The oracle (FP64 CPU restatement, OpenMP over pixels) on this host's cores, on whole rows
+    of the same frame taken in a strided order until the time budget is used."""
+    from oracle import kto
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    fo = kto.make_frame(W, H, chart="ks", integrator="dopri5", tol=RTOL, **FRAME)
+    order = np.concatenate([np.arange(s, H, 16) for s in range(16)])
+    done_x, done_y = [], []
+    t0 = time.perf_counter()
+    outs = []
+    for k in range(0, H, 8):
+        rows = order[k:k + 8]
+        ys, xs = np.meshgrid(rows, np.arange(W), indexing="ij")
+        px, py = xs.ravel().astype(np.int32), ys.ravel().astype(np.int32)
+        outs.append(kto.trace_pixels(fo, px, py, n_threads=threads))
+        done_x.append(px)
+        done_y.append(py)
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    el = time.perf_counter() - t0
+    px, py = np.concatenate(done_x), np.concatenate(done_y)
+    return {"value": px.size / el, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{px.size} rays = {px.size // W} strided rows of the 1280x720 frame, oracle FP64 "
+                      f"(reference DOPRI5 semantics, tol 1e-9, BL fallback as the reference runs it), {el:.1f} s"}, \
+        np.concatenate(outs), px, py
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
+              "(rays and frames shard across GPUs without exchange; see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+    import kerrtracecpp2_amd as kt
+    ctx = kt.Context(0)
+    wall_m, st_m, geo_m, info = gpu_run(ctx, kt, kt.KT_PRECISION_MIXED, a.steps, a.warmup)
+    wall_d, st_d, geo_d, _ = gpu_run(ctx, kt, kt.KT_PRECISION_FP64, a.steps, a.warmup)
+    ms_m = 1e3 * wall_m / a.steps
+    ms_d = 1e3 * wall_d / a.steps
+    rays = W * H
+    trace_ms_m = float(np.mean([s["trace_ms"] for s in st_m]))
+    trace_ms_d = float(np.mean([s["trace_ms"] for s in st_d]))
+    fl_m = flops_of(st_m[-1])
+    fl_d = flops_of(st_d[-1])
+    achieved = fl_m / (trace_ms_m * 1e-3) / 1e12
+    result = {
+        "metric": METRIC,
+        "value": rays / (ms_m * 1e-3),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_m,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (deterministic camera pixel grid; no input data)",
+        "config": {"workload": "KNdS a=0.9 Q=0.3 Lambda=1e-5 1280x720 r_obs=40 theta=85 fov=30 disk<=12, "
+                               "KS chart, DOPRI5 rtol=1e-9 atol=1e-12, mixed precision",
+                   "rays_per_step": rays, "chart": "ks", "integrator": "dopri5"},
+        "fp64": {"rays_per_s": rays / (ms_d * 1e-3), "ms_per_step": ms_d, "trace_kernel_ms": trace_ms_d,
+                 "steps_accepted": st_d[-1]["steps_accepted"], "steps_rejected": st_d[-1]["steps_rejected"],
+                 "achieved_tflops": fl_d / (trace_ms_d * 1e-3) / 1e12},
+        "mixed_vs_fp64": ms_d / ms_m,
+        "mixed": {"trace_kernel_ms": trace_ms_m, "steps_accepted": st_m[-1]["steps_accepted"],
+                  "steps_rejected": st_m[-1]["steps_rejected"], "rhs_fp32": st_m[-1]["rhs_fp32"],
+                  "rhs_fp64": st_m[-1]["rhs_fp64"]},
+        "frame": {"r_horizon": info["r_horizon"], "r_isco": info["r_isco"]},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": None,
+                     "kernel": "kt_trace_kernel<KS,DOPRI5,MIXED>", "flops_per_launch": fl_m},
+    }
+    if not a.no_cpu:
+        cpu, ref, px, py = cpu_baseline(a.cpu_budget)
+        result["cpu_baseline"] = cpu
+        result["accuracy_vs_oracle"] = {"mixed": accuracy(geo_m, ref, px, py), "fp64": accuracy(geo_d, ref, px, py)}
+    ctx.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

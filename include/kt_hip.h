@@ -85,6 +85,10 @@ typedef struct kt_opts {
     double exposure, gamma, temp_scale;   /* ColorParams (main.cpp:72-81)                   */
     int32_t tuning_waves_per_cu;          /* 0: autotuned / default                        */
     int32_t tuning_block;                 /* 0: default (256)                              */
+    double mixed_noise;     /* MIXED: binary32 stage-noise coefficient of the island test;
+                               0 selects the built-in default                               */
+    double mixed_r_island;  /* MIXED: binary64 steps below r < mixed_r_island * r_horizon;
+                               0 selects the built-in default                               */
 } kt_opts;
 
 /* Background: RGB8 equirectangular image (BackgroundImage, main.cpp:140-171). */

@@ -79,24 +79,22 @@ template <class T> struct SinCos { T s, c; };
 
 KT_HD SinCos<double> rotate(const SinCos<double>& b, double d) {
     const double d2 = d*d;
-    // sin d = d (1 - d2/(2.3) (1 - d2/(4.5) (1 - ... (1 - d2/(14.15)))))
-    double ps = fma(d2, -1.0/210.0, 1.0);
-    ps = fma(d2 * (-1.0/156.0), ps, 1.0);
-    ps = fma(d2 * (-1.0/110.0), ps, 1.0);
-    ps = fma(d2 * (-1.0/72.0), ps, 1.0);
-    ps = fma(d2 * (-1.0/42.0), ps, 1.0);
-    ps = fma(d2 * (-1.0/20.0), ps, 1.0);
-    ps = fma(d2 * (-1.0/6.0), ps, 1.0);
-    const double sd = d * ps;
-    // 1 - cos d = d2/2 (1 - d2/(3.4) (1 - ... (1 - d2/(15.16))))
-    double pc = fma(d2, -1.0/240.0, 1.0);
-    pc = fma(d2 * (-1.0/182.0), pc, 1.0);
-    pc = fma(d2 * (-1.0/132.0), pc, 1.0);
-    pc = fma(d2 * (-1.0/90.0), pc, 1.0);
-    pc = fma(d2 * (-1.0/56.0), pc, 1.0);
-    pc = fma(d2 * (-1.0/30.0), pc, 1.0);
-    pc = fma(d2 * (-1.0/12.0), pc, 1.0);
-    const double omc = 0.5 * d2 * pc;                 // 1 - cos d, no cancellation
+    // sin d = d + d^3 P(d2) (Taylor to d^15), 1 - cos d = d2 Q(d2) (Taylor to d^16)
+    double ps = fma(d2, -1.0/1307674368000.0, 1.0/6227020800.0);
+    ps = fma(d2, ps, -1.0/39916800.0);
+    ps = fma(d2, ps, 1.0/362880.0);
+    ps = fma(d2, ps, -1.0/5040.0);
+    ps = fma(d2, ps, 1.0/120.0);
+    ps = fma(d2, ps, -1.0/6.0);
+    const double sd = fma(d*d2, ps, d);
+    double pc = fma(d2, -1.0/20922789888000.0, 1.0/87178291200.0);
+    pc = fma(d2, pc, -1.0/479001600.0);
+    pc = fma(d2, pc, 1.0/3628800.0);
+    pc = fma(d2, pc, -1.0/40320.0);
+    pc = fma(d2, pc, 1.0/720.0);
+    pc = fma(d2, pc, -1.0/24.0);
+    pc = fma(d2, pc, 0.5);
+    const double omc = d2*pc;                          // 1 - cos d, no cancellation
     SinCos<double> o;
     o.s = fma(b.c, sd, fma(-b.s, omc, b.s));           // s cos d + c sin d
     o.c = fma(-b.s, sd, fma(-b.c, omc, b.c));          // c cos d - s sin d
@@ -104,18 +102,17 @@ KT_HD SinCos<double> rotate(const SinCos<double>& b, double d) {
 }
 KT_HD SinCos<float> rotate(const SinCos<float>& b, float d) {
     const float d2 = d*d;
-    float ps = fmaf(d2, -1.0f/110.0f, 1.0f);
-    ps = fmaf(d2 * (-1.0f/72.0f), ps, 1.0f);
-    ps = fmaf(d2 * (-1.0f/42.0f), ps, 1.0f);
-    ps = fmaf(d2 * (-1.0f/20.0f), ps, 1.0f);
-    ps = fmaf(d2 * (-1.0f/6.0f), ps, 1.0f);
-    const float sd = d * ps;
-    float pc = fmaf(d2, -1.0f/132.0f, 1.0f);
-    pc = fmaf(d2 * (-1.0f/90.0f), pc, 1.0f);
-    pc = fmaf(d2 * (-1.0f/56.0f), pc, 1.0f);
-    pc = fmaf(d2 * (-1.0f/30.0f), pc, 1.0f);
-    pc = fmaf(d2 * (-1.0f/12.0f), pc, 1.0f);
-    const float omc = 0.5f * d2 * pc;
+    float ps = fmaf(d2, -1.0f/39916800.0f, 1.0f/362880.0f);
+    ps = fmaf(d2, ps, -1.0f/5040.0f);
+    ps = fmaf(d2, ps, 1.0f/120.0f);
+    ps = fmaf(d2, ps, -1.0f/6.0f);
+    const float sd = fmaf(d*d2, ps, d);
+    float pc = fmaf(d2, 1.0f/479001600.0f, -1.0f/3628800.0f);
+    pc = fmaf(d2, pc, 1.0f/40320.0f);
+    pc = fmaf(d2, pc, -1.0f/720.0f);
+    pc = fmaf(d2, pc, 1.0f/24.0f);
+    pc = fmaf(d2, pc, -0.5f);
+    const float omc = -d2*pc;
     SinCos<float> o;
     o.s = fmaf(b.c, sd, fmaf(-b.s, omc, b.s));
     o.c = fmaf(-b.s, sd, fmaf(-b.c, omc, b.c));

@@ -619,8 +619,8 @@ static RunOpts run_opts(const kt_opts* o) {
     R.step_init = o->step_init < 1e-10 ? 1e-10 : o->step_init;
     R.rtol = o->rtol < 1e-10 && o->integrator == KT_INTEGRATOR_RK4 ? 1e-10 : o->rtol;
     R.atol = o->atol;
-    R.isl_noise = 3e-8;
-    R.isl_r = 2.5;
+    R.isl_noise = o->mixed_noise > 0.0 ? o->mixed_noise : kMixedNoiseDefault;
+    R.isl_r = o->mixed_r_island > 0.0 ? o->mixed_r_island : kMixedRIslandDefault;
     return R;
 }
 

@@ -50,6 +50,10 @@ struct FrameDev {
     double U00, U03, U11, U22, U33;
 };
 
+// default island parameters of the mixed-precision path
+static constexpr double kMixedNoiseDefault = 1e-9;
+static constexpr double kMixedRIslandDefault = 2.5;
+
 struct RunOpts {
     int32_t max_steps;
     uint32_t quirks;
@@ -303,9 +307,9 @@ struct LaneVote {
 template <int CHART, class Vote>
 KT_HD void stage_f64(const Knds<double>& P, const double y0[4], const double d[4], const SinCos<double>& sc0,
                      double pt, double pp, double k[5], SinCos<double>& sc) {
-    const double th = y0[1] + d[1];
-    if (Vote::any(!(fabs(d[1]) <= kRotateMax))) kt_sincos(th, &sc.s, &sc.c);
-    else sc = rotate(sc0, d[1]);
+    // per lane (not a wave vote): a ray's result must not depend on its wave-mates
+    if (fabs(d[1]) <= kRotateMax) sc = rotate(sc0, d[1]);
+    else kt_sincos(y0[1] + d[1], &sc.s, &sc.c);
     rhs<CHART>(P, y0[0] + d[0], sc, y0[2] + d[2], y0[3] + d[3], pt, pp, k);
 }
 
@@ -314,8 +318,8 @@ KT_HD void stage_f32(const Knds<float>& Pf, const double y0[4], const double d[4
                      double pt, double pp, double k[5]) {
     SinCos<float> scf;
     const float d1 = (float)d[1];
-    if (Vote::any(!(fabs(d[1]) <= kRotateMax))) kt_sincos((float)(y0[1] + d[1]), &scf.s, &scf.c);
-    else scf = rotate(SinCos<float>{(float)sc0.s, (float)sc0.c}, d1);
+    if (fabs(d[1]) <= kRotateMax) scf = rotate(SinCos<float>{(float)sc0.s, (float)sc0.c}, d1);
+    else kt_sincos((float)(y0[1] + d[1]), &scf.s, &scf.c);
     float kf[5];
     rhs<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
                (float)pt, (float)pp, kf);
@@ -379,8 +383,8 @@ KT_HD bool dp5_attempt(const Knds<double>& P, const Knds<float>& Pf, const RunOp
     } else {
         stage_f32<CHART, Vote>(Pf, y, d5, S.sc, S.pt, S.pp, k7);
         S.rhs32++;
-        if (Vote::any(!(fabs(d5[1]) <= kRotateMax))) kt_sincos(y[1] + d5[1], &sc7.s, &sc7.c);
-        else sc7 = rotate(S.sc, d5[1]);
+        if (fabs(d5[1]) <= kRotateMax) sc7 = rotate(S.sc, d5[1]);
+        else kt_sincos(y[1] + d5[1], &sc7.s, &sc7.c);
     }
     double e2 = 0.0;
     for (int i = 0; i < 4; ++i) {
@@ -432,8 +436,8 @@ KT_HD void rk4_incr(const Knds<double>& P, const Knds<float>& Pf, bool use64, co
 template <int CHART, int PREC, class Vote>
 KT_HD SinCos<double> sc_at(const SinCos<double>& sc0, double th0, double dth) {
     SinCos<double> o;
-    if (Vote::any(!(fabs(dth) <= kRotateMax))) kt_sincos(th0 + dth, &o.s, &o.c);
-    else o = rotate(sc0, dth);
+    if (fabs(dth) <= kRotateMax) o = rotate(sc0, dth);
+    else kt_sincos(th0 + dth, &o.s, &o.c);
     return o;
 }
 
@@ -491,8 +495,10 @@ KT_HD bool rk4_attempt(const Knds<double>& P, const Knds<float>& Pf, const RunOp
     return false;
 }
 
-// (p_r, p_theta) -> kappa (p_r, p_theta) with H = 0 exactly (binary64); used by
-// the mixed-precision path after accepted steps.
+// Constraint renormalisation (binary64) for the mixed-precision path: one Newton
+// step of minimal norm in (p_r, p_theta) onto N = 2 Sigma H = 0,
+//   dp = -N grad_p N / |grad_p N|^2,
+// applied only when the drift exceeds 1e-12 of the constraint's scale.
 template <int CHART>
 KT_HD void renormalise(const Knds<double>& P, RayState& S) {
     const double r = S.y[0], s2 = S.sc.s*S.sc.s, c2 = S.sc.c*S.sc.c;
@@ -501,22 +507,27 @@ KT_HD void renormalise(const Knds<double>& P, RayState& S) {
     const double dT = 1.0 + P.lam3a2*c2;
     const double K = ra*S.pt + P.a*S.pp;
     const double L = P.a*s2*S.pt + S.pp;
-    const double A = dR*S.y[2]*S.y[2] + dT*S.y[3]*S.y[3];
-    double kap = 1.0;
+    const double pr = S.y[2], pth = S.y[3];
+    const double tL = P.xi2*L*L/(dT*s2);
+    double N, gr, scale;
     if (CHART == CHART_BL) {
-        const double C = -P.xi2*K*K/dR + P.xi2*L*L/(dT*s2);
-        if (A > 0.0 && -C > 0.0) kap = sqrt(-C / A);
+        const double tK = P.xi2*K*K/dR;
+        N = dR*pr*pr + dT*pth*pth - tK + tL;
+        gr = 2.0*dR*pr;
+        scale = fabs(dR*pr*pr) + dT*pth*pth + fabs(tK) + tL;
     } else {
-        const double B = 2.0*P.xi*S.y[2]*K;
-        const double C = P.xi2*L*L/(dT*s2);
-        const double disc = B*B - 4.0*A*C;
-        if (fabs(A) > 1e-300 && disc >= 0.0) {
-            const double sq = sqrt(disc);
-            const double k1 = (-B + sq) / (2.0*A), k2 = (-B - sq) / (2.0*A);
-            kap = (fabs(k1 - 1.0) < fabs(k2 - 1.0)) ? k1 : k2;
-        }
+        const double xk = 2.0*P.xi*K*pr;
+        N = dR*pr*pr + xk + dT*pth*pth + tL;
+        gr = 2.0*(dR*pr + P.xi*K);
+        scale = fabs(dR*pr*pr) + fabs(xk) + dT*pth*pth + tL;
     }
-    if (isfinite(kap) && fabs(kap - 1.0) < 1e-4) { S.y[2] *= kap; S.y[3] *= kap; }
+    const double gt = 2.0*dT*pth;
+    const double g2 = gr*gr + gt*gt;
+    if (!(fabs(N) > 1e-12*scale) || !(g2 > 0.0)) return;
+    const double f = N / g2;
+    if (!isfinite(f)) return;
+    S.y[2] = pr - f*gr;
+    S.y[3] = pth - f*gt;
 }
 
 // Event test after an accepted step (main.cpp:363-431).  Returns true if the ray

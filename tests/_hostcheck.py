@@ -1,0 +1,69 @@
+"""ctypes loader of tests/hostcheck/libkt_hostcheck.so: the product's per-ray tracer
+(kerrtracecpp2_amd/csrc/kt_trace.hpp) compiled for the host.  Test-only."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from kerrtracecpp2_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "hostcheck", "kt_hostcheck.cpp")
+LIB = os.path.join(HERE, "hostcheck", "libkt_hostcheck.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared",
+                           "-o", LIB, SRC])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.c_void_p
+        L.ktc_trace_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P, P, P, P]
+        L.ktc_shade.argtypes = [P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_double, P, ctypes.c_int, ctypes.c_int, P]
+        L.ktc_isco_exact.argtypes = [ctypes.c_double] * 4 + [P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def trace_pixels(frame, opts, px, py):
+    px = np.ascontiguousarray(px, np.int32)
+    py = np.ascontiguousarray(py, np.int32)
+    n = px.size
+    out = np.zeros(n, _abi.GEOPIXEL_DTYPE)
+    acc = np.zeros(n, np.int64)
+    r32 = np.zeros(n, np.int64)
+    r64 = np.zeros(n, np.int64)
+    info = np.zeros(5)
+    rc = lib().ktc_trace_pixels(ctypes.byref(frame), ctypes.byref(opts), _p(px), _p(py), n, _p(out), _p(acc),
+                                _p(r32), _p(r64), _p(info))
+    assert rc == 0
+    return out, dict(accepted=acc, rhs32=r32, rhs64=r64, r_horizon=info[0], r_isco=info[1])
+
+
+def shade(geo, M, r_isco, bg=None, exposure=1.0, gamma=2.2, temp_scale=1.0):
+    geo = np.ascontiguousarray(geo)
+    out = np.zeros(geo.size, np.uint32)
+    bh, bw = (bg.shape[0], bg.shape[1]) if bg is not None else (0, 0)
+    lib().ktc_shade(_p(geo), geo.size, M, r_isco, exposure, gamma, temp_scale,
+                    _p(np.ascontiguousarray(bg)) if bg is not None else None, bw, bh, _p(out))
+    return out
+
+
+def isco_exact(M, a, Q, Lam):
+    seq = np.zeros(1)
+    seg = np.zeros(1)
+    lib().ktc_isco_exact(M, a, Q, Lam, _p(seq), _p(seg))
+    return float(seq[0]), float(seg[0])

@@ -61,7 +61,8 @@ extern "C" int ktc_trace_pixels(const kt_frame* f, const kt_opts* o, const int32
     O.quirks = o->quirks;
     O.step_init = o->step_init < 1e-10 ? 1e-10 : o->step_init;
     O.rtol = o->rtol; O.atol = o->atol;
-    O.isl_noise = 3e-8; O.isl_r = 2.5;
+    O.isl_noise = o->mixed_noise > 0.0 ? o->mixed_noise : kMixedNoiseDefault;
+    O.isl_r = o->mixed_r_island > 0.0 ? o->mixed_r_island : kMixedRIslandDefault;
     if (info5) { info5[0] = F.r_h; info5[1] = F.r_isco; info5[2] = F.r_esc; info5[3] = F.chart; info5[4] = F.ks_ref_init; }
     const int key = F.chart*4 + o->integrator*2 + o->precision;
     switch (key) {

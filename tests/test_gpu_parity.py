@@ -1,0 +1,226 @@
+"""GPU parity: the HIP kernels (through the C ABI of libkt_hip.so) against the oracle and
+the reference golden fixtures.
+
+Tolerances (floating point, per pixel, outside the fate-boundary band -- pixels with an
+8-neighbour of another fate in the oracle image, i.e. the photon ring and the disk/shadow
+edges where the map is discontinuous):
+  fate agreement >= 99.9 %, |dg| <= 1e-4, |dr_hit| <= 2e-3 (tol 1e-7 RK4) / 1e-3 (tol 1e-9).
+Frame scalars (horizon, ISCO) are bit-exact.
+"""
+import numpy as np
+import pytest
+
+import kerrtracecpp2_amd as kt
+from oracle import kto
+from tests._util import angle_diff, fate_boundary_band, golden_manifest, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _opts(chart="bl", method="rk4", prec="fp64", rtol=1e-7, atol=None, **kw):
+    o = kt.default_opts()
+    o.chart = 1 if chart == "ks" else 0
+    o.integrator = 1 if method == "dopri5" else 0
+    o.precision = 1 if prec == "mixed" else 0
+    o.rtol = rtol
+    o.atol = rtol if atol is None else atol
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _frame_from_manifest(kw):
+    keys = dict(kw)
+    for k in ("chart", "integrator", "tol", "bundles", "max_steps", "step_init"):
+        keys.pop(k, None)
+    return kt.make_frame(**keys)
+
+
+def _check(ref, got, W, H, dg_tol=1e-4, dr_tol=2e-3, fate_min=0.999):
+    band = fate_boundary_band(ref["outcome"], W, H)
+    keep = ~band
+    fate = (ref["outcome"] == got["outcome"])[keep].mean()
+    assert fate >= fate_min, f"fate agreement {fate}"
+    d = keep & (ref["outcome"] == 1) & (got["outcome"] == 1)
+    if d.any():
+        dg = np.abs(ref["redshift"][d] - got["redshift"][d]).max()
+        dr = np.abs(ref["r"][d] - got["r"][d]).max()
+        assert dg <= dg_tol, f"max |dg| {dg}"
+        assert dr <= dr_tol, f"max |dr| {dr}"
+    return band
+
+
+def test_frame_scalars_bit_exact(gpu_ctx):
+    sets = [(1.0, 0.5, 0.0, 0.0), (1.0, 0.9, 0.3, 1e-5), (1.0, -0.7, 0.2, 0.0), (1.0, 0.998, 0.0, 0.0),
+            (1.0, 0.0, 0.4, 0.0), (1.0, 0.3, 0.1, -1e-4), (1.0, 0.0, 0.0, 0.0)]
+    frames = [kt.make_frame(8, 8, a=a, Q=Q, Lambda=L, M=M) for (M, a, Q, L) in sets]
+    info = gpu_ctx.frame_info(frames, _opts())
+    for (M, a, Q, L), i in zip(sets, info):
+        assert i["r_horizon"] == kto.r_horizon([M, a, Q, L]), (a, Q, L)
+        assert i["r_isco"] == kto.r_isco([M, a, Q, L]), (a, Q, L)
+        assert i["r_escape"] == 40.0 * 1.05
+
+
+def test_c1_full_frame_vs_reference(gpu_ctx):
+    """Configuration 1: BL, RK4 doubling, tol 1e-7, 320x180 (GeoPixels of the reference binary)."""
+    g = load_golden("c1_bl_rk4_320x180")
+    W, H = 320, 180
+    out = gpu_ctx.render([kt.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12)], _opts())
+    band = _check(g["pixels"], out["geo"], W, H)
+    geo = out["geo"]
+    assert np.all(geo["magnif"] == 1.0) and np.all(geo["pad"] == 0)
+    assert out["info"][0]["r_isco"] == g["meta"][4]
+    esc = (~band) & (g["pixels"]["outcome"] == 0) & (geo["outcome"] == 0)
+    # positional escape fill: phi_disk <- theta, theta_esc <- phi (main.cpp:428-430)
+    assert np.abs(geo["phi_disk"][esc] - g["pixels"]["phi_disk"][esc]).max() < 2e-3
+    assert angle_diff(geo["theta_esc"][esc], g["pixels"]["theta_esc"][esc]).max() < 2e-3
+    assert np.all(geo["phi_esc"][esc] == 0.0)
+
+
+def test_c1_rgb_vs_reference_png(gpu_ctx):
+    from tests.golden.make_golden import checker
+    gpu_ctx.set_background(checker())
+    try:
+        W, H = 320, 180
+        out = gpu_ctx.render([kt.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12)], _opts())
+    finally:
+        gpu_ctx.set_background(None)
+    want = load_golden("c1_rgb_320x180")["rgb"].reshape(-1, 3).astype(int)
+    got = kt.rgba_to_rgb(out["rgba"]).astype(int)
+    band = fate_boundary_band(load_golden("c1_bl_rk4_320x180")["pixels"]["outcome"], W, H)
+    diff = np.abs(got - want).max(axis=1)
+    assert np.mean(diff[~band] <= 2) >= 0.999
+    assert np.all(out["rgba"] >> 24 == 0xFF)
+
+
+@pytest.mark.parametrize("name", ["bl_dopri5_160x90", "kn_ks_64x64"])
+def test_frames_vs_reference(gpu_ctx, name):
+    man = golden_manifest()["frames"][name]["frame"]
+    g = load_golden(name)
+    W, H = int(g["W"]), int(g["H"])
+    o = _opts(chart=man.get("chart", "bl"), method=man.get("integrator", "rk4"), rtol=man.get("tol", 1e-7))
+    out = gpu_ctx.render([_frame_from_manifest(man)], o)
+    # kn_ks: the reference's CPU KS path ignores --dopri5 and runs RK4 at tol 1e-7 (main.cpp:1539-1545)
+    _check(g["pixels"], out["geo"], W, H, dg_tol=2e-4 if name == "kn_ks_64x64" else 1e-4)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_c2_ks_lambda_vs_reference(gpu_ctx, prec):
+    """Configuration 2 at 160x90: KNdS KS chart + DOPRI5 vs the reference (which runs BL, main.cpp:2034)."""
+    g = load_golden("c2_ks_dopri5_lambda_160x90")
+    W, H = 160, 90
+    o = _opts(chart="ks", method="dopri5", prec=prec, rtol=1e-9, atol=1e-12)
+    out = gpu_ctx.render([kt.make_frame(W, H, a=0.9, Q=0.3, Lambda=1e-5, theta=85, r_obs=40, disk_out=12)], o)
+    band = _check(g["pixels"], out["geo"], W, H, dr_tol=1e-3)
+    esc = (~band) & (g["pixels"]["outcome"] == 0) & (out["geo"]["outcome"] == 0)
+    assert np.all(out["geo"]["theta_esc"][esc] == 0.0)          # DOPRI5: phi frozen at phi_obs = 0
+    if prec == "mixed":
+        st = out["stats"]
+        assert st["rhs_fp32"] + st["rhs_fp64"] > 0
+
+
+@pytest.mark.parametrize("name,a", [("ks_rk4_spin_pos_160x90", 0.5), ("ks_rk4_spin_neg_160x90", -0.5)])
+def test_ks_lambda0_vs_reference(gpu_ctx, name, a):
+    g = load_golden(name)
+    W, H = 160, 90
+    out = gpu_ctx.render([kt.make_frame(W, H, a=a, theta=80, r_obs=40, fov=45, disk_out=12)], _opts(chart="ks"))
+    _check(g["pixels"], out["geo"], W, H, dg_tol=3e-4, dr_tol=3e-3)
+
+
+def test_spin_orientation_flips(gpu_ctx):
+    """Image-level property of tests/spin_orientation_regression.cpp:107-212 (with the 28-byte
+    GeoPixel layout): the Doppler-bright side of the disk flips with the sign of a."""
+    W, H = 320, 180
+    sides = []
+    for a in (0.5, -0.5):
+        geo = gpu_ctx.render([kt.make_frame(W, H, a=a, theta=80, r_obs=40, fov=45, disk_out=12)],
+                             _opts(chart="ks"))["geo"].reshape(H, W)
+        disk = geo["outcome"] == 1
+        g = np.where(disk, geo["redshift"], np.nan)
+        thr = np.nanpercentile(g, 97)
+        ys, xs = np.nonzero(disk & (g >= thr))
+        sides.append(xs.mean() > W / 2)
+    assert sides[0] != sides[1]
+
+
+def test_c2_full_size_fp64_vs_mixed_and_oracle_sample(gpu_ctx):
+    """Configuration 2 at full size (1280x720): oracle on a strided sample, mixed vs FP64 on
+    every pixel, bit-identical FP64 reruns."""
+    W, H = 1280, 720
+    fr = kt.make_frame(W, H, a=0.9, Q=0.3, Lambda=1e-5, theta=85, r_obs=40, disk_out=12)
+    o64 = _opts(chart="ks", method="dopri5", rtol=1e-9, atol=1e-12)
+    a = gpu_ctx.render([fr], o64)
+    b = gpu_ctx.render([fr], o64)
+    assert a["geo"].tobytes() == b["geo"].tobytes()
+    geo = a["geo"]
+    assert set(np.unique(geo["outcome"])) <= {0, 1, 2}
+    mixed = gpu_ctx.render([fr], _opts(chart="ks", method="dopri5", prec="mixed", rtol=1e-9, atol=1e-12))
+    _check(geo, mixed["geo"], W, H, dr_tol=1e-3)
+    ys, xs = np.mgrid[4:H:16, 4:W:16]
+    px, py = xs.ravel(), ys.ravel()
+    fo = kto.make_frame(W, H, a=0.9, Q=0.3, Lambda=1e-5, theta=85, r_obs=40, disk_out=12, chart="ks",
+                        integrator="dopri5", tol=1e-9)
+    ref = kto.trace_pixels(fo, px, py)
+    band = fate_boundary_band(geo["outcome"], W, H).reshape(H, W)[py, px]
+    sub = geo.reshape(H, W)[py, px]
+    keep = ~band
+    assert (sub["outcome"] == ref["outcome"])[keep].mean() >= 0.999
+    d = keep & (sub["outcome"] == 1) & (ref["outcome"] == 1)
+    assert np.abs(sub["redshift"][d] - ref["redshift"][d]).max() <= 1e-4
+
+
+def test_batched_frames_equal_single_frames(gpu_ctx):
+    fa = kt.make_frame(96, 40, a=0.9, Q=0.3, Lambda=1e-5, theta=85, r_obs=40, disk_out=12)
+    fb = kt.make_frame(37, 53, a=0.6, Q=0.1, Lambda=3e-6, theta=60, r_obs=25, disk_out=15)
+    o = _opts(chart="ks", method="dopri5", rtol=1e-9, atol=1e-12)
+    both = gpu_ctx.render([fa, fb], o)
+    ga = gpu_ctx.render([fa], o)["geo"]
+    gb = gpu_ctx.render([fb], o)["geo"]
+    assert both["geo"][:96 * 40].tobytes() == ga.tobytes()
+    assert both["geo"][96 * 40:].tobytes() == gb.tobytes()
+    assert both["info"][1]["r_isco"] == kto.r_isco([1.0, 0.6, 0.1, 3e-6])
+
+
+@pytest.mark.parametrize("W,H", [(1, 1), (3, 7), (1, 64), (65, 1), (9, 9)])
+def test_odd_sizes_vs_oracle(gpu_ctx, W, H):
+    fr = kt.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12)
+    got = gpu_ctx.render([fr], _opts())["geo"]
+    ref = kto.trace_frame(kto.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12))
+    assert np.array_equal(got["outcome"], ref["outcome"])
+    d = got["outcome"] == 1
+    assert np.abs(got["redshift"][d] - ref["redshift"][d]).max(initial=0) <= 1e-4
+
+
+def test_max_steps_one_vs_oracle(gpu_ctx):
+    W, H = 32, 18
+    got = gpu_ctx.render([kt.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12)],
+                         _opts(max_steps=1))["geo"]
+    ref = kto.trace_frame(kto.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12, max_steps=1))
+    assert np.array_equal(got["outcome"], ref["outcome"])
+    np.testing.assert_allclose(got["r"], ref["r"], rtol=1e-6)
+    np.testing.assert_allclose(got["phi_disk"], ref["phi_disk"], atol=1e-6)
+    np.testing.assert_allclose(got["theta_esc"], ref["theta_esc"], atol=1e-6)
+
+
+def test_device_colorize_vs_oracle(gpu_ctx):
+    from tests.golden.make_golden import checker
+    g = load_golden("c1_bl_rk4_320x180")
+    bg = checker()
+    gpu_ctx.set_background(bg)
+    try:
+        rgba = gpu_ctx.colorize(g["pixels"], 1.0, g["meta"][4], _opts())
+    finally:
+        gpu_ctx.set_background(None)
+    want = load_golden("c1_rgb_320x180")["rgb"].reshape(-1, 3).astype(int)
+    got = kt.rgba_to_rgb(rgba).astype(int)
+    assert np.abs(got - want).max() <= 1
+    assert (got != want).any(axis=1).mean() < 1e-3
+
+
+def test_invalid_arguments_are_reported(gpu_ctx):
+    with pytest.raises(kt.KtError):
+        gpu_ctx.render([kt.make_frame(0, 10, a=0.5)], _opts())
+    o = _opts()
+    o.chart = 7
+    with pytest.raises(kt.KtError):
+        gpu_ctx.render([kt.make_frame(4, 4, a=0.5)], o)

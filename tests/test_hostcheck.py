@@ -1,0 +1,107 @@
+"""The product's per-ray tracer (kt_trace.hpp / kt_setup.hpp / kt_shade.hpp), compiled for
+the host, against the oracle.  Same sources as the HIP kernel; catches logic errors on the
+CPU before any GPU run.  Tolerances are those of the GPU parity tests."""
+import numpy as np
+import pytest
+
+from kerrtracecpp2_amd import _abi
+from oracle import kto
+from tests import _hostcheck as hc
+from tests._util import angle_diff, fate_boundary_band, golden_manifest, load_golden
+
+
+def _grid(W, H, step):
+    ys, xs = np.mgrid[0:H:step, 0:W:step]
+    return xs.ravel().astype(np.int32), ys.ravel().astype(np.int32)
+
+
+def _compare(ref, got, band, dg_tol, dr_tol, fate_min=0.999):
+    keep = ~band
+    fate_ok = (ref["outcome"] == got["outcome"])[keep].mean()
+    assert fate_ok >= fate_min, fate_ok
+    d = keep & (ref["outcome"] == 1) & (got["outcome"] == 1)
+    if d.any():
+        assert np.abs(ref["redshift"][d] - got["redshift"][d]).max() <= dg_tol
+        assert np.abs(ref["r"][d] - got["r"][d]).max() <= dr_tol
+
+
+@pytest.mark.parametrize("method", ["rk4", "dopri5"])
+def test_c1_bl_full_frame(method):
+    W, H = 320, 180
+    fo = kto.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12, chart="bl", integrator=method,
+                        tol=1e-7)
+    ref = load_golden("c1_bl_rk4_320x180")["pixels"] if method == "rk4" else kto.trace_frame(fo)
+    px, py = _grid(W, H, 1)
+    op = _abi.default_opts()
+    op.integrator = 1 if method == "dopri5" else 0
+    got, ex = hc.trace_pixels(_abi.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12), op, px, py)
+    band = fate_boundary_band(ref["outcome"], W, H)
+    _compare(ref, got, band, dg_tol=1e-4, dr_tol=2e-3)
+    assert ex["r_isco"] == kto.frame_info(fo).r_isco
+
+
+def test_c2_ks_lambda_matches_bl_oracle():
+    """KS chart with Lambda != 0 (no reference counterpart) against the reference's BL fallback."""
+    g = load_golden("c2_ks_dopri5_lambda_160x90")
+    W, H = 160, 90
+    px, py = _grid(W, H, 1)
+    op = _abi.default_opts()
+    op.chart, op.integrator, op.rtol, op.atol = 1, 1, 1e-9, 1e-12
+    fr = _abi.make_frame(W, H, a=0.9, Q=0.3, Lambda=1e-5, theta=85, r_obs=40, disk_out=12)
+    got, ex = hc.trace_pixels(fr, op, px, py)
+    ref = g["pixels"]
+    band = fate_boundary_band(ref["outcome"], W, H)
+    _compare(ref, got, band, dg_tol=1e-4, dr_tol=1e-3)
+    assert ex["r_horizon"] == kto.r_horizon([1.0, 0.9, 0.3, 1e-5])
+    assert ex["r_isco"] == g["meta"][4]
+    esc = (~band) & (ref["outcome"] == 0) & (got["outcome"] == 0)
+    # DOPRI5 keeps phi frozen (phi_obs = 0) and escapes fill theta_esc <- phi: exact zeros
+    assert np.all(got["theta_esc"][esc] == 0.0) and np.all(ref["theta_esc"][esc] == 0.0)
+    assert np.abs(got["phi_disk"][esc] - ref["phi_disk"][esc]).max() < 2e-3     # theta at escape
+
+
+@pytest.mark.parametrize("name,a", [("ks_rk4_spin_pos_160x90", 0.5), ("ks_rk4_spin_neg_160x90", -0.5)])
+def test_ks_lambda0_matches_reference_ks(name, a):
+    ref = load_golden(name)["pixels"]
+    W, H = 160, 90
+    px, py = _grid(W, H, 1)
+    op = _abi.default_opts()
+    op.chart = 1
+    got, _ = hc.trace_pixels(_abi.make_frame(W, H, a=a, theta=80, r_obs=40, fov=45, disk_out=12), op, px, py)
+    band = fate_boundary_band(ref["outcome"], W, H)
+    _compare(ref, got, band, dg_tol=3e-4, dr_tol=3e-3)
+    esc = (~band) & (ref["outcome"] == 0) & (got["outcome"] == 0)
+    assert angle_diff(got["theta_esc"][esc], ref["theta_esc"][esc]).max() < 5e-3
+
+
+def test_setup_isco_exact_scan_matches_sequential():
+    for mp in ((1.0, 0.9, 0.3, 1e-5), (1.0, 0.0, 0.4, 0.0), (1.0, -0.6, 0.1, 0.0), (1.0, 0.3, 0.0, -1e-4)):
+        seq, seg = hc.isco_exact(*mp)
+        assert seq == seg == kto.r_isco(list(mp)), mp
+
+
+def test_shading_matches_oracle():
+    g = load_golden("c1_bl_rk4_320x180")
+    from tests.golden.make_golden import checker
+    bg = checker()
+    want = load_golden("c1_rgb_320x180")["rgb"].reshape(-1, 3)
+    rgba = hc.shade(g["pixels"], 1.0, g["meta"][4], bg=bg)
+    got = np.stack([rgba & 0xFF, (rgba >> 8) & 0xFF, (rgba >> 16) & 0xFF], -1)
+    assert np.abs(got.astype(int) - want.astype(int)).max() <= 1
+    assert (got != want).any(axis=1).mean() < 1e-3
+
+
+def test_max_steps_one_positional_results():
+    """--max-steps 1: every ray stops after one accepted step with the reference's
+    positional TraceResult fill {ESCAPED, r, 1, theta, phi} (main.cpp:433)."""
+    W, H = 16, 9
+    px, py = _grid(W, H, 1)
+    fo = kto.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12, max_steps=1)
+    ref = kto.trace_pixels(fo, px, py)
+    op = _abi.default_opts()
+    op.max_steps = 1
+    got, _ = hc.trace_pixels(_abi.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12), op, px, py)
+    assert np.all(got["outcome"] == ref["outcome"])
+    np.testing.assert_allclose(got["r"], ref["r"], rtol=1e-6)
+    np.testing.assert_allclose(got["phi_disk"], ref["phi_disk"], atol=1e-6)   # theta
+    np.testing.assert_allclose(got["theta_esc"], ref["theta_esc"], atol=1e-6)  # phi
