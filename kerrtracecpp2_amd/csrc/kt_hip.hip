@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kSetupThreads) void kt_setup_kernel(FrameDev* frame
     // ---- ISCO (knds_metric.hpp:318-349) ----
     double risco = 0.0;
     if (fabs(F.Lambda) < 1e-15 && fabs(F.Q) < 1e-15) {
-        risco = isco_bardeen(F.M, F.a);
+        risco = F.isco_closed;
     } else {
         const double rmax = 30.0*F.M;
         if (t == 0) s_ns = isco_segments(rh * 1.05, 1e-4*F.M, rmax, s_seg);
@@ -583,6 +583,7 @@ static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, co
         F.ray_base = rays;
         F.pix_base = pix;
         F.chart = ch;
+        F.isco_closed = isco_bardeen(f.M, f.a);
         F.ks_ref_init = (ch == KT_CHART_KS && (o->quirks & KT_QUIRK_KS_INIT_NO_SHIFT) && std::fabs(f.Lambda) <= 1e-15)
                             ? 1 : 0;
         rays += (int64_t)F.tiles_x * F.tiles_y * 64;

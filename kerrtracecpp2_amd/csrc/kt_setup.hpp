@@ -57,6 +57,9 @@ KT_HD double isco_L(const FrameDev& F, double rr) {
 
 // Bardeen-Press-Teukolsky ISCO (knds_metric.hpp:320-327)
 KT_HD double isco_bardeen(double M, double a) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
     if (fabs(a) < 1e-10) return 6.0*M;
     const double ap = a/M;
     const double z1 = 1.0 + cbrt(1.0-ap*ap) * (cbrt(1.0+ap)+cbrt(1.0-ap));

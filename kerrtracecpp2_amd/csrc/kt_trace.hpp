@@ -48,6 +48,7 @@ struct FrameDev {
     double u0, ep, et;
     int32_t frame_ok, pad0;
     double U00, U03, U11, U22, U33;
+    double isco_closed;    // host-evaluated Bardeen ISCO when Q = Lambda = 0 (host libm, as the reference)
 };
 
 // default island parameters of the mixed-precision path

@@ -38,3 +38,23 @@ def angle_diff(a, b):
     return np.abs((d + np.pi) % (2 * np.pi) - np.pi)
 
 
+
+
+def steep_redshift_band(geo, W, H, thr=0.05):
+    """Disk pixels whose oracle redshift differs from an 8-neighbour disk pixel by more than thr:
+    near-critical rays on the disk's inner rim next to the photon orbit, where g varies by ~3 per M
+    of r_hit and the oracle's own error at tol 1e-9 exceeds 1e-4."""
+    g = np.where(np.asarray(geo["outcome"]) == 1, np.asarray(geo["redshift"], np.float64), np.nan).reshape(H, W)
+    pad = np.pad(g, 1, constant_values=np.nan)
+    band = np.zeros((H, W), dtype=bool)
+    with np.errstate(invalid="ignore"):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                band |= np.abs(pad[1 + dy:1 + dy + H, 1 + dx:1 + dx + W] - g) > thr
+    return band.ravel()
+
+
+def photon_ring_band(geo, W, H):
+    """The stated exclusion band of the per-pixel parity checks: fate-boundary pixels (photon
+    ring, shadow and disk edges) plus the steep inner rim of the disk."""
+    return fate_boundary_band(geo["outcome"], W, H) | steep_redshift_band(geo, W, H)

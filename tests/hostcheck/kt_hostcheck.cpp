@@ -25,6 +25,7 @@ static void fill_frame(const kt_frame& f, const kt_opts& o, FrameDev& F) {
     frame_knds(F);
     frame_camera_consts(F);
     F.r_h = horizon_seq(F);
+    F.isco_closed = isco_bardeen(F.M, F.a);
     F.r_isco = isco_seq(F, F.r_h);
     F.r_in = F.r_isco; F.r_out = F.disk_out; F.r_esc = F.r_obs * 1.05; F.r_cut = F.r_h * 1.03;
 }

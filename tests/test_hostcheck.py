@@ -7,7 +7,7 @@ import pytest
 from kerrtracecpp2_amd import _abi
 from oracle import kto
 from tests import _hostcheck as hc
-from tests._util import angle_diff, fate_boundary_band, golden_manifest, load_golden
+from tests._util import angle_diff, golden_manifest, load_golden, photon_ring_band
 
 
 def _grid(W, H, step):
@@ -35,7 +35,7 @@ def test_c1_bl_full_frame(method):
     op = _abi.default_opts()
     op.integrator = 1 if method == "dopri5" else 0
     got, ex = hc.trace_pixels(_abi.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12), op, px, py)
-    band = fate_boundary_band(ref["outcome"], W, H)
+    band = photon_ring_band(ref, W, H)
     _compare(ref, got, band, dg_tol=1e-4, dr_tol=2e-3)
     assert ex["r_isco"] == kto.frame_info(fo).r_isco
 
@@ -50,7 +50,7 @@ def test_c2_ks_lambda_matches_bl_oracle():
     fr = _abi.make_frame(W, H, a=0.9, Q=0.3, Lambda=1e-5, theta=85, r_obs=40, disk_out=12)
     got, ex = hc.trace_pixels(fr, op, px, py)
     ref = g["pixels"]
-    band = fate_boundary_band(ref["outcome"], W, H)
+    band = photon_ring_band(ref, W, H)
     _compare(ref, got, band, dg_tol=1e-4, dr_tol=1e-3)
     assert ex["r_horizon"] == kto.r_horizon([1.0, 0.9, 0.3, 1e-5])
     assert ex["r_isco"] == g["meta"][4]
@@ -68,7 +68,7 @@ def test_ks_lambda0_matches_reference_ks(name, a):
     op = _abi.default_opts()
     op.chart = 1
     got, _ = hc.trace_pixels(_abi.make_frame(W, H, a=a, theta=80, r_obs=40, fov=45, disk_out=12), op, px, py)
-    band = fate_boundary_band(ref["outcome"], W, H)
+    band = photon_ring_band(ref, W, H)
     _compare(ref, got, band, dg_tol=3e-4, dr_tol=3e-3)
     esc = (~band) & (ref["outcome"] == 0) & (got["outcome"] == 0)
     assert angle_diff(got["theta_esc"][esc], ref["theta_esc"][esc]).max() < 5e-3
