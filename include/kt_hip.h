@@ -83,8 +83,12 @@ typedef struct kt_opts {
     double rtol, atol;      /* RK4: err < rtol (absolute, geodesic.hpp:234);
                                DOPRI5: scale = atol + rtol*|y0| (reference: atol = rtol = tol) */
     double exposure, gamma, temp_scale;   /* ColorParams (main.cpp:72-81)                   */
-    int32_t tuning_waves_per_cu;          /* 0: autotuned / default                        */
-    int32_t tuning_block;                 /* 0: default (256)                              */
+    int32_t tuning_blocks_per_cu;         /* persistent grid = blocks_per_cu x CUs (256-thread
+                                             blocks); 0: autotuned / built-in default         */
+    int32_t tuning_occupancy;             /* register budget variant: 0 default (2 waves per
+                                             SIMD), 1 the compiler's own budget, 2            */
+    int32_t tuning_chunk;                 /* rays per work-queue grab; 0: 64                   */
+    int32_t tuning_pad;
     double mixed_noise;     /* MIXED: binary32 stage-noise coefficient of the island test;
                                0 selects the built-in default                               */
     double mixed_r_island;  /* MIXED: binary64 steps below r < mixed_r_island * r_horizon;

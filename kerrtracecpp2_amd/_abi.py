@@ -34,7 +34,8 @@ class KtOpts(ctypes.Structure):
                 ("bundles", ctypes.c_int32), ("max_steps", ctypes.c_int32), ("quirks", ctypes.c_uint32),
                 ("step_init", ctypes.c_double), ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
                 ("exposure", ctypes.c_double), ("gamma", ctypes.c_double), ("temp_scale", ctypes.c_double),
-                ("tuning_waves_per_cu", ctypes.c_int32), ("tuning_block", ctypes.c_int32),
+                ("tuning_blocks_per_cu", ctypes.c_int32), ("tuning_occupancy", ctypes.c_int32),
+                ("tuning_chunk", ctypes.c_int32), ("tuning_pad", ctypes.c_int32),
                 ("mixed_noise", ctypes.c_double), ("mixed_r_island", ctypes.c_double)]
 
 

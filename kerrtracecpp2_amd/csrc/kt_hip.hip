@@ -235,10 +235,12 @@ struct TraceArgs {
     uint32_t* geo;           // 7 dwords per pixel
     unsigned long long* words;
     RunOpts O;
+    int32_t chunk;           // rays per queue grab
 };
 
 constexpr int kBlock = 256;
-constexpr int kChunk = 64;
+constexpr int kDefaultBlocksPerCu = 2;
+constexpr int kDefaultChunk = 64;
 
 __device__ __forceinline__ int find_frame(const FrameDev* fr, int n, int64_t g) {
     int lo = 0, hi = n - 1;
@@ -254,8 +256,8 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
-template <int CHART, int METHOD, int PREC>
-__global__ __launch_bounds__(kBlock) void kt_trace_kernel(TraceArgs A) {
+template <int CHART, int METHOD, int PREC, bool PHI, bool SINGLE>
+__device__ __forceinline__ void trace_loop(const TraceArgs& A) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     RayState S;
@@ -263,11 +265,12 @@ __global__ __launch_bounds__(kBlock) void kt_trace_kernel(TraceArgs A) {
     bool active = false;
     int frame = 0;
     int px = 0, py = 0;
+    // single frame: the constants are wave-uniform (scalar registers)
     Knds<double> P = A.frames[0].P;
     Knds<float> Pf = knds_cast<float>(P);
     int64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
-    unsigned long long c_acc = 0, c_rej = 0, c_r32 = 0, c_r64 = 0, c_herm = 0, c_rays = 0;
+    uint32_t c_acc = 0, c_rej = 0, c_r32 = 0, c_r64 = 0, c_herm = 0, c_rays = 0;
 
     for (;;) {
         // ---- refill idle lanes from the wave's pool / the global queue ----
@@ -276,19 +279,19 @@ __global__ __launch_bounds__(kBlock) void kt_trace_kernel(TraceArgs A) {
             const int need = __popcll(idle);
             const int rank = __popcll(idle & lt_mask);
             int64_t mine = -1;
-            int64_t avail = pool_end - pool_next;
-            int taken = (int)(avail < need ? avail : need);
+            const int64_t avail = pool_end - pool_next;
+            const int taken = (int)(avail < need ? avail : need);
             if (!active && rank < taken) mine = pool_next + rank;
             pool_next += taken;
             if (taken < need && !exhausted) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&A.words[W_QUEUE], (unsigned long long)kChunk);
+                if (lane == 0) base = atomicAdd(&A.words[W_QUEUE], (unsigned long long)A.chunk);
                 base = __shfl(base, 0, 64);
                 if ((int64_t)base >= A.total_rays) {
                     exhausted = true;
                 } else {
                     pool_next = (int64_t)base;
-                    pool_end = (int64_t)base + kChunk < A.total_rays ? (int64_t)base + kChunk : A.total_rays;
+                    pool_end = (int64_t)base + A.chunk < A.total_rays ? (int64_t)base + A.chunk : A.total_rays;
                     const int avail2 = (int)(pool_end - pool_next);
                     const int rest = need - taken;
                     const int take2 = avail2 < rest ? avail2 : rest;
@@ -297,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void kt_trace_kernel(TraceArgs A) {
                 }
             }
             if (mine >= 0) {
-                frame = A.single ? 0 : find_frame(A.frames, A.n_frames, mine);
+                frame = SINGLE ? 0 : find_frame(A.frames, A.n_frames, mine);
                 const FrameDev& F = A.frames[frame];
                 const int64_t loc = mine - F.ray_base;
                 const int64_t tile = loc >> 6;
@@ -305,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void kt_trace_kernel(TraceArgs A) {
                 px = (int)(tile % F.tiles_x) * 8 + (w & 7);
                 py = (int)(tile / F.tiles_x) * 8 + (w >> 3);
                 if (px < F.W && py < F.H) {
-                    if (!A.single) { P = F.P; Pf = knds_cast<float>(P); }
+                    if (!SINGLE) { P = F.P; Pf = knds_cast<float>(P); }
                     ray_init<CHART, PREC>(F, A.O, px, py, S);
                     active = true;
                 }
@@ -316,35 +319,42 @@ __global__ __launch_bounds__(kBlock) void kt_trace_kernel(TraceArgs A) {
             continue;
         }
         if (active) {
-            const FrameDev& F = A.frames[A.single ? 0 : frame];
+            const FrameDev& F = A.frames[SINGLE ? 0 : frame];
             bool use64 = true;
             if (PREC == PREC_MIXED) use64 = WaveVote::any(wants_fp64(F, A.O, S));
-            const bool done = ray_advance<CHART, METHOD, PREC, WaveVote>(F, P, Pf, A.O, use64, S, o);
-            if (done) {
+            if (ray_advance<CHART, METHOD, PREC, PHI>(F, P, Pf, A.O, use64, S, o)) {
                 uint32_t wv[7];
                 pack_geopixel(o, wv);
                 uint32_t* dst = A.geo + 7 * (F.pix_base + (int64_t)py * F.W + px);
                 for (int i = 0; i < 7; ++i) dst[i] = wv[i];
-                c_acc += (unsigned long long)S.n_acc;
-                c_rej += (unsigned long long)S.rej_total;
-                c_r32 += (unsigned long long)S.rhs32;
-                c_r64 += (unsigned long long)S.rhs64;
-                c_herm += (unsigned long long)S.hermites;
-                c_rays += 1ull;
+                c_acc += (uint32_t)S.n_acc;
+                c_rej += (uint32_t)S.rej_total;
+                c_r32 += (uint32_t)S.rhs32;
+                c_r64 += (uint32_t)S.rhs64;
+                c_herm += (uint32_t)S.hermites;
+                c_rays += 1u;
                 active = false;
             }
         }
     }
-    c_acc = wave_sum(c_acc); c_rej = wave_sum(c_rej); c_r32 = wave_sum(c_r32);
-    c_r64 = wave_sum(c_r64); c_herm = wave_sum(c_herm); c_rays = wave_sum(c_rays);
+    const unsigned long long t_acc = wave_sum(c_acc), t_rej = wave_sum(c_rej), t_r32 = wave_sum(c_r32);
+    const unsigned long long t_r64 = wave_sum(c_r64), t_herm = wave_sum(c_herm), t_rays = wave_sum(c_rays);
     if (lane == 0) {
-        atomicAdd(&A.words[W_ACC], c_acc);
-        atomicAdd(&A.words[W_REJ], c_rej);
-        atomicAdd(&A.words[W_RHS32], c_r32);
-        atomicAdd(&A.words[W_RHS64], c_r64);
-        atomicAdd(&A.words[W_HERM], c_herm);
-        atomicAdd(&A.words[W_RAYS], c_rays);
+        atomicAdd(&A.words[W_ACC], t_acc);
+        atomicAdd(&A.words[W_REJ], t_rej);
+        atomicAdd(&A.words[W_RHS32], t_r32);
+        atomicAdd(&A.words[W_RHS64], t_r64);
+        atomicAdd(&A.words[W_HERM], t_herm);
+        atomicAdd(&A.words[W_RAYS], t_rays);
     }
+}
+
+// OCC > 0 caps the register budget so that OCC waves fit per SIMD (amdgpu_waves_per_eu)
+template <int CHART, int METHOD, int PREC, bool PHI, int OCC>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
+void kt_trace_kernel(TraceArgs A) {
+    if (A.single) trace_loop<CHART, METHOD, PREC, PHI, true>(A);
+    else trace_loop<CHART, METHOD, PREC, PHI, false>(A);
 }
 
 // ---------------------------------------------------------------------------
@@ -387,19 +397,29 @@ __global__ __launch_bounds__(256) void kt_shade_kernel(ShadeArgs A) {
 
 typedef void (*TraceKernelFn)(TraceArgs);
 
-TraceKernelFn pick_kernel(int chart, int method, int prec) {
-    const int key = chart*4 + method*2 + prec;
-    switch (key) {
-        case 0: return kt_trace_kernel<CHART_BL, METHOD_RK4, PREC_FP64>;
-        case 1: return kt_trace_kernel<CHART_BL, METHOD_RK4, PREC_MIXED>;
-        case 2: return kt_trace_kernel<CHART_BL, METHOD_DP5, PREC_FP64>;
-        case 3: return kt_trace_kernel<CHART_BL, METHOD_DP5, PREC_MIXED>;
-        case 4: return kt_trace_kernel<CHART_KS, METHOD_RK4, PREC_FP64>;
-        case 5: return kt_trace_kernel<CHART_KS, METHOD_RK4, PREC_MIXED>;
-        case 6: return kt_trace_kernel<CHART_KS, METHOD_DP5, PREC_FP64>;
-        case 7: return kt_trace_kernel<CHART_KS, METHOD_DP5, PREC_MIXED>;
+template <int CHART, int OCC>
+TraceKernelFn pick_kernel_chart(int method, int prec, bool phi) {
+    if (method == METHOD_RK4)
+        return prec ? kt_trace_kernel<CHART, METHOD_RK4, PREC_MIXED, true, OCC>
+                    : kt_trace_kernel<CHART, METHOD_RK4, PREC_FP64, true, OCC>;
+    if (phi)
+        return prec ? kt_trace_kernel<CHART, METHOD_DP5, PREC_MIXED, true, OCC>
+                    : kt_trace_kernel<CHART, METHOD_DP5, PREC_FP64, true, OCC>;
+    return prec ? kt_trace_kernel<CHART, METHOD_DP5, PREC_MIXED, false, OCC>
+                : kt_trace_kernel<CHART, METHOD_DP5, PREC_FP64, false, OCC>;
+}
+
+template <int OCC>
+TraceKernelFn pick_kernel_occ(int chart, int method, int prec, bool phi) {
+    return chart == CHART_KS ? pick_kernel_chart<CHART_KS, OCC>(method, prec, phi)
+                             : pick_kernel_chart<CHART_BL, OCC>(method, prec, phi);
+}
+
+TraceKernelFn pick_kernel(int chart, int method, int prec, bool phi, int occ) {
+    switch (occ) {
+        case 1: return pick_kernel_occ<0>(chart, method, prec, phi);   // compiler's own register budget
+        default: return pick_kernel_occ<2>(chart, method, prec, phi);  // measured best on gfx950
     }
-    return nullptr;
 }
 
 }  // namespace
@@ -635,14 +655,13 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     int chart = 0;
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
-    TraceKernelFn fn = pick_kernel(chart, o->integrator, o->precision);
+    const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
+    TraceKernelFn fn = pick_kernel(chart, o->integrator, o->precision, phi, o->tuning_occupancy);
     if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");
 
-    // persistent grid: as many resident workgroups as the kernel's register use allows
-    int per_cu = 0;
-    KT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kBlock, 0));
-    if (per_cu < 1) per_cu = 1;
-    if (o->tuning_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, o->tuning_waves_per_cu / (kBlock / 64)));
+    // persistent grid: blocks beyond the resident limit simply start when others retire
+    // and drain the queue tail, so over-subscription is harmless
+    int per_cu = o->tuning_blocks_per_cu > 0 ? o->tuning_blocks_per_cu : kDefaultBlocksPerCu;
     int64_t grid = (int64_t)c->n_cu * per_cu;
     const int64_t max_useful = (rays + kBlock - 1) / kBlock;
     if (grid > max_useful) grid = max_useful;
@@ -656,6 +675,7 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     A.geo = c->d_geo;
     A.words = c->d_words;
     A.O = run_opts(o);
+    A.chunk = o->tuning_chunk > 0 ? o->tuning_chunk : kDefaultChunk;
 
     hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
     KT_HIP(c, hipEventRecord(c->ev[0], c->stream));

@@ -293,7 +293,7 @@ struct RayState {
     double y[4];           // r, theta, p_r, p_theta
     double ph, pt, pp;     // phi (phi~ in KS), conserved p_t (p_v), p_phi
     double h;
-    double k1[5];          // RHS at y (FSAL / first stage)
+    double k1[5];          // RHS at y (FSAL / first stage); [4] = dphi/dl
     SinCos<double> sc;     // sin, cos of y[1]
     int32_t n_acc, n_rej;
     int32_t rhs32, rhs64, hermites, rej_total;
@@ -305,37 +305,39 @@ struct LaneVote {
     KT_HD static bool any(bool b) { return b; }
 };
 
-template <int CHART, class Vote>
+// stage derivative at y0 + d (NC = 4: r, theta, p_r, p_theta; NC = 5: + phi)
+template <int CHART, int NC>
 KT_HD void stage_f64(const Knds<double>& P, const double y0[4], const double d[4], const SinCos<double>& sc0,
-                     double pt, double pp, double k[5], SinCos<double>& sc) {
+                     double pt, double pp, double k[NC], SinCos<double>& sc) {
     // per lane (not a wave vote): a ray's result must not depend on its wave-mates
     if (fabs(d[1]) <= kRotateMax) sc = rotate(sc0, d[1]);
     else kt_sincos(y0[1] + d[1], &sc.s, &sc.c);
-    rhs<CHART>(P, y0[0] + d[0], sc, y0[2] + d[2], y0[3] + d[3], pt, pp, k);
+    double o[5];
+    rhs<CHART>(P, y0[0] + d[0], sc, y0[2] + d[2], y0[3] + d[3], pt, pp, o);
+    for (int i = 0; i < NC; ++i) k[i] = o[i];
 }
 
-template <int CHART, class Vote>
+template <int CHART, int NC>
 KT_HD void stage_f32(const Knds<float>& Pf, const double y0[4], const double d[4], const SinCos<double>& sc0,
-                     double pt, double pp, double k[5]) {
+                     double pt, double pp, double k[NC]) {
     SinCos<float> scf;
-    const float d1 = (float)d[1];
-    if (fabs(d[1]) <= kRotateMax) scf = rotate(SinCos<float>{(float)sc0.s, (float)sc0.c}, d1);
+    if (fabs(d[1]) <= kRotateMax) scf = rotate(SinCos<float>{(float)sc0.s, (float)sc0.c}, (float)d[1]);
     else kt_sincos((float)(y0[1] + d[1]), &scf.s, &scf.c);
     float kf[5];
     rhs<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
                (float)pt, (float)pp, kf);
-    for (int i = 0; i < 5; ++i) k[i] = (double)kf[i];
+    for (int i = 0; i < NC; ++i) k[i] = (double)kf[i];
 }
 
-template <int CHART, int PREC, class Vote>
+template <int CHART, int PREC, int NC>
 KT_HD void stage(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4], const double d[4],
-                 const SinCos<double>& sc0, double pt, double pp, double k[5], RayState& S) {
+                 const SinCos<double>& sc0, double pt, double pp, double k[NC], RayState& S) {
     if (PREC == PREC_FP64 || use64) {
         SinCos<double> sc;
-        stage_f64<CHART, Vote>(P, y0, d, sc0, pt, pp, k, sc);
+        stage_f64<CHART, NC>(P, y0, d, sc0, pt, pp, k, sc);
         S.rhs64++;
     } else {
-        stage_f32<CHART, Vote>(Pf, y0, d, sc0, pt, pp, k);
+        stage_f32<CHART, NC>(Pf, y0, d, sc0, pt, pp, k);
         S.rhs32++;
     }
 }
@@ -350,91 +352,6 @@ KT_HD double pow_neg(double err, float q) {
 #endif
 }
 
-// DOPRI5 attempt.  Returns true if accepted; on acceptance y_prev/k_prev keep
-// the start of the step (for events) and S.k1 holds f(y_new) (FSAL).
-template <int CHART, int PREC, class Vote>
-KT_HD bool dp5_attempt(const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64, bool phi,
-                       RayState& S, double y_prev[4], double& ph_prev, double k_prev[5], double& h_used) {
-    typedef DP5 C;
-    const double h = S.h;
-    h_used = h;
-    const double* y = S.y;
-    double k2[5], k3[5], k4[5], k5[5], k6[5], k7[5];
-    const double* k1 = S.k1;
-    double d[4];
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
-    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2, S);
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
-    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3, S);
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
-    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4, S);
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
-    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5, S);
-    for (int i = 0; i < 4; ++i)
-        d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
-    stage<CHART, PREC, Vote>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6, S);
-    double d5[4];
-    for (int i = 0; i < 4; ++i)
-        d5[i] = h*(C::b1*k1[i] + C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i] + C::b6*k6[i]);
-    // FSAL stage at y5: always in the stage precision; its sin/cos becomes the next base
-    SinCos<double> sc7;
-    if (PREC == PREC_FP64 || use64) {
-        stage_f64<CHART, Vote>(P, y, d5, S.sc, S.pt, S.pp, k7, sc7);
-        S.rhs64++;
-    } else {
-        stage_f32<CHART, Vote>(Pf, y, d5, S.sc, S.pt, S.pp, k7);
-        S.rhs32++;
-        if (fabs(d5[1]) <= kRotateMax) sc7 = rotate(S.sc, d5[1]);
-        else kt_sincos(y[1] + d5[1], &sc7.s, &sc7.c);
-    }
-    double e2 = 0.0;
-    for (int i = 0; i < 4; ++i) {
-        const double ev = h*(C::e1*k1[i] + C::e3*k3[i] + C::e4*k4[i] + C::e5*k5[i] + C::e6*k6[i] + C::e7*k7[i]);
-        const double q = ev / (O.atol + O.rtol*fabs(y[i]));
-        e2 += q*q;
-    }
-    const double err = sqrt(e2 * 0.25);
-    if (!isfinite(err)) {
-        double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
-        S.h = hn < 1e-10 ? 1e-10 : hn;
-        return false;
-    }
-    if (err <= 1.0 || h < 1e-10) {
-        for (int i = 0; i < 4; ++i) { y_prev[i] = S.y[i]; k_prev[i] = S.k1[i]; }
-        k_prev[4] = S.k1[4];
-        ph_prev = S.ph;
-        if (phi) S.ph += h*(C::b1*k1[4] + C::b3*k3[4] + C::b4*k4[4] + C::b5*k5[4] + C::b6*k6[4]);
-        for (int i = 0; i < 4; ++i) S.y[i] += d5[i];
-        for (int i = 0; i < 5; ++i) S.k1[i] = k7[i];
-        S.sc = sc7;
-        double hn = h * ((err > 1e-12) ? 0.9*pow_neg(err, 0.2f) : 4.0);
-        if (!isfinite(hn)) hn = h;
-        S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
-        return true;
-    }
-    double hn = h * 0.9 * pow_neg(err, 0.25f);
-    if (!isfinite(hn)) hn = h*0.5;
-    S.h = hn < 1e-10 ? 1e-10 : hn;
-    return false;
-}
-
-// Classic RK4 over [0, h] from y0 with f(y0) = k0; returns the increment (5 comps,
-// phi last) and, through scy, the sin/cos at the end point.
-template <int CHART, int PREC, class Vote>
-KT_HD void rk4_incr(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
-                    const SinCos<double>& sc0, const double k0[5], double h, double pt, double pp,
-                    double inc[5], RayState& S) {
-    double k2[5], k3[5], k4[5], d[4];
-    for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k0[i];
-    stage<CHART, PREC, Vote>(P, Pf, use64, y0, d, sc0, pt, pp, k2, S);
-    for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k2[i];
-    stage<CHART, PREC, Vote>(P, Pf, use64, y0, d, sc0, pt, pp, k3, S);
-    for (int i = 0; i < 4; ++i) d[i] = h*k3[i];
-    stage<CHART, PREC, Vote>(P, Pf, use64, y0, d, sc0, pt, pp, k4, S);
-    for (int i = 0; i < 5; ++i) inc[i] = h/6.0*(k0[i] + 2.0*k2[i] + 2.0*k3[i] + k4[i]);
-}
-
-template <int CHART, int PREC, class Vote>
 KT_HD SinCos<double> sc_at(const SinCos<double>& sc0, double th0, double dth) {
     SinCos<double> o;
     if (fabs(dth) <= kRotateMax) o = rotate(sc0, dth);
@@ -442,58 +359,75 @@ KT_HD SinCos<double> sc_at(const SinCos<double>& sc0, double th0, double dth) {
     return o;
 }
 
-// RK4 step doubling attempt (geodesic.hpp:204-257).  On acceptance S.k1 is
-// refreshed to f(y_new) (needed by the events and the next attempt).
-template <int CHART, int PREC, class Vote>
-KT_HD bool rk4_attempt(const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
-                       RayState& S, double y_prev[4], double& ph_prev, double k_prev[5], double& h_used) {
-    const double h = S.h;
-    h_used = h;
-    const double tol = O.rtol;
-    double incA[5], incB1[5], incB2[5];
-    rk4_incr<CHART, PREC, Vote>(P, Pf, use64, S.y, S.sc, S.k1, h, S.pt, S.pp, incA, S);
-    rk4_incr<CHART, PREC, Vote>(P, Pf, use64, S.y, S.sc, S.k1, 0.5*h, S.pt, S.pp, incB1, S);
-    double yB1[4];
-    for (int i = 0; i < 4; ++i) yB1[i] = S.y[i] + incB1[i];
-    const SinCos<double> scB1 = sc_at<CHART, PREC, Vote>(S.sc, S.y[1], incB1[1]);
-    double kB1[5];
-    {
-        const double z[4] = {0.0, 0.0, 0.0, 0.0};
-        stage<CHART, PREC, Vote>(P, Pf, use64, yB1, z, scB1, S.pt, S.pp, kB1, S);
+// ---------------------------------------------------------------------------
+//  events after an accepted step (main.cpp:363-431)
+// ---------------------------------------------------------------------------
+// The reference's KS tracer reports azimuths through atan2 (KS_to_BL_spatial,
+// knds_metric.hpp:389-411), i.e. in (-pi, pi]; BL reports the integrated phi.
+template <int CHART> KT_HD double out_phi(double ph) {
+    if (CHART == CHART_KS) { double s, c; kt_sincos(ph, &s, &c); return atan2(s, c); }
+    return ph;
+}
+
+// Step from (r0, th0, ph0) with derivatives (dr0, dth0) to the new state S over h.
+// Returns true if the ray is finished (o filled).
+template <int CHART>
+KT_HD bool step_events(const FrameDev& F, double r0, double th0, double ph0, double dr0, double dth0,
+                       const RayState& S, double hu, uint32_t quirks, RayOut& o, int& hermites) {
+    const double halfpi = kPi / 2.0;
+    const double r1 = S.y[0], th1 = S.y[1], dr1 = S.k1[0], dth1 = S.k1[1];
+    double best = 2.0;
+    int ev = 0;
+    double d_r = 0.0, d_g = 1.0, d_ph = 0.0;
+    const double q0 = th0 - halfpi, q1 = th1 - halfpi;
+    const bool near_eq = crosses(q0, q1) || (fmin(fabs(q0), fabs(q1)) < 0.35);
+    if (near_eq && hermite_may_touch(th0, th1, dth0, dth1, hu, halfpi)) {
+        hermites++;
+        double al = 0.0;
+        if (hermite_first_root(th0, th1, dth0, dth1, hu, halfpi, al)) {
+            const double rh = hermite(r0, r1, dr0, dr1, hu, al);
+            if (rh >= F.r_in && rh <= F.r_out) {
+                d_r = rh;
+                d_g = disk_redshift(F, rh, S.pt, S.pp);
+                d_ph = out_phi<CHART>(ph0 + al*(S.ph - ph0));
+                best = al;
+                ev = 1;
+            }
+        }
     }
-    rk4_incr<CHART, PREC, Vote>(P, Pf, use64, yB1, scB1, kB1, 0.5*h, S.pt, S.pp, incB2, S);
-    double yA[4], yB[4];
-    for (int i = 0; i < 4; ++i) { yA[i] = S.y[i] + incA[i]; yB[i] = yB1[i] + incB2[i]; }
-    double e2 = 0.0;
-    for (int i = 0; i < 4; ++i) { const double t = yA[i] - yB[i]; e2 += t*t; }
-    const double err = sqrt(e2) / 15.0;
-    if (!isfinite(err)) {
-        double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
-        S.h = hn < 1e-10 ? 1e-10 : hn;
-        return false;
+    if (((r0 > F.r_cut) && (r1 <= F.r_cut)) || (r1 <= F.r_cut)) {
+        const double dn = r0 - r1;
+        double ah = (fabs(dn) > 1e-12) ? ((r0 - F.r_cut) / dn) : 0.0;
+        ah = ah < 0.0 ? 0.0 : (ah > 1.0 ? 1.0 : ah);
+        if (ah < best) { best = ah; ev = 2; }
     }
-    if (err < tol || h < 1e-10) {
-        for (int i = 0; i < 4; ++i) { y_prev[i] = S.y[i]; k_prev[i] = S.k1[i]; }
-        k_prev[4] = S.k1[4];
-        ph_prev = S.ph;
-        S.ph = S.ph + incB1[4] + incB2[4];
-        const double dth = (yB1[1] - S.y[1]) + incB2[1];
-        for (int i = 0; i < 4; ++i) S.y[i] = yB[i];
-        S.sc = sc_at<CHART, PREC, Vote>(S.sc, y_prev[1], dth);
-        double hn = h * ((err > 1e-14) ? 0.9*pow_neg(err / tol, 0.2f) : 4.0);
-        if (!isfinite(hn)) hn = h;
-        S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
-        // f(y_new), always binary64 in FP64 mode; stage precision otherwise
-        const double z[4] = {0.0, 0.0, 0.0, 0.0};
-        stage<CHART, PREC, Vote>(P, Pf, use64, S.y, z, S.sc, S.pt, S.pp, S.k1, S);
-        return true;
+    if (((r0 < F.r_esc) && (r1 >= F.r_esc)) || (r1 >= F.r_esc)) {
+        const double dn = r1 - r0;
+        double ae = (fabs(dn) > 1e-12) ? ((F.r_esc - r0) / dn) : 1.0;
+        ae = ae < 0.0 ? 0.0 : (ae > 1.0 ? 1.0 : ae);
+        if (ae < best) { best = ae; ev = 3; }
     }
-    const double half = h*0.5;
-    double hn = h * 0.9 * pow_neg(err / tol, 0.25f);
-    if (!isfinite(hn)) hn = half;
-    if (hn > half) hn = half;
-    S.h = hn < 1e-10 ? 1e-10 : hn;
-    return false;
+    if (ev == 0) return false;
+    o.ph_disk = o.th_esc = o.ph_esc = 0.0;
+    if (ev == 1) { o.fate = 1; o.r = d_r; o.g = d_g; o.ph_disk = d_ph; }
+    else if (ev == 2) { o.fate = 2; o.r = r0 + best*(r1 - r0); o.g = 0.0; }
+    else {
+        o.fate = 0; o.r = F.r_esc; o.g = 1.0;
+        const double te = th0 + best*(th1 - th0);
+        const double pe = out_phi<CHART>(ph0 + best*(S.ph - ph0));
+        if (quirks & Q_ESC_POSITIONAL) { o.ph_disk = te; o.th_esc = pe; o.ph_esc = 0.0; }
+        else { o.th_esc = te; o.ph_esc = pe; }
+    }
+    return true;
+}
+
+// result of a ray stopped by the reject cap or max_steps (main.cpp:360,433)
+template <int CHART>
+KT_HD void stopped_result(const RayState& S, uint32_t quirks, RayOut& o) {
+    o.fate = 0; o.r = S.y[0]; o.g = 1.0;
+    const double ph = out_phi<CHART>(S.ph);
+    if (quirks & Q_ESC_POSITIONAL) { o.ph_disk = S.y[1]; o.th_esc = ph; o.ph_esc = 0.0; }
+    else { o.ph_disk = 0.0; o.th_esc = S.y[1]; o.ph_esc = ph; }
 }
 
 // Constraint renormalisation (binary64) for the mixed-precision path: one Newton
@@ -531,72 +465,164 @@ KT_HD void renormalise(const Knds<double>& P, RayState& S) {
     S.y[3] = pth - f*gt;
 }
 
-// Event test after an accepted step (main.cpp:363-431).  Returns true if the ray
-// is finished (o filled).
-// The reference's KS tracer reports azimuths through atan2 (KS_to_BL_spatial,
-// knds_metric.hpp:389-411), i.e. in (-pi, pi]; BL reports the integrated phi.
-template <int CHART> KT_HD double out_phi(double ph) {
-    if (CHART == CHART_KS) { double s, c; kt_sincos(ph, &s, &c); return atan2(s, c); }
-    return ph;
+enum { STEP_REJECTED = 0, STEP_ACCEPTED = 1, STEP_DONE = 2 };
+
+// bookkeeping after an accepted step: constraint projection, events, max_steps
+template <int CHART, int PREC>
+KT_HD int after_accept(const FrameDev& F, const Knds<double>& P, const RunOpts& O, RayState& S, double r0,
+                       double th0, double ph0, double dr0, double dth0, double hu, RayOut& o) {
+    S.n_rej = 0;
+    S.n_acc++;
+    if (PREC == PREC_MIXED) renormalise<CHART>(P, S);
+    if (step_events<CHART>(F, r0, th0, ph0, dr0, dth0, S, hu, O.quirks, o, S.hermites)) return STEP_DONE;
+    if (S.n_acc >= O.max_steps) { stopped_result<CHART>(S, O.quirks, o); return STEP_DONE; }
+    return STEP_ACCEPTED;
 }
 
-template <int CHART>
-KT_HD bool step_events(const FrameDev& F, const RayState& S, const double yp[4], double php,
-                       const double kp[5], double hu, uint32_t quirks, RayOut& o, int& hermites) {
-    const double halfpi = kPi / 2.0;
-    double best = 2.0;
-    int ev = 0;
-    double d_r = 0.0, d_g = 1.0, d_ph = 0.0;
-    const double q0 = yp[1] - halfpi, q1 = S.y[1] - halfpi;
-    const bool near_eq = crosses(q0, q1) || (fmin(fabs(q0), fabs(q1)) < 0.35);
-    if (near_eq && hermite_may_touch(yp[1], S.y[1], kp[1], S.k1[1], hu, halfpi)) {
-        hermites++;
-        double al = 0.0;
-        if (hermite_first_root(yp[1], S.y[1], kp[1], S.k1[1], hu, halfpi, al)) {
-            const double rh = hermite(yp[0], S.y[0], kp[0], S.k1[0], hu, al);
-            if (rh >= F.r_in && rh <= F.r_out) {
-                d_r = rh;
-                d_g = disk_redshift(F, rh, S.pt, S.pp);
-                d_ph = out_phi<CHART>(php + al*(S.ph - php));
-                best = al;
-                ev = 1;
-            }
-        }
-    }
-    const double r = S.y[0];
-    if (((yp[0] > F.r_cut) && (r <= F.r_cut)) || (r <= F.r_cut)) {
-        const double dn = yp[0] - r;
-        double ah = (fabs(dn) > 1e-12) ? ((yp[0] - F.r_cut) / dn) : 0.0;
-        ah = ah < 0.0 ? 0.0 : (ah > 1.0 ? 1.0 : ah);
-        if (ah < best) { best = ah; ev = 2; }
-    }
-    if (((yp[0] < F.r_esc) && (r >= F.r_esc)) || (r >= F.r_esc)) {
-        const double dn = r - yp[0];
-        double ae = (fabs(dn) > 1e-12) ? ((F.r_esc - yp[0]) / dn) : 1.0;
-        ae = ae < 0.0 ? 0.0 : (ae > 1.0 ? 1.0 : ae);
-        if (ae < best) { best = ae; ev = 3; }
-    }
-    if (ev == 0) return false;
-    o.ph_disk = o.th_esc = o.ph_esc = 0.0;
-    if (ev == 1) { o.fate = 1; o.r = d_r; o.g = d_g; o.ph_disk = d_ph; }
-    else if (ev == 2) { o.fate = 2; o.r = yp[0] + best*(r - yp[0]); o.g = 0.0; }
-    else {
-        o.fate = 0; o.r = F.r_esc; o.g = 1.0;
-        const double te = yp[1] + best*(S.y[1] - yp[1]);
-        const double pe = out_phi<CHART>(php + best*(S.ph - php));
-        if (quirks & Q_ESC_POSITIONAL) { o.ph_disk = te; o.th_esc = pe; o.ph_esc = 0.0; }
-        else { o.th_esc = te; o.ph_esc = pe; }
-    }
-    return true;
+KT_HD int after_reject(const RunOpts& O, RayState& S) {
+    S.rej_total++;
+    if (!isfinite(S.h) || ++S.n_rej > 64) return STEP_DONE;
+    return STEP_REJECTED;
 }
 
-// result of a ray stopped by the reject cap or max_steps (main.cpp:360,433)
-template <int CHART>
-KT_HD void stopped_result(const RayState& S, uint32_t quirks, RayOut& o) {
-    o.fate = 0; o.r = S.y[0]; o.g = 1.0;
-    const double ph = out_phi<CHART>(S.ph);
-    if (quirks & Q_ESC_POSITIONAL) { o.ph_disk = S.y[1]; o.th_esc = ph; o.ph_esc = 0.0; }
-    else { o.ph_disk = 0.0; o.th_esc = S.y[1]; o.ph_esc = ph; }
+// ---------------------------------------------------------------------------
+//  DOPRI5 attempt (geodesic.hpp:312-414).  The 5th-order update and the embedded
+//  error are accumulated as the stages arrive so k2..k6 die early (register
+//  economy); NC = 5 also advances phi with the same weights.
+// ---------------------------------------------------------------------------
+template <int CHART, int PREC, int NC>
+KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
+                   RayState& S, RayOut& o) {
+    typedef DP5 C;
+    const double h = S.h;
+    const double* y = S.y;
+    const double* k1 = S.k1;
+    double k2[NC], k3[NC], k4[NC], k5[NC], k6[NC];
+    double d[4], a5[NC], ae[4];
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
+    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2, S);
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
+    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3, S);
+    for (int i = 0; i < NC; ++i) a5[i] = C::b1*k1[i] + C::b3*k3[i];
+    for (int i = 0; i < 4; ++i) ae[i] = C::e1*k1[i] + C::e3*k3[i];
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
+    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4, S);
+    for (int i = 0; i < NC; ++i) a5[i] += C::b4*k4[i];
+    for (int i = 0; i < 4; ++i) ae[i] += C::e4*k4[i];
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
+    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5, S);
+    for (int i = 0; i < NC; ++i) a5[i] += C::b5*k5[i];
+    for (int i = 0; i < 4; ++i) ae[i] += C::e5*k5[i];
+    for (int i = 0; i < 4; ++i)
+        d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
+    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6, S);
+    for (int i = 0; i < NC; ++i) a5[i] += C::b6*k6[i];
+    for (int i = 0; i < 4; ++i) ae[i] += C::e6*k6[i];
+    for (int i = 0; i < 4; ++i) d[i] = h*a5[i];
+    // FSAL stage at y5 in the stage precision; its sin/cos becomes the next base
+    double k7[5];
+    SinCos<double> sc7;
+    if (PREC == PREC_FP64 || use64) {
+        stage_f64<CHART, 5>(P, y, d, S.sc, S.pt, S.pp, k7, sc7);
+        S.rhs64++;
+    } else {
+        stage_f32<CHART, 5>(Pf, y, d, S.sc, S.pt, S.pp, k7);
+        S.rhs32++;
+        sc7 = sc_at(S.sc, y[1], d[1]);
+    }
+    // scaled RMS of the embedded error; the accept test needs ~3 digits, so the
+    // quotients are formed in binary32 (reciprocal, no IEEE division sequence)
+    float e2 = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        const float num = (float)(h*(ae[i] + C::e7*k7[i]));
+        const float q = num * kt_rcp((float)(O.atol + O.rtol*fabs(y[i])));
+        e2 = fmaf(q, q, e2);
+    }
+    const double err = (double)sqrtf(e2 * 0.25f);
+    if (!isfinite(err)) {
+        const double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
+        S.h = hn < 1e-10 ? 1e-10 : hn;
+        return after_reject(O, S);
+    }
+    if (err <= 1.0 || h < 1e-10) {
+        const double r0 = y[0], th0 = y[1], ph0 = S.ph, dr0 = k1[0], dth0 = k1[1];
+        if (NC == 5) S.ph += h*a5[NC - 1];
+        for (int i = 0; i < 4; ++i) S.y[i] += d[i];
+        for (int i = 0; i < 5; ++i) S.k1[i] = k7[i];
+        S.sc = sc7;
+        double hn = h * ((err > 1e-12) ? 0.9*pow_neg(err, 0.2f) : 4.0);
+        if (!isfinite(hn)) hn = h;
+        S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
+        return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o);
+    }
+    double hn = h * 0.9 * pow_neg(err, 0.25f);
+    if (!isfinite(hn)) hn = h*0.5;
+    S.h = hn < 1e-10 ? 1e-10 : hn;
+    return after_reject(O, S);
+}
+
+// Classic RK4 over [0, h] from y0 with f(y0) = k0 (geodesic.hpp:169-194): returns
+// the increment of (r, theta, p_r, p_theta, phi).
+template <int CHART, int PREC>
+KT_HD void rk4_incr(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
+                    const SinCos<double>& sc0, const double k0[5], double h, double pt, double pp,
+                    double inc[5], RayState& S) {
+    double k2[5], k3[5], k4[5], d[4];
+    for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k0[i];
+    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k2, S);
+    for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k2[i];
+    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k3, S);
+    for (int i = 0; i < 4; ++i) d[i] = h*k3[i];
+    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k4, S);
+    for (int i = 0; i < 5; ++i) inc[i] = h/6.0*(k0[i] + 2.0*k2[i] + 2.0*k3[i] + k4[i]);
+}
+
+// RK4 step doubling attempt (geodesic.hpp:204-257); phi is always advanced.
+template <int CHART, int PREC>
+KT_HD int rk4_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
+                   RayState& S, RayOut& o) {
+    const double h = S.h;
+    const double tol = O.rtol;
+    double incA[5], incB1[5], incB2[5];
+    rk4_incr<CHART, PREC>(P, Pf, use64, S.y, S.sc, S.k1, h, S.pt, S.pp, incA, S);
+    rk4_incr<CHART, PREC>(P, Pf, use64, S.y, S.sc, S.k1, 0.5*h, S.pt, S.pp, incB1, S);
+    double yB1[4];
+    for (int i = 0; i < 4; ++i) yB1[i] = S.y[i] + incB1[i];
+    const SinCos<double> scB1 = sc_at(S.sc, S.y[1], incB1[1]);
+    double kB1[5];
+    {
+        const double z[4] = {0.0, 0.0, 0.0, 0.0};
+        stage<CHART, PREC, 5>(P, Pf, use64, yB1, z, scB1, S.pt, S.pp, kB1, S);
+    }
+    rk4_incr<CHART, PREC>(P, Pf, use64, yB1, scB1, kB1, 0.5*h, S.pt, S.pp, incB2, S);
+    double e2 = 0.0;
+    for (int i = 0; i < 4; ++i) { const double t = (S.y[i] + incA[i]) - (yB1[i] + incB2[i]); e2 += t*t; }
+    const double err = sqrt(e2) / 15.0;
+    if (!isfinite(err)) {
+        const double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
+        S.h = hn < 1e-10 ? 1e-10 : hn;
+        return after_reject(O, S);
+    }
+    if (err < tol || h < 1e-10) {
+        const double r0 = S.y[0], th0 = S.y[1], ph0 = S.ph, dr0 = S.k1[0], dth0 = S.k1[1];
+        S.ph = S.ph + incB1[4] + incB2[4];
+        const double dth = (yB1[1] - S.y[1]) + incB2[1];
+        for (int i = 0; i < 4; ++i) S.y[i] = yB1[i] + incB2[i];
+        S.sc = sc_at(S.sc, th0, dth);
+        double hn = h * ((err > 1e-14) ? 0.9*pow_neg(err / tol, 0.2f) : 4.0);
+        if (!isfinite(hn)) hn = h;
+        S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
+        // f(y_new): event derivatives and first stage of the next attempt
+        const double z[4] = {0.0, 0.0, 0.0, 0.0};
+        stage<CHART, PREC, 5>(P, Pf, use64, S.y, z, S.sc, S.pt, S.pp, S.k1, S);
+        return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o);
+    }
+    const double half = h*0.5;
+    double hn = h * 0.9 * pow_neg(err / tol, 0.25f);
+    if (!isfinite(hn)) hn = half;
+    if (hn > half) hn = half;
+    S.h = hn < 1e-10 ? 1e-10 : hn;
+    return after_reject(O, S);
 }
 
 // Initialise a ray at pixel (px, py) of frame F.
@@ -641,25 +667,16 @@ KT_HD void pack_geopixel(const RayOut& o, uint32_t w[7]) {
 }
 
 // One attempt + bookkeeping for an active ray; returns true when the ray is done.
-template <int CHART, int METHOD, int PREC, class Vote>
+// PHI: DOPRI5 advances phi (false reproduces the reference's frozen phi).
+template <int CHART, int METHOD, int PREC, bool PHI>
 KT_HD bool ray_advance(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O,
                        bool use64, RayState& S, RayOut& o) {
-    double yp[4], kp[5], php, hu;
-    const bool phi = !(METHOD == METHOD_DP5 && (O.quirks & Q_DP5_PHI_FROZEN));
-    bool ok;
-    if (METHOD == METHOD_DP5) ok = dp5_attempt<CHART, PREC, Vote>(P, Pf, O, use64, phi, S, yp, php, kp, hu);
-    else ok = rk4_attempt<CHART, PREC, Vote>(P, Pf, O, use64, S, yp, php, kp, hu);
-    if (!ok) {
-        S.rej_total++;
-        if (!isfinite(S.h) || ++S.n_rej > 64) { stopped_result<CHART>(S, O.quirks, o); return true; }
-        return false;
-    }
-    S.n_rej = 0;
-    S.n_acc++;
-    if (PREC == PREC_MIXED) renormalise<CHART>(P, S);
-    if (step_events<CHART>(F, S, yp, php, kp, hu, O.quirks, o, S.hermites)) return true;
-    if (S.n_acc >= O.max_steps) { stopped_result<CHART>(S, O.quirks, o); return true; }
-    return false;
+    int rc;
+    if (METHOD == METHOD_DP5) rc = dp5_step<CHART, PREC, PHI ? 5 : 4>(F, P, Pf, O, use64, S, o);
+    else rc = rk4_step<CHART, PREC>(F, P, Pf, O, use64, S, o);
+    if (rc != STEP_DONE) return false;
+    if (S.n_rej > 64 || !isfinite(S.h)) stopped_result<CHART>(S, O.quirks, o);   // reject cap
+    return true;
 }
 
 }  // namespace kt

@@ -30,7 +30,7 @@ static void fill_frame(const kt_frame& f, const kt_opts& o, FrameDev& F) {
     F.r_in = F.r_isco; F.r_out = F.disk_out; F.r_esc = F.r_obs * 1.05; F.r_cut = F.r_h * 1.03;
 }
 
-template <int CHART, int METHOD, int PREC>
+template <int CHART, int METHOD, int PREC, bool PHI>
 static void run(const FrameDev& F, const RunOpts& O, const int32_t* px, const int32_t* py, int64_t n,
                 kt_geopixel* out, int64_t* acc, int64_t* rhs32, int64_t* rhs64) {
     const Knds<float> Pf = knds_cast<float>(F.P);
@@ -41,7 +41,7 @@ static void run(const FrameDev& F, const RunOpts& O, const int32_t* px, const in
         ray_init<CHART, PREC>(F, O, px[i], py[i], S);
         for (;;) {
             const bool use64 = (PREC == PREC_FP64) ? true : wants_fp64(F, O, S);
-            if (ray_advance<CHART, METHOD, PREC, LaneVote>(F, F.P, Pf, O, use64, S, o)) break;
+            if (ray_advance<CHART, METHOD, PREC, PHI>(F, F.P, Pf, O, use64, S, o)) break;
         }
         uint32_t w[7];
         pack_geopixel(o, w);
@@ -49,6 +49,21 @@ static void run(const FrameDev& F, const RunOpts& O, const int32_t* px, const in
         if (acc) acc[i] = S.n_acc;
         if (rhs32) rhs32[i] = S.rhs32;
         if (rhs64) rhs64[i] = S.rhs64;
+    }
+}
+
+template <int CHART>
+static void run_chart(int method, int prec, bool phi, const FrameDev& F, const RunOpts& O, const int32_t* px,
+                      const int32_t* py, int64_t n, kt_geopixel* out, int64_t* acc, int64_t* r32, int64_t* r64) {
+    if (method == METHOD_RK4) {
+        if (prec) run<CHART, METHOD_RK4, PREC_MIXED, true>(F, O, px, py, n, out, acc, r32, r64);
+        else run<CHART, METHOD_RK4, PREC_FP64, true>(F, O, px, py, n, out, acc, r32, r64);
+    } else if (phi) {
+        if (prec) run<CHART, METHOD_DP5, PREC_MIXED, true>(F, O, px, py, n, out, acc, r32, r64);
+        else run<CHART, METHOD_DP5, PREC_FP64, true>(F, O, px, py, n, out, acc, r32, r64);
+    } else {
+        if (prec) run<CHART, METHOD_DP5, PREC_MIXED, false>(F, O, px, py, n, out, acc, r32, r64);
+        else run<CHART, METHOD_DP5, PREC_FP64, false>(F, O, px, py, n, out, acc, r32, r64);
     }
 }
 
@@ -65,18 +80,9 @@ extern "C" int ktc_trace_pixels(const kt_frame* f, const kt_opts* o, const int32
     O.isl_noise = o->mixed_noise > 0.0 ? o->mixed_noise : kMixedNoiseDefault;
     O.isl_r = o->mixed_r_island > 0.0 ? o->mixed_r_island : kMixedRIslandDefault;
     if (info5) { info5[0] = F.r_h; info5[1] = F.r_isco; info5[2] = F.r_esc; info5[3] = F.chart; info5[4] = F.ks_ref_init; }
-    const int key = F.chart*4 + o->integrator*2 + o->precision;
-    switch (key) {
-        case 0: run<CHART_BL, METHOD_RK4, PREC_FP64>(F, O, px, py, n, out, acc, rhs32, rhs64); break;
-        case 1: run<CHART_BL, METHOD_RK4, PREC_MIXED>(F, O, px, py, n, out, acc, rhs32, rhs64); break;
-        case 2: run<CHART_BL, METHOD_DP5, PREC_FP64>(F, O, px, py, n, out, acc, rhs32, rhs64); break;
-        case 3: run<CHART_BL, METHOD_DP5, PREC_MIXED>(F, O, px, py, n, out, acc, rhs32, rhs64); break;
-        case 4: run<CHART_KS, METHOD_RK4, PREC_FP64>(F, O, px, py, n, out, acc, rhs32, rhs64); break;
-        case 5: run<CHART_KS, METHOD_RK4, PREC_MIXED>(F, O, px, py, n, out, acc, rhs32, rhs64); break;
-        case 6: run<CHART_KS, METHOD_DP5, PREC_FP64>(F, O, px, py, n, out, acc, rhs32, rhs64); break;
-        case 7: run<CHART_KS, METHOD_DP5, PREC_MIXED>(F, O, px, py, n, out, acc, rhs32, rhs64); break;
-        default: return -1;
-    }
+    const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
+    if (F.chart == CHART_KS) run_chart<CHART_KS>(o->integrator, o->precision, phi, F, O, px, py, n, out, acc, rhs32, rhs64);
+    else run_chart<CHART_BL>(o->integrator, o->precision, phi, F, O, px, py, n, out, acc, rhs32, rhs64);
     return 0;
 }
 

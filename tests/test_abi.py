@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     assert ctypes.sizeof(_abi.KtFrame) == 11 * 8 + 2 * 4
-    assert ctypes.sizeof(_abi.KtOpts) == 6 * 4 + 6 * 8 + 2 * 4 + 2 * 8
+    assert ctypes.sizeof(_abi.KtOpts) == 6 * 4 + 6 * 8 + 4 * 4 + 2 * 8
     assert _abi.GEOPIXEL_DTYPE.itemsize == 28
 
 
