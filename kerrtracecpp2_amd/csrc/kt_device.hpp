@@ -153,23 +153,25 @@ template <class T, class S> KT_HD Knds<T> knds_cast(const Knds<S>& k) {
 // Arithmetic: one reciprocal of Sigma * Dr * (Dt s^2) (BL) / Sigma * Dt s^2 (KS).
 enum { CHART_BL = 0, CHART_KS = 1 };
 
-template <int CHART, class T>
-KT_HD void rhs(const Knds<T>& P, T r, const SinCos<T>& sc, T pr, T pth, T pt, T pp, T out[5]) {
+// T: type of the state variables (double, float, or the dual numbers of the
+// bundle kernel); S: type of the spacetime constants and the conserved p_t, p_phi.
+template <int CHART, class T, class S = T>
+KT_HD void rhs(const Knds<S>& P, T r, const SinCos<T>& sc, T pr, T pth, S pt, S pp, T out[5]) {
     const T s = sc.s, c = sc.c;
     const T s2 = s*s, c2 = c*c, sc2 = s*c;
     const T r2 = r*r;
     const T ra = r2 + P.a2;
     const T sig = fma(P.a2, c2, r2);
-    const T one_m = fma(-P.lam3, r2, (T)1);                 // 1 - Lambda r^2/3
-    const T dR = fma(ra, one_m, fma(-P.M2, r, P.Q2));       // Delta_r
+    const T one_m = fma(-P.lam3, r2, S(1));                  // 1 - Lambda r^2/3
+    const T dR = fma(ra, one_m, fma(-P.M2, r, P.Q2));        // Delta_r
     // dDelta_r/dr = 2r(1 - L r^2/3) - (r^2+a^2) 2 L r/3 - 2M
-    const T dR_r = fma((T)2*r, one_m, fma(-(T)2*P.lam3*r, ra, -P.M2));
-    const T dT = fma(P.lam3a2, c2, (T)1);                   // Delta_theta
-    const T dT_th = -(T)2*P.lam3a2*sc2;                      // dDelta_theta/dtheta
+    const T dR_r = fma(S(2)*r, one_m, fma((S(-2)*P.lam3)*r, ra, -P.M2));
+    const T dT = fma(P.lam3a2, c2, S(1));                    // Delta_theta
+    const T dT_th = (S(-2)*P.lam3a2)*sc2;                     // dDelta_theta/dtheta
     const T K = fma(ra, pt, P.a*pp);
     const T L = fma(P.a*s2, pt, pp);
-    const T Kr = (T)2*r*pt;                                  // dK/dr
-    const T Lth = (T)2*P.a*sc2*pt;                           // dL/dtheta
+    const T Kr = (S(2)*pt)*r;                                 // dK/dr
+    const T Lth = (S(2)*P.a*pt)*sc2;                          // dL/dtheta
     const T dTs2 = dT*s2;
     if (CHART == CHART_BL) {
         const T p12 = sig*dR;
@@ -177,35 +179,35 @@ KT_HD void rhs(const Knds<T>& P, T r, const SinCos<T>& sc, T pr, T pth, T pt, T 
         const T inv_sig = inv*dR*dTs2;
         const T inv_dR = inv*sig*dTs2;
         const T inv_ts = inv*p12;
-        const T A1 = P.xi2*K*inv_dR;                         // Xi^2 K / Dr
-        const T B1 = P.xi2*L*inv_ts;                         // Xi^2 L / (Dt s^2)
+        const T A1 = P.xi2*K*inv_dR;                          // Xi^2 K / Dr
+        const T B1 = P.xi2*L*inv_ts;                          // Xi^2 L / (Dt s^2)
         const T N = fma(dR*pr, pr, fma(dT*pth, pth, fma(-A1, K, B1*L)));
         out[0] = dR*pr*inv_sig;
         out[1] = dT*pth*inv_sig;
         out[4] = fma(-A1, P.a, B1)*inv_sig;
         // dN/dr = dR_r pr^2 - 2 A1 Kr + A1 K dR_r / Dr
-        const T dNr = fma(dR_r*pr, pr, fma(-(T)2*A1, Kr, A1*K*inv_dR*dR_r));
+        const T dNr = fma(dR_r*pr, pr, fma(S(-2)*A1, Kr, A1*K*inv_dR*dR_r));
         // dN/dth = dT_th pth^2 + 2 B1 Lth - B1 L (dT_th s^2 + 2 Dt s c)/(Dt s^2)
-        const T dNth = fma(dT_th*pth, pth, fma((T)2*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, (T)2*dT*sc2)));
-        const T hs = (T)0.5*inv_sig;
-        out[2] = -fma(-N*(T)2*r, inv_sig, dNr)*hs;
-        out[3] = -fma(N*(T)2*P.a2*sc2, inv_sig, dNth)*hs;
+        const T dNth = fma(dT_th*pth, pth, fma(S(2)*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, S(2)*dT*sc2)));
+        const T hs = S(0.5)*inv_sig;
+        out[2] = -fma(-N*(S(2)*r), inv_sig, dNr)*hs;
+        out[3] = -fma(N*(S(2)*P.a2)*sc2, inv_sig, dNth)*hs;
     } else {
         const T inv = kt_rcp(sig*dTs2);
         const T inv_sig = inv*dTs2;
         const T inv_ts = inv*sig;
-        const T B1 = P.xi2*L*inv_ts;                         // Xi^2 L / (Dt s^2)
+        const T B1 = P.xi2*L*inv_ts;                          // Xi^2 L / (Dt s^2)
         const T XK = P.xi*K;
-        const T N = fma(dR*pr, pr, fma((T)2*XK, pr, fma(dT*pth, pth, B1*L)));
+        const T N = fma(dR*pr, pr, fma(S(2)*XK, pr, fma(dT*pth, pth, B1*L)));
         out[0] = fma(dR, pr, XK)*inv_sig;
         out[1] = dT*pth*inv_sig;
         out[4] = fma(P.xi*P.a, pr, B1)*inv_sig;
         // dN/dr = dR_r pr^2 + 2 Xi pr Kr
-        const T dNr = fma(dR_r*pr, pr, (T)2*P.xi*pr*Kr);
-        const T dNth = fma(dT_th*pth, pth, fma((T)2*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, (T)2*dT*sc2)));
-        const T hs = (T)0.5*inv_sig;
-        out[2] = -fma(-N*(T)2*r, inv_sig, dNr)*hs;
-        out[3] = -fma(N*(T)2*P.a2*sc2, inv_sig, dNth)*hs;
+        const T dNr = fma(dR_r*pr, pr, (S(2)*P.xi)*pr*Kr);
+        const T dNth = fma(dT_th*pth, pth, fma(S(2)*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, S(2)*dT*sc2)));
+        const T hs = S(0.5)*inv_sig;
+        out[2] = -fma(-N*(S(2)*r), inv_sig, dNr)*hs;
+        out[3] = -fma(N*(S(2)*P.a2)*sc2, inv_sig, dNth)*hs;
     }
 }
 

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/kt_hip.h"
+#include "kt_bundle.hpp"
 #include "kt_setup.hpp"
 #include "kt_shade.hpp"
 #include "kt_trace.hpp"
@@ -256,12 +257,42 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
-template <int CHART, int METHOD, int PREC, bool PHI, bool SINGLE>
+// Per-ray state machines driven by the persistent loop.
+template <int CHART, int METHOD, int PREC, bool PHI>
+struct GeodesicPolicy {
+    typedef RayState State;
+    __device__ static void init(const FrameDev& F, const RunOpts& O, int px, int py, State& S) {
+        ray_init<CHART, PREC>(F, O, px, py, S);
+    }
+    __device__ static bool advance(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf,
+                                   const RunOpts& O, State& S, RayOut& o, double& magnif) {
+        bool use64 = true;
+        if (PREC == PREC_MIXED) use64 = WaveVote::any(wants_fp64(F, O, S));
+        magnif = 1.0;
+        return ray_advance<CHART, METHOD, PREC, PHI>(F, P, Pf, O, use64, S, o);
+    }
+    __device__ static const RayState& geo(const State& S) { return S; }
+};
+
+struct BundlePolicy {
+    typedef BundleState State;
+    __device__ static void init(const FrameDev& F, const RunOpts& O, int px, int py, State& B) {
+        bundle_init(F, O, px, py, B);
+    }
+    __device__ static bool advance(const FrameDev& F, const Knds<double>&, const Knds<float>&, const RunOpts& O,
+                                   State& B, RayOut& o, double& magnif) {
+        return bundle_step(F, O, B, o, magnif) == STEP_DONE;
+    }
+    __device__ static const RayState& geo(const State& B) { return B.g; }
+};
+
+template <class Policy, bool SINGLE>
 __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    RayState S;
+    typename Policy::State S;
     RayOut o;
+    double magnif = 1.0;
     bool active = false;
     int frame = 0;
     int px = 0, py = 0;
@@ -309,7 +340,7 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
                 py = (int)(tile / F.tiles_x) * 8 + (w >> 3);
                 if (px < F.W && py < F.H) {
                     if (!SINGLE) { P = F.P; Pf = knds_cast<float>(P); }
-                    ray_init<CHART, PREC>(F, A.O, px, py, S);
+                    Policy::init(F, A.O, px, py, S);
                     active = true;
                 }
             }
@@ -320,18 +351,17 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
         }
         if (active) {
             const FrameDev& F = A.frames[SINGLE ? 0 : frame];
-            bool use64 = true;
-            if (PREC == PREC_MIXED) use64 = WaveVote::any(wants_fp64(F, A.O, S));
-            if (ray_advance<CHART, METHOD, PREC, PHI>(F, P, Pf, A.O, use64, S, o)) {
+            if (Policy::advance(F, P, Pf, A.O, S, o, magnif)) {
                 uint32_t wv[7];
-                pack_geopixel(o, wv);
+                pack_geopixel_m(o, magnif, wv);
                 uint32_t* dst = A.geo + 7 * (F.pix_base + (int64_t)py * F.W + px);
                 for (int i = 0; i < 7; ++i) dst[i] = wv[i];
-                c_acc += (uint32_t)S.n_acc;
-                c_rej += (uint32_t)S.rej_total;
-                c_r32 += (uint32_t)S.rhs32;
-                c_r64 += (uint32_t)S.rhs64;
-                c_herm += (uint32_t)S.hermites;
+                const RayState& G = Policy::geo(S);
+                c_acc += (uint32_t)G.n_acc;
+                c_rej += (uint32_t)G.rej_total;
+                c_r32 += (uint32_t)G.rhs32;
+                c_r64 += (uint32_t)G.rhs64;
+                c_herm += (uint32_t)G.hermites;
                 c_rays += 1u;
                 active = false;
             }
@@ -353,8 +383,16 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
 template <int CHART, int METHOD, int PREC, bool PHI, int OCC>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
 void kt_trace_kernel(TraceArgs A) {
-    if (A.single) trace_loop<CHART, METHOD, PREC, PHI, true>(A);
-    else trace_loop<CHART, METHOD, PREC, PHI, false>(A);
+    typedef GeodesicPolicy<CHART, METHOD, PREC, PHI> Pol;
+    if (A.single) trace_loop<Pol, true>(A);
+    else trace_loop<Pol, false>(A);
+}
+
+// Jacobi ray bundles: BL chart, RK4 step doubling, binary64 (ray_bundle.hpp)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void kt_bundle_kernel(TraceArgs A) {
+    if (A.single) trace_loop<BundlePolicy, true>(A);
+    else trace_loop<BundlePolicy, false>(A);
 }
 
 // ---------------------------------------------------------------------------
@@ -560,8 +598,6 @@ static int check_opts(kt_hip_ctx* c, const kt_opts* o) {
         return fail(c, KT_E_ARG, "bad integrator %d", o->integrator);
     if (o->precision != KT_PRECISION_FP64 && o->precision != KT_PRECISION_MIXED)
         return fail(c, KT_E_ARG, "bad precision %d", o->precision);
-    if (o->bundles) return fail(c, KT_E_UNSUPPORTED, "ray bundles are provided by kt_hip_render with bundles=1 "
-                                                     "only in builds with the bundle kernel");
     if (!(o->rtol > 0.0) || !(o->atol > 0.0)) return fail(c, KT_E_ARG, "tolerances must be positive");
     return KT_OK;
 }
@@ -569,6 +605,7 @@ static int check_opts(kt_hip_ctx* c, const kt_opts* o) {
 // Effective chart of a frame: the reference runs KS only for Lambda == 0
 // (main.cpp:2034-2041); KT_QUIRK_KS_LAMBDA_TO_BL reproduces that fallback.
 static int frame_chart(const kt_opts* o, const kt_frame& f) {
+    if (o->bundles) return KT_CHART_BL;          // bundles run in BL (main.cpp:2042-2049)
     if (o->chart == KT_CHART_KS && (o->quirks & KT_QUIRK_KS_LAMBDA_TO_BL) && std::fabs(f.Lambda) > 1e-15)
         return KT_CHART_BL;
     return o->chart;
@@ -656,7 +693,8 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
     const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
-    TraceKernelFn fn = pick_kernel(chart, o->integrator, o->precision, phi, o->tuning_occupancy);
+    TraceKernelFn fn = o->bundles ? kt_bundle_kernel
+                                  : pick_kernel(chart, o->integrator, o->precision, phi, o->tuning_occupancy);
     if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");
 
     // persistent grid: blocks beyond the resident limit simply start when others retire

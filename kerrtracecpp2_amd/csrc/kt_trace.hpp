@@ -72,10 +72,14 @@ struct RayOut {
 // ---------------------------------------------------------------------------
 //  camera: pixel -> BL state with p_t re-solved from H = 0   camera.hpp:38-145
 // ---------------------------------------------------------------------------
-KT_HD void camera_ray(const FrameDev& F, double px, double py, double st[7]) {
+KT_HD void pixel_angles(const FrameDev& F, double px, double py, double& al, double& be) {
     const int span = (F.W > 1) ? (F.W - 1) : 1;
-    const double al = F.fov * (px + F.off_x - 0.5*(F.W-1)) / span;
-    const double be = F.fov * (0.5*(F.H-1) - (py + F.off_y)) / span;
+    al = F.fov * (px + F.off_x - 0.5*(F.W-1)) / span;
+    be = F.fov * (0.5*(F.H-1) - (py + F.off_y)) / span;
+}
+
+// screen angles (alpha, beta) -> BL state (camera.hpp:50-145)
+KT_HD void camera_shoot(const FrameDev& F, double al, double be, double st[7]) {
     double sa, ca, sb, cb;
     kt_sincos(al, &sa, &ca);
     kt_sincos(be, &sb, &cb);
@@ -107,6 +111,12 @@ KT_HD void camera_ray(const FrameDev& F, double px, double py, double st[7]) {
     }
     st[0] = F.r_obs; st[1] = F.th_obs; st[2] = F.ph_obs;
     st[3] = pr; st[4] = pth; st[5] = pt; st[6] = pp;
+}
+
+KT_HD void camera_ray(const FrameDev& F, double px, double py, double st[7]) {
+    double al, be;
+    pixel_angles(F, px, py, al, be);
+    camera_shoot(F, al, be, st);
 }
 
 // BL state -> ingoing Kerr-Schild covector (v, r, theta, phi~), same event point.
@@ -664,6 +674,13 @@ KT_HD void pack_geopixel(const RayOut& o, uint32_t w[7]) {
     cv.f = (float)o.ph_disk; w[4] = cv.u;
     cv.f = (float)o.th_esc; w[5] = cv.u;
     cv.f = (float)o.ph_esc; w[6] = cv.u;
+}
+
+KT_HD void pack_geopixel_m(const RayOut& o, double magnif, uint32_t w[7]) {
+    pack_geopixel(o, w);
+    union { float f; uint32_t u; } cv;
+    cv.f = (float)magnif;
+    w[3] = cv.u;
 }
 
 // One attempt + bookkeeping for an active ray; returns true when the ray is done.

@@ -30,6 +30,7 @@ def lib():
         L.ktc_shade.argtypes = [P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_double, P, ctypes.c_int, ctypes.c_int, P]
         L.ktc_isco_exact.argtypes = [ctypes.c_double] * 4 + [P, P]
+        L.ktc_trace_bundle_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
         _lib = L
     return _lib
 
@@ -67,3 +68,13 @@ def isco_exact(M, a, Q, Lam):
     seg = np.zeros(1)
     lib().ktc_isco_exact(M, a, Q, Lam, _p(seq), _p(seg))
     return float(seq[0]), float(seg[0])
+
+
+def trace_bundle_pixels(frame, opts, px, py):
+    px = np.ascontiguousarray(px, np.int32)
+    py = np.ascontiguousarray(py, np.int32)
+    out = np.zeros(px.size, _abi.GEOPIXEL_DTYPE)
+    acc = np.zeros(px.size, np.int64)
+    assert lib().ktc_trace_bundle_pixels(ctypes.byref(frame), ctypes.byref(opts), _p(px), _p(py), px.size, _p(out),
+                                         _p(acc)) == 0
+    return out, acc

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "../../include/kt_hip.h"
+#include "../../kerrtracecpp2_amd/csrc/kt_bundle.hpp"
 #include "../../kerrtracecpp2_amd/csrc/kt_setup.hpp"
 #include "../../kerrtracecpp2_amd/csrc/kt_shade.hpp"
 #include "../../kerrtracecpp2_amd/csrc/kt_trace.hpp"
@@ -110,4 +111,32 @@ extern "C" int ktc_isco_exact(double M, double a, double Q, double Lambda, doubl
         if (!(rk < 30.0 * M)) { *seg_out = rk; return ns; }
         if (isco_L(F, isco_abscissa(seg, ns, k + 1)) < isco_L(F, rk) && isco_L(F, rk) > 0.0) { *seg_out = rk; return ns; }
     }
+}
+
+extern "C" int ktc_trace_bundle_pixels(const kt_frame* f, const kt_opts* o, const int32_t* px, const int32_t* py,
+                                       int64_t n, kt_geopixel* out, int64_t* acc) {
+    kt_opts ob = *o;
+    ob.bundles = 1;
+    FrameDev F;
+    fill_frame(*f, ob, F);
+    F.chart = CHART_BL;
+    RunOpts O;
+    O.max_steps = o->max_steps < 1 ? 1 : o->max_steps;
+    O.quirks = o->quirks;
+    O.step_init = o->step_init < 1e-10 ? 1e-10 : o->step_init;
+    O.rtol = o->rtol; O.atol = o->atol;
+    O.isl_noise = kMixedNoiseDefault; O.isl_r = kMixedRIslandDefault;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 0; i < n; ++i) {
+        BundleState B;
+        RayOut ro;
+        double mag = 1.0;
+        bundle_init(F, O, px[i], py[i], B);
+        while (bundle_step(F, O, B, ro, mag) != STEP_DONE) {}
+        uint32_t w[7];
+        pack_geopixel_m(ro, mag, w);
+        std::memcpy(&out[i], w, 28);
+        if (acc) acc[i] = B.g.n_acc;
+    }
+    return 0;
 }

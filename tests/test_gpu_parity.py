@@ -227,3 +227,23 @@ def test_invalid_arguments_are_reported(gpu_ctx):
     o.chart = 7
     with pytest.raises(kt.KtError):
         gpu_ctx.render([kt.make_frame(4, 4, a=0.5)], o)
+
+
+def test_bundles_vs_reference(gpu_ctx):
+    """Configuration-3 style Jacobi bundles (a=0.99, theta=89, BL, RK4 doubling) at 64x36 against
+    the reference binary: fate per pixel, r_hit, Doppler factor, magnification within 10 %
+    (the reference's own sensitivity to its tolerance, see tests/test_hostcheck.py)."""
+    g = load_golden("bundles_bl_64x36")["pixels"]
+    W, H = 64, 36
+    o = _opts(bundles=1)
+    out = gpu_ctx.render([kt.make_frame(W, H, a=0.99, theta=89, r_obs=40, disk_out=12)], o)
+    geo = out["geo"]
+    band = photon_ring_band(g, W, H)
+    keep = ~band
+    assert (geo["outcome"] == g["outcome"])[keep].mean() >= 0.999
+    d = keep & (geo["outcome"] == 1) & (g["outcome"] == 1)
+    assert np.abs(geo["r"][d] - g["r"][d]).max() < 2e-3
+    assert np.abs(geo["redshift"][d] - g["redshift"][d]).max() < 5e-4
+    rel = np.abs(geo["magnif"][d] / g["magnif"][d] - 1.0)
+    assert np.mean(rel < 0.10) >= 0.99
+    assert out["info"][0]["eff_chart"] == 0

@@ -105,3 +105,41 @@ def test_max_steps_one_positional_results():
     np.testing.assert_allclose(got["r"], ref["r"], rtol=1e-6)
     np.testing.assert_allclose(got["phi_disk"], ref["phi_disk"], atol=1e-6)   # theta
     np.testing.assert_allclose(got["theta_esc"], ref["theta_esc"], atol=1e-6)  # phi
+
+
+BUNDLE_PTS = [(3, 4), (20, 4), (40, 18), (32, 18), (50, 10), (5, 30), (33, 12), (30, 24)]
+
+
+def test_bundles_vs_reference_sample():
+    """Jacobi bundles (dual-number J_f W) vs the reference's FD-Hessian bundles: same fate, r_hit,
+    Doppler factor; magnification |det J| within 10 % -- the reference's own value at tol 1e-7
+    moves by 9 % when the tolerance is tightened to 1e-8 (W is not error controlled)."""
+    g = load_golden("bundles_bl_64x36")["pixels"].reshape(36, 64)
+    px = np.array([p[0] for p in BUNDLE_PTS])
+    py = np.array([p[1] for p in BUNDLE_PTS])
+    op = _abi.default_opts()
+    op.bundles = 1
+    got, _ = hc.trace_bundle_pixels(_abi.make_frame(64, 36, a=0.99, theta=89, r_obs=40, disk_out=12), op, px, py)
+    ref = g[py, px]
+    assert np.array_equal(got["outcome"], ref["outcome"])
+    d = ref["outcome"] == 1
+    assert np.abs(got["r"][d] - ref["r"][d]).max() < 1e-3
+    assert np.abs(got["redshift"][d] - ref["redshift"][d]).max() < 1e-4
+    assert np.all(np.abs(got["magnif"][d] / ref["magnif"][d] - 1.0) < 0.10)
+    e = ~d
+    np.testing.assert_allclose(got["theta_esc"][e], ref["theta_esc"][e], atol=1e-4)
+    np.testing.assert_allclose(got["phi_esc"][e], ref["phi_esc"][e], atol=1e-4)
+
+
+def test_bundle_magnification_converged_vs_oracle():
+    """At tol 1e-9 both magnifications are converged: 1 % agreement."""
+    pts = [(20, 4), (40, 18), (32, 18), (50, 10)]
+    px = np.array([p[0] for p in pts])
+    py = np.array([p[1] for p in pts])
+    ref = kto.trace_pixels(kto.make_frame(64, 36, a=0.99, theta=89, r_obs=40, disk_out=12, bundles=True, tol=1e-9),
+                           px, py)
+    op = _abi.default_opts()
+    op.bundles, op.rtol, op.atol = 1, 1e-9, 1e-9
+    got, _ = hc.trace_bundle_pixels(_abi.make_frame(64, 36, a=0.99, theta=89, r_obs=40, disk_out=12), op, px, py)
+    assert np.array_equal(got["outcome"], ref["outcome"])
+    assert np.all(np.abs(got["magnif"] / ref["magnif"] - 1.0) < 0.01)
