@@ -160,6 +160,42 @@ int kt_hip_frame_info(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, con
 int kt_make_checker_background(uint64_t seed, int32_t width, int32_t height, int32_t cells_x,
                                int32_t cells_y, uint8_t* rgb_out);
 
+/* --camera-spp (render_image's multi-pass branch, main.cpp:2232-2322).
+ * Every frame is expanded into spp passes whose sub-pixel offsets (kt_spp_offsets) are added to the
+ * frame's own offsets; all passes of all frames are traced in ONE persistent launch, shaded, and
+ * averaged on the device in linear light: mean_s (c_s/255)^gamma, back through ^(1/gamma) and
+ * round-to-nearest, gamma = max(opts->gamma, 1e-6).  The result equals the reference's host
+ * resolve bit for bit (the display rounding is evaluated through per-level thresholds computed on
+ * the host with the same pow).  rgb_out: sum(W*H*3) bytes, frames back to back.               */
+int kt_hip_render_spp(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, int spp, const kt_opts* opts,
+                      uint8_t* rgb_out, kt_stats* stats);
+
+/* Host utilities (no device work; callable without a GPU). */
+/* Offsets of the spp jittered passes, (x, y) pairs in pixels (build_offsets, main.cpp:2240-2279). */
+int kt_spp_offsets(int32_t spp, double* xy_out);
+/* Host resolve of n_pass packed-ABGR passes (pass-major) into RGB8, same rule as kt_hip_render_spp. */
+int kt_spp_resolve(const uint32_t* passes, int32_t n_pass, int64_t n_pix, double gamma, uint8_t* rgb_out);
+/* The 512-entry table kt_hip_render_spp uploads: [0,256) (v/255)^gamma, [256,512) the least x in
+ * [0,1] whose display level lround(255 x^(1/gamma)) is >= k ([256] = -inf).                      */
+int kt_spp_tables(double gamma, double* tab_out);
+
+/* .kgeo container (save_kgeo / load_kgeo, main.cpp:96-137): "KGEO", uint32 version 1, this
+ * 88-byte header, then W*H GeoPixel records.  theta/phi in degrees as the reference writes them. */
+typedef struct kt_kgeo_meta {
+    uint32_t W, H;
+    double M, a, Q, Lambda;
+    double r_isco, r_disk_in, r_disk_out;
+    double theta_deg, phi_deg, r_obs;
+} kt_kgeo_meta;
+int kt_save_kgeo(const char* path, const kt_kgeo_meta* meta, const kt_geopixel* geo);
+/* geo_out may be NULL to read the header only; geo_cap is its capacity in records. */
+int kt_load_kgeo(const char* path, kt_kgeo_meta* meta, kt_geopixel* geo_out, int64_t geo_cap);
+/* 8-bit RGB PNG writer (stbi_write_png's role at main.cpp:2481-2484). */
+int kt_write_png(const char* path, const uint8_t* rgb, int32_t width, int32_t height);
+/* Background reader: PNG (8-bit, non-interlaced) or binary PPM -> RGB8 (stbi_load's role at
+ * main.cpp:145-152; JPEG is not read).  rgb_out may be NULL to query the size.                */
+int kt_read_image_rgb(const char* path, uint8_t* rgb_out, int64_t cap_bytes, int32_t* width, int32_t* height);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,6 +115,18 @@ class Context:
                     "kt_hip_frame_info")
         return [_info_dict(i) for i in info]
 
+    def render_spp(self, frames, spp, opts):
+        """camera-spp: every frame as spp jittered passes in one launch, resolved on the device.
+        Returns dict(rgb=(sum W*H, 3) uint8, stats)."""
+        frames = list(frames)
+        arr = (_abi.KtFrame * len(frames))(*frames)
+        npix = sum(f.width * f.height for f in frames)
+        rgb = np.zeros((npix, 3), np.uint8)
+        st = _abi.KtStats()
+        self._check(self.lib.kt_hip_render_spp(self.h, arr, len(frames), int(spp), ctypes.byref(opts), _ptr(rgb),
+                                               ctypes.byref(st)), "kt_hip_render_spp")
+        return {"rgb": rgb, "stats": st.as_dict()}
+
     def colorize(self, geo, M, r_isco, opts):
         geo = np.ascontiguousarray(geo, dtype=GEOPIXEL_DTYPE)
         rgba = np.zeros(geo.size, np.uint32)
@@ -141,3 +153,70 @@ def rgba_to_rgb(rgba):
     """ABGR uint32 (R in bits 0-7) -> (n, 3) uint8, as main.cpp:2438-2443 unpacks it."""
     rgba = np.asarray(rgba, dtype=np.uint32)
     return np.stack([(rgba & 0xFF), (rgba >> 8) & 0xFF, (rgba >> 16) & 0xFF], axis=-1).astype(np.uint8)
+
+
+def spp_offsets(spp):
+    """(spp, 2) sub-pixel offsets of the jittered passes (main.cpp:2240-2279)."""
+    out = np.zeros((int(spp), 2), np.float64)
+    if load_library().kt_spp_offsets(int(spp), _ptr(out)) != 0:
+        raise KtError("kt_spp_offsets failed")
+    return out
+
+
+def spp_resolve(passes_abgr, gamma=2.2):
+    """Host resolve of (n_pass, n_pix) ABGR passes -> (n_pix, 3) uint8."""
+    p = np.ascontiguousarray(passes_abgr, dtype=np.uint32)
+    out = np.zeros((p.shape[1], 3), np.uint8)
+    if load_library().kt_spp_resolve(_ptr(p), p.shape[0], p.shape[1], float(gamma), _ptr(out)) != 0:
+        raise KtError("kt_spp_resolve failed")
+    return out
+
+
+def spp_tables(gamma=2.2):
+    """(256,) linear table and (256,) display thresholds used by the device resolve."""
+    tab = np.zeros(512, np.float64)
+    if load_library().kt_spp_tables(float(gamma), _ptr(tab)) != 0:
+        raise KtError("kt_spp_tables failed")
+    return tab[:256], tab[256:]
+
+
+def save_kgeo(path, geo, meta):
+    """Write a .kgeo file; meta: dict with W, H, M, a, Q, Lambda, r_isco, r_disk_in, r_disk_out,
+    theta_deg, phi_deg, r_obs."""
+    geo = np.ascontiguousarray(geo, dtype=GEOPIXEL_DTYPE)
+    m = _abi.KtKgeoMeta(**meta)
+    if geo.size != m.W * m.H:
+        raise KtError("geo size does not match W*H")
+    if load_library().kt_save_kgeo(os.fsencode(path), ctypes.byref(m), _ptr(geo)) != 0:
+        raise KtError(f"cannot write {path}")
+
+
+def load_kgeo(path):
+    """Read a .kgeo file -> (geo, meta dict)."""
+    lib = load_library()
+    m = _abi.KtKgeoMeta()
+    if lib.kt_load_kgeo(os.fsencode(path), ctypes.byref(m), None, 0) != 0:
+        raise KtError(f"cannot read {path}")
+    geo = np.zeros(m.W * m.H, GEOPIXEL_DTYPE)
+    if lib.kt_load_kgeo(os.fsencode(path), ctypes.byref(m), _ptr(geo), geo.size) != 0:
+        raise KtError(f"cannot read {path}")
+    return geo, {k: getattr(m, k) for k, _ in m._fields_}
+
+
+def write_png(path, rgb):
+    """rgb: (h, w, 3) uint8."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    if load_library().kt_write_png(os.fsencode(path), _ptr(rgb), rgb.shape[1], rgb.shape[0]) != 0:
+        raise KtError(f"cannot write {path}")
+
+
+def read_image_rgb(path):
+    """PNG / binary PPM -> (h, w, 3) uint8."""
+    lib = load_library()
+    w, h = ctypes.c_int32(), ctypes.c_int32()
+    if lib.kt_read_image_rgb(os.fsencode(path), None, 0, ctypes.byref(w), ctypes.byref(h)) != 0:
+        raise KtError(f"cannot read {path}")
+    out = np.zeros((h.value, w.value, 3), np.uint8)
+    if lib.kt_read_image_rgb(os.fsencode(path), _ptr(out), out.size, ctypes.byref(w), ctypes.byref(h)) != 0:
+        raise KtError(f"cannot read {path}")
+    return out

@@ -49,6 +49,13 @@ class KtFrameInfo(ctypes.Structure):
                 ("eff_chart", ctypes.c_int32), ("eff_integrator", ctypes.c_int32)]
 
 
+class KtKgeoMeta(ctypes.Structure):
+    _fields_ = [("W", ctypes.c_uint32), ("H", ctypes.c_uint32), ("M", ctypes.c_double), ("a", ctypes.c_double),
+                ("Q", ctypes.c_double), ("Lambda", ctypes.c_double), ("r_isco", ctypes.c_double),
+                ("r_disk_in", ctypes.c_double), ("r_disk_out", ctypes.c_double), ("theta_deg", ctypes.c_double),
+                ("phi_deg", ctypes.c_double), ("r_obs", ctypes.c_double)]
+
+
 class KtStats(ctypes.Structure):
     _fields_ = [("trace_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("rays", ctypes.c_uint64),
                 ("steps_accepted", ctypes.c_uint64), ("steps_rejected", ctypes.c_uint64),
@@ -74,6 +81,14 @@ PROTOTYPES = {
     "kt_hip_frame_info": (ctypes.c_int, [_P, _P, _I, _P, _P]),
     "kt_make_checker_background": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                    ctypes.c_int32, _P]),
+    "kt_hip_render_spp": (ctypes.c_int, [_P, _P, _I, _I, _P, _P, _P]),
+    "kt_spp_offsets": (ctypes.c_int, [ctypes.c_int32, _P]),
+    "kt_spp_resolve": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64, _D, _P]),
+    "kt_spp_tables": (ctypes.c_int, [_D, _P]),
+    "kt_save_kgeo": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),
+    "kt_load_kgeo": (ctypes.c_int, [ctypes.c_char_p, _P, _P, ctypes.c_int64]),
+    "kt_write_png": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32]),
+    "kt_read_image_rgb": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int64, _P, _P]),
 }
 
 

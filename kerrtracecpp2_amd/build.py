@@ -23,7 +23,7 @@ def build_lib():
     src = os.path.join(PKG, "csrc", "kt_hip.hip")
     out = os.path.join(PKG, "libkt_hip.so")
     tmp = out + ".tmp"
-    run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src, "-lz"])
     os.replace(tmp, out)
     return out
 

@@ -25,6 +25,7 @@
 
 #include "../../include/kt_hip.h"
 #include "kt_bundle.hpp"
+#include "kt_image.hpp"
 #include "kt_setup.hpp"
 #include "kt_shade.hpp"
 #include "kt_trace.hpp"
@@ -433,6 +434,50 @@ __global__ __launch_bounds__(256) void kt_shade_kernel(ShadeArgs A) {
     A.rgba[i] = shade_pixel((uint8_t)(g[0] & 0xFF), r.f, red.f, mag.f, th.f, ph.f, p, A.bg, A.bw, A.bh);
 }
 
+// ---------------------------------------------------------------------------
+//  camera-spp resolve: mean of the passes in linear light, back to display
+// ---------------------------------------------------------------------------
+struct ResolveArgs {
+    const FrameDev* frames;  // expanded table: pass s of logical frame f at index f*spp + s
+    int32_t n_logical, spp;
+    int64_t n_out;           // sum of W*H over the logical frames
+    const uint32_t* rgba;    // shaded passes (packed ABGR)
+    const double* tab;       // [0,256): (v/255)^gamma;  [256,512): display thresholds
+    double inv_n;
+    uint8_t* rgb;
+};
+
+__global__ __launch_bounds__(256) void kt_spp_resolve_kernel(ResolveArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_out) return;
+    // every earlier logical frame holds spp * its pixels, so its output base is pix_base / spp
+    int lo = 0, hi = A.n_logical - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.frames[(int64_t)mid * A.spp].pix_base / A.spp <= i) lo = mid; else hi = mid - 1;
+    }
+    const FrameDev* F = A.frames + (int64_t)lo * A.spp;
+    const int64_t p = i - F[0].pix_base / A.spp;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < A.spp; ++s) {
+        const uint32_t v = A.rgba[F[s].pix_base + p];
+        acc[0] += A.tab[v & 0xFF];
+        acc[1] += A.tab[(v >> 8) & 0xFF];
+        acc[2] += A.tab[(v >> 16) & 0xFF];
+    }
+    const double* thr = A.tab + 256;
+    for (int c = 0; c < 3; ++c) {
+        double x = acc[c] * A.inv_n;
+        x = x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x);
+        int l = 0, h = 255;  // largest level k with thr[k] <= x (thr[0] = -inf)
+        while (l < h) {
+            const int m = (l + h + 1) >> 1;
+            if (thr[m] <= x) l = m; else h = m - 1;
+        }
+        A.rgb[3 * i + c] = (uint8_t)l;
+    }
+}
+
 typedef void (*TraceKernelFn)(TraceArgs);
 
 template <int CHART, int OCC>
@@ -478,6 +523,8 @@ struct kt_hip_ctx {
     uint32_t* d_rgba = nullptr;
     uint8_t* d_bg = nullptr; int bg_w = 0, bg_h = 0; size_t cap_bg = 0;
     unsigned long long* d_words = nullptr;
+    uint8_t* d_rgb = nullptr; size_t cap_rgb = 0;       // camera-spp resolved RGB8
+    double* d_tab = nullptr; double tab_gamma = -1.0;  // camera-spp linear table + thresholds
     std::vector<FrameDev> h_frames;
 };
 
@@ -558,6 +605,8 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     (void)hipFree(c->d_rgba);
     (void)hipFree(c->d_bg);
     (void)hipFree(c->d_words);
+    (void)hipFree(c->d_rgb);
+    (void)hipFree(c->d_tab);
     for (int i = 0; i < 4; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -867,6 +916,189 @@ int kt_make_checker_background(uint64_t seed, int32_t w, int32_t h, int32_t cx, 
             for (int ch = 0; ch < 3; ++ch)
                 out[((size_t)y * w + x) * 3 + ch] = pal[((size_t)j * cx + i) * 3 + ch];
         }
+    }
+    return KT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+//  camera-spp: offsets, tables and the resolve rule
+// ---------------------------------------------------------------------------
+static double spp_gamma(double gamma) { return gamma > 1e-6 ? gamma : 1e-6; }
+
+static int spp_display_level(double x, double g) {
+    x = x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x);
+    return (int)std::lround(255.0 * std::pow(x, 1.0 / g));
+}
+
+// tab[0,256): (v/255)^g.  tab[256+k]: least binary64 x in [0,1] whose display level is >= k,
+// found by bisection on the ordered bit patterns of non-negative doubles (tab[256] = -inf).
+static void spp_tables(double gamma, double* tab) {
+    const double g = spp_gamma(gamma);
+    for (int v = 0; v < 256; ++v) tab[v] = std::pow((double)v / 255.0, g);
+    tab[256] = -HUGE_VAL;
+    uint64_t one;
+    const double d1 = 1.0;
+    std::memcpy(&one, &d1, 8);
+    for (int k = 1; k < 256; ++k) {
+        uint64_t lo = 0, hi = one;   // level(0) = 0 < k, level(1) = 255 >= k
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            double x;
+            std::memcpy(&x, &mid, 8);
+            if (spp_display_level(x, g) >= k) hi = mid; else lo = mid;
+        }
+        std::memcpy(&tab[256 + k], &hi, 8);
+    }
+}
+
+extern "C" {
+
+int kt_spp_offsets(int32_t spp, double* xy) {
+    if (!xy || spp < 1) return KT_E_ARG;
+    std::vector<std::pair<double, double>> o;
+    if (spp == 1) {
+        o.emplace_back(0.0, 0.0);
+    } else if (spp == 2) {             // horizontal pair
+        o.emplace_back(-0.35, 0.0);
+        o.emplace_back(0.35, 0.0);
+    } else {
+        if (spp % 2) o.emplace_back(0.0, 0.0);
+        static const double seeds[7][2] = {{0.35, 0.00}, {0.00, 0.35}, {0.25, 0.25}, {0.45, 0.20},
+                                           {0.20, 0.45}, {0.40, 0.00}, {0.00, 0.40}};
+        for (const auto& s : seeds) {
+            if ((int)o.size() + 2 > spp) break;
+            o.emplace_back(s[0], s[1]);
+            o.emplace_back(-s[0], -s[1]);
+        }
+        const double golden = 2.39996322972865332;   // 2*pi*(1 - 1/phi)
+        for (int n = 0; (int)o.size() < spp; ++n) {
+            const double rad = 0.49 * std::sqrt((double)(n + 1) / (double)(spp + 1));
+            const double ang = golden * (double)(n + 1);
+            const double x = rad * std::cos(ang), y = rad * std::sin(ang);
+            o.emplace_back(x, y);
+            if ((int)o.size() < spp) o.emplace_back(-x, -y);
+        }
+    }
+    for (int i = 0; i < spp; ++i) { xy[2 * i] = o[i].first; xy[2 * i + 1] = o[i].second; }
+    return KT_OK;
+}
+
+int kt_spp_resolve(const uint32_t* passes, int32_t n_pass, int64_t n_pix, double gamma, uint8_t* rgb) {
+    if (!passes || !rgb || n_pass < 1 || n_pix < 0) return KT_E_ARG;
+    const double g = spp_gamma(gamma);
+    double lut[256];
+    for (int v = 0; v < 256; ++v) lut[v] = std::pow((double)v / 255.0, g);
+    const double inv = 1.0 / (double)n_pass;
+    for (int64_t i = 0; i < n_pix; ++i) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; s < n_pass; ++s) {
+            const uint32_t v = passes[(int64_t)s * n_pix + i];
+            acc[0] += lut[v & 0xFF];
+            acc[1] += lut[(v >> 8) & 0xFF];
+            acc[2] += lut[(v >> 16) & 0xFF];
+        }
+        for (int c = 0; c < 3; ++c) rgb[3 * i + c] = (uint8_t)spp_display_level(acc[c] * inv, g);
+    }
+    return KT_OK;
+}
+
+int kt_spp_tables(double gamma, double* tab) {
+    if (!tab) return KT_E_ARG;
+    spp_tables(gamma, tab);
+    return KT_OK;
+}
+
+int kt_hip_render_spp(kt_hip_ctx* c, const kt_frame* frames, int n_frames, int spp, const kt_opts* o, uint8_t* rgb_out,
+                      kt_stats* st) {
+    if (!c) return KT_E_ARG;
+    if (!frames || n_frames <= 0 || !rgb_out) return fail(c, KT_E_ARG, "frames and rgb_out are required");
+    if (spp < 1 || spp > 256) return fail(c, KT_E_ARG, "spp must be in [1, 256]");
+    if (!o) return fail(c, KT_E_ARG, "opts is NULL");
+    std::vector<double> xy(2 * (size_t)spp);
+    kt_spp_offsets(spp, xy.data());
+    std::vector<kt_frame> ex((size_t)n_frames * spp);
+    int64_t n_out = 0;
+    for (int f = 0; f < n_frames; ++f) {
+        for (int s = 0; s < spp; ++s) {
+            kt_frame k = frames[f];
+            if (spp > 1) { k.offset_x += xy[2 * s]; k.offset_y += xy[2 * s + 1]; }
+            ex[(size_t)f * spp + s] = k;
+        }
+        n_out += (int64_t)frames[f].width * frames[f].height;
+    }
+    int64_t pix = 0;
+    int rc = render_on_device(c, ex.data(), (int)ex.size(), o, true, st, &pix);
+    if (rc) return rc;
+    if ((size_t)n_out * 3 > c->cap_rgb) {
+        (void)hipFree(c->d_rgb);
+        c->d_rgb = nullptr;
+        c->cap_rgb = 0;
+        if (hipMalloc(&c->d_rgb, (size_t)n_out * 3) != hipSuccess) return fail(c, KT_E_ALLOC, "spp output allocation failed");
+        c->cap_rgb = (size_t)n_out * 3;
+    }
+    if (!c->d_tab && hipMalloc(&c->d_tab, 512 * sizeof(double)) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "spp table allocation failed");
+    if (c->tab_gamma != o->gamma) {
+        double tab[512];
+        spp_tables(o->gamma, tab);
+        KT_HIP(c, hipMemcpy(c->d_tab, tab, sizeof(tab), hipMemcpyHostToDevice));
+        c->tab_gamma = o->gamma;
+    }
+    ResolveArgs R;
+    R.frames = c->d_frames;
+    R.n_logical = n_frames;
+    R.spp = spp;
+    R.n_out = n_out;
+    R.rgba = c->d_rgba;
+    R.tab = c->d_tab;
+    R.inv_n = 1.0 / (double)spp;
+    R.rgb = c->d_rgb;
+    hipLaunchKernelGGL(kt_spp_resolve_kernel, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, c->stream, R);
+    KT_HIP(c, hipGetLastError());
+    KT_HIP(c, hipMemcpyAsync(rgb_out, c->d_rgb, (size_t)n_out * 3, hipMemcpyDeviceToHost, c->stream));
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    return KT_OK;
+}
+
+int kt_save_kgeo(const char* path, const kt_kgeo_meta* meta, const kt_geopixel* geo) {
+    if (!path || !meta || !geo || meta->W == 0 || meta->H == 0) return KT_E_ARG;
+    static_assert(sizeof(kt_kgeo_meta) == sizeof(kthip::KGeoMeta), "kgeo header layout");
+    kthip::KGeoMeta m;
+    std::memcpy(&m, meta, sizeof(m));
+    std::vector<kt_geopixel> v(geo, geo + (size_t)meta->W * meta->H);
+    return kthip::save_kgeo(path, v, m) ? KT_OK : KT_E_ARG;
+}
+
+int kt_load_kgeo(const char* path, kt_kgeo_meta* meta, kt_geopixel* geo_out, int64_t cap) {
+    if (!path || !meta) return KT_E_ARG;
+    std::vector<kt_geopixel> v;
+    kthip::KGeoMeta m;
+    if (!kthip::load_kgeo(path, v, m, nullptr)) return KT_E_ARG;
+    std::memcpy(meta, &m, sizeof(m));
+    if (geo_out) {
+        if (cap < (int64_t)v.size()) return KT_E_ARG;
+        std::memcpy(geo_out, v.data(), v.size() * sizeof(kt_geopixel));
+    }
+    return KT_OK;
+}
+
+int kt_write_png(const char* path, const uint8_t* rgb, int32_t w, int32_t h) {
+    if (!path || !rgb || w <= 0 || h <= 0) return KT_E_ARG;
+    return kthip::write_png(path, rgb, w, h) ? KT_OK : KT_E_ARG;
+}
+
+int kt_read_image_rgb(const char* path, uint8_t* rgb_out, int64_t cap, int32_t* w, int32_t* h) {
+    if (!path || !w || !h) return KT_E_ARG;
+    std::vector<uint8_t> rgb;
+    int W = 0, H = 0;
+    if (!kthip::load_rgb_image(path, rgb, W, H, nullptr)) return KT_E_ARG;
+    *w = W;
+    *h = H;
+    if (rgb_out) {
+        if (cap < (int64_t)rgb.size()) return KT_E_ARG;
+        std::memcpy(rgb_out, rgb.data(), rgb.size());
     }
     return KT_OK;
 }
