@@ -7,7 +7,12 @@ the frame (device setup + persistent trace + shading) with its parameters alread
 device and the outputs (GeoPixel + RGBA) left in HBM.  `value` is the mixed-precision
 rays/s; the FP64 path is timed the same way and reported beside it.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--only mixed|fp64]
+    python bench.py --copy-calibration     # known-size device copy, to scale the PMC byte counters
+
+roofline.traffic (HBM bytes per trace launch) comes from the committed rocprofv3 --pmc summary
+profiles/kt_trace_kernel_c2_1280x720_traffic.txt (FETCH_SIZE + WRITE_SIZE of the trace kernel,
+each scaled by the same counter's reading on the calibration copy); null when that file is absent.
 """
 import argparse
 import json
@@ -76,6 +81,32 @@ def accuracy(geo, ref, px, py, band_geo):
             "fate_agreement_all": float((sub["outcome"] == ref["outcome"]).mean())}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kt_trace_kernel_c2_1280x720_traffic.txt")
+COPY_BYTES = 1 << 28
+
+
+def traffic_from_profile(kernel_key):
+    """HBM bytes per launch of the trace kernel recorded in the committed PMC summary."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch") and kernel_key in line:
+                return float(line.split()[-1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """A device-to-device copy of COPY_BYTES bytes (reads and writes each byte once)."""
+    import torch
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": COPY_BYTES, "launches": 3}))
+
+
 def cpu_baseline(budget_s):
     """The oracle (FP64 CPU restatement, OpenMP over pixels) on this host's cores, on whole rows
     of the same frame taken in a strided order until the time budget is used."""
@@ -110,13 +141,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--copy-calibration", action="store_true")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
               "(rays and frames shard across GPUs without exchange; see the sharding notes)", file=sys.stderr)
         sys.exit(2)
     import kerrtracecpp2_amd as kt
     ctx = kt.Context(0)
+    if a.only != "both":   # profiling runs: one kernel variant, no JSON contract
+        prec = kt.KT_PRECISION_MIXED if a.only == "mixed" else kt.KT_PRECISION_FP64
+        wall, st, _, _ = gpu_run(ctx, kt, prec, a.steps, a.warmup)
+        print(json.dumps({"only": a.only, "ms_per_step": 1e3 * wall / a.steps,
+                          "trace_ms": float(np.mean([s["trace_ms"] for s in st]))}))
+        ctx.close()
+        return
     wall_m, st_m, geo_m, info = gpu_run(ctx, kt, kt.KT_PRECISION_MIXED, a.steps, a.warmup)
     wall_d, st_d, geo_d, _ = gpu_run(ctx, kt, kt.KT_PRECISION_FP64, a.steps, a.warmup)
     ms_m = 1e3 * wall_m / a.steps
@@ -152,7 +195,7 @@ def main():
                   "rhs_fp64": st_m[-1]["rhs_fp64"]},
         "frame": {"r_horizon": info["r_horizon"], "r_isco": info["r_isco"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": traffic_from_profile("MIXED"),
                      "kernel": "kt_trace_kernel<KS,DOPRI5,MIXED>", "flops_per_launch": fl_m},
     }
     if not a.no_cpu:

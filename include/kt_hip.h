@@ -85,8 +85,8 @@ typedef struct kt_opts {
     double exposure, gamma, temp_scale;   /* ColorParams (main.cpp:72-81)                   */
     int32_t tuning_blocks_per_cu;         /* persistent grid = blocks_per_cu x CUs (256-thread
                                              blocks); 0: autotuned / built-in default         */
-    int32_t tuning_occupancy;             /* register budget variant: 0 default (2 waves per
-                                             SIMD), 1 the compiler's own budget, 2            */
+    int32_t tuning_occupancy;             /* waves per SIMD the trace kernel's register budget
+                                             is sized for: 0 built-in default (2), 2 or 3       */
     int32_t tuning_chunk;                 /* rays per work-queue grab; 0: 64                   */
     int32_t tuning_pad;
     double mixed_noise;     /* MIXED: binary32 stage-noise coefficient of the island test;

@@ -241,7 +241,8 @@ struct TraceArgs {
 };
 
 constexpr int kBlock = 256;
-constexpr int kDefaultBlocksPerCu = 2;
+constexpr int kDefaultBlocksPerCu = 3;
+constexpr int kDefaultOcc = 2;     // waves per SIMD the register budget is sized for
 constexpr int kDefaultChunk = 64;
 
 __device__ __forceinline__ int find_frame(const FrameDev* fr, int n, int64_t g) {
@@ -380,20 +381,20 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
     }
 }
 
-// OCC > 0 caps the register budget so that OCC waves fit per SIMD (amdgpu_waves_per_eu)
-template <int CHART, int METHOD, int PREC, bool PHI, int OCC>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
+// OCC caps the register budget so that OCC waves fit per SIMD (amdgpu_waves_per_eu).
+// SINGLE (one frame per launch) keeps the frame constants wave-uniform; it is a
+// separate kernel so that each code object holds one loop (instruction cache).
+template <int CHART, int METHOD, int PREC, bool PHI, int OCC, bool SINGLE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
 void kt_trace_kernel(TraceArgs A) {
-    typedef GeodesicPolicy<CHART, METHOD, PREC, PHI> Pol;
-    if (A.single) trace_loop<Pol, true>(A);
-    else trace_loop<Pol, false>(A);
+    trace_loop<GeodesicPolicy<CHART, METHOD, PREC, PHI>, SINGLE>(A);
 }
 
 // Jacobi ray bundles: BL chart, RK4 step doubling, binary64 (ray_bundle.hpp)
+template <bool SINGLE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void kt_bundle_kernel(TraceArgs A) {
-    if (A.single) trace_loop<BundlePolicy, true>(A);
-    else trace_loop<BundlePolicy, false>(A);
+    trace_loop<BundlePolicy, SINGLE>(A);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,29 +481,31 @@ __global__ __launch_bounds__(256) void kt_spp_resolve_kernel(ResolveArgs A) {
 
 typedef void (*TraceKernelFn)(TraceArgs);
 
-template <int CHART, int OCC>
+template <int CHART, int OCC, bool SINGLE>
 TraceKernelFn pick_kernel_chart(int method, int prec, bool phi) {
     if (method == METHOD_RK4)
-        return prec ? kt_trace_kernel<CHART, METHOD_RK4, PREC_MIXED, true, OCC>
-                    : kt_trace_kernel<CHART, METHOD_RK4, PREC_FP64, true, OCC>;
+        return prec ? kt_trace_kernel<CHART, METHOD_RK4, PREC_MIXED, true, OCC, SINGLE>
+                    : kt_trace_kernel<CHART, METHOD_RK4, PREC_FP64, true, OCC, SINGLE>;
     if (phi)
-        return prec ? kt_trace_kernel<CHART, METHOD_DP5, PREC_MIXED, true, OCC>
-                    : kt_trace_kernel<CHART, METHOD_DP5, PREC_FP64, true, OCC>;
-    return prec ? kt_trace_kernel<CHART, METHOD_DP5, PREC_MIXED, false, OCC>
-                : kt_trace_kernel<CHART, METHOD_DP5, PREC_FP64, false, OCC>;
+        return prec ? kt_trace_kernel<CHART, METHOD_DP5, PREC_MIXED, true, OCC, SINGLE>
+                    : kt_trace_kernel<CHART, METHOD_DP5, PREC_FP64, true, OCC, SINGLE>;
+    return prec ? kt_trace_kernel<CHART, METHOD_DP5, PREC_MIXED, false, OCC, SINGLE>
+                : kt_trace_kernel<CHART, METHOD_DP5, PREC_FP64, false, OCC, SINGLE>;
 }
 
-template <int OCC>
+template <int OCC, bool SINGLE>
 TraceKernelFn pick_kernel_occ(int chart, int method, int prec, bool phi) {
-    return chart == CHART_KS ? pick_kernel_chart<CHART_KS, OCC>(method, prec, phi)
-                             : pick_kernel_chart<CHART_BL, OCC>(method, prec, phi);
+    return chart == CHART_KS ? pick_kernel_chart<CHART_KS, OCC, SINGLE>(method, prec, phi)
+                             : pick_kernel_chart<CHART_BL, OCC, SINGLE>(method, prec, phi);
 }
 
-TraceKernelFn pick_kernel(int chart, int method, int prec, bool phi, int occ) {
-    switch (occ) {
-        case 1: return pick_kernel_occ<0>(chart, method, prec, phi);   // compiler's own register budget
-        default: return pick_kernel_occ<2>(chart, method, prec, phi);  // measured best on gfx950
-    }
+// occ: 0 built-in default, 2 or 3 waves per SIMD
+TraceKernelFn pick_kernel(int chart, int method, int prec, bool phi, int occ, bool single) {
+    if (occ == 3)
+        return single ? pick_kernel_occ<3, true>(chart, method, prec, phi)
+                      : pick_kernel_occ<3, false>(chart, method, prec, phi);
+    return single ? pick_kernel_occ<kDefaultOcc, true>(chart, method, prec, phi)
+                  : pick_kernel_occ<kDefaultOcc, false>(chart, method, prec, phi);
 }
 
 }  // namespace
@@ -665,7 +668,6 @@ static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, co
     if (!frames || n_frames <= 0) return fail(c, KT_E_ARG, "no frames");
     c->h_frames.assign(n_frames, FrameDev());
     int64_t rays = 0, pix = 0;
-    const double d2r = kPi / 180.0;
     int chart = -1;
     for (int i = 0; i < n_frames; ++i) {
         const kt_frame& f = frames[i];
@@ -678,9 +680,9 @@ static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, co
         std::memset(&F, 0, sizeof(F));
         F.M = f.M; F.a = f.a; F.Q = f.Q; F.Lambda = f.Lambda;
         F.r_obs = f.r_obs;
-        F.th_obs = f.theta_deg * d2r;
-        F.ph_obs = f.phi_deg * d2r;
-        F.fov = f.fov_deg * d2r;
+        F.th_obs = f.theta_deg * kPi / 180.0;   // the Camera constructor's rounding (camera.hpp:33-35)
+        F.ph_obs = f.phi_deg * kPi / 180.0;
+        F.fov = f.fov_deg * kPi / 180.0;
         F.disk_out = f.disk_out;
         F.off_x = f.offset_x; F.off_y = f.offset_y;
         F.W = f.width; F.H = f.height;
@@ -742,8 +744,9 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
     const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
-    TraceKernelFn fn = o->bundles ? kt_bundle_kernel
-                                  : pick_kernel(chart, o->integrator, o->precision, phi, o->tuning_occupancy);
+    const bool single = n_frames == 1;
+    TraceKernelFn fn = o->bundles ? (single ? kt_bundle_kernel<true> : kt_bundle_kernel<false>)
+                                  : pick_kernel(chart, o->integrator, o->precision, phi, o->tuning_occupancy, single);
     if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");
 
     // persistent grid: blocks beyond the resident limit simply start when others retire

@@ -373,9 +373,13 @@ KT_HD SinCos<double> sc_at(const SinCos<double>& sc0, double th0, double dth) {
 //  events after an accepted step (main.cpp:363-431)
 // ---------------------------------------------------------------------------
 // The reference's KS tracer reports azimuths through atan2 (KS_to_BL_spatial,
-// knds_metric.hpp:389-411), i.e. in (-pi, pi]; BL reports the integrated phi.
+// knds_metric.hpp:389-411), i.e. wrapped into [-pi, pi]; BL reports the integrated
+// phi.  The wrap is phi - 2 pi rint(phi / 2 pi) with 2 pi in two parts.
 template <int CHART> KT_HD double out_phi(double ph) {
-    if (CHART == CHART_KS) { double s, c; kt_sincos(ph, &s, &c); return atan2(s, c); }
+    if (CHART == CHART_KS) {
+        const double k = rint(ph * 0.15915494309189535);
+        return fma(-k, 2.4492935982947064e-16, fma(-k, 6.283185307179586, ph));
+    }
     return ph;
 }
 

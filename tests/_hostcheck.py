@@ -31,6 +31,7 @@ def lib():
                                 ctypes.c_double, ctypes.c_double, P, ctypes.c_int, ctypes.c_int, P]
         L.ktc_isco_exact.argtypes = [ctypes.c_double] * 4 + [P, P]
         L.ktc_trace_bundle_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
+        L.ktc_sincos.argtypes = [P, ctypes.c_int64, P, P]
         _lib = L
     return _lib
 
@@ -78,3 +79,10 @@ def trace_bundle_pixels(frame, opts, px, py):
     assert lib().ktc_trace_bundle_pixels(ctypes.byref(frame), ctypes.byref(opts), _p(px), _p(py), px.size, _p(out),
                                          _p(acc)) == 0
     return out, acc
+
+
+def sincos(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    lib().ktc_sincos(_p(x), x.size, _p(s), _p(c))
+    return s, c

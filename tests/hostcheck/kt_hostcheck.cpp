@@ -14,9 +14,9 @@ using namespace kt;
 
 static void fill_frame(const kt_frame& f, const kt_opts& o, FrameDev& F) {
     std::memset(&F, 0, sizeof(F));
-    const double d2r = kPi / 180.0;
     F.M = f.M; F.a = f.a; F.Q = f.Q; F.Lambda = f.Lambda;
-    F.r_obs = f.r_obs; F.th_obs = f.theta_deg * d2r; F.ph_obs = f.phi_deg * d2r; F.fov = f.fov_deg * d2r;
+    F.r_obs = f.r_obs;
+    F.th_obs = f.theta_deg * kPi / 180.0; F.ph_obs = f.phi_deg * kPi / 180.0; F.fov = f.fov_deg * kPi / 180.0;
     F.disk_out = f.disk_out; F.off_x = f.offset_x; F.off_y = f.offset_y;
     F.W = f.width; F.H = f.height;
     int ch = o.chart;
@@ -138,5 +138,11 @@ extern "C" int ktc_trace_bundle_pixels(const kt_frame* f, const kt_opts* o, cons
         std::memcpy(&out[i], w, 28);
         if (acc) acc[i] = B.g.n_acc;
     }
+    return 0;
+}
+
+// the inline binary64 sin/cos of the kernels, on n arguments
+extern "C" int ktc_sincos(const double* x, int64_t n, double* s, double* c) {
+    for (int64_t i = 0; i < n; ++i) kt_sincos(x[i], &s[i], &c[i]);
     return 0;
 }

@@ -143,3 +143,20 @@ def test_bundle_magnification_converged_vs_oracle():
     got, _ = hc.trace_bundle_pixels(_abi.make_frame(64, 36, a=0.99, theta=89, r_obs=40, disk_out=12), op, px, py)
     assert np.array_equal(got["outcome"], ref["outcome"])
     assert np.all(np.abs(got["magnif"] / ref["magnif"] - 1.0) < 0.01)
+
+
+def test_kernel_sincos_accuracy():
+    """The kernels' inline sin/cos (Cody-Waite reduction + Taylor series) against x87
+    extended precision: <= 2 ulp over the angles a ray meets, exact special values."""
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-4, 4, 200000), rng.uniform(-200, 200, 200000), rng.uniform(-1e6, 1e6, 20000),
+                        np.arange(-64, 65) * np.pi / 4, [1e-300, 5e-324, np.pi, 3 * np.pi / 2]])
+    s, c = hc.sincos(x)
+    xl = x.astype(np.longdouble)
+    for got, ref in ((s, np.sin(xl)), (c, np.cos(xl))):
+        ulp = np.spacing(np.abs(ref.astype(np.float64))).astype(np.longdouble)
+        assert float((np.abs(got.astype(np.longdouble) - ref) / ulp).max()) <= 2.0
+    s, c = hc.sincos(np.array([0.0, -0.0, np.pi / 2, np.nan, np.inf, -np.inf]))
+    assert np.array_equal(s[:2], [0.0, -0.0]) and np.signbit(s[1]) and np.array_equal(c[:2], [1.0, 1.0])
+    assert s[2] == 1.0 and c[2] == 6.123233995736766e-17        # glibc's cos(M_PI/2)
+    assert np.all(np.isnan(s[3:])) and np.all(np.isnan(c[3:]))
