@@ -188,7 +188,7 @@ def main():
                    "rays_per_step": rays, "chart": "ks", "integrator": "dopri5"},
         "fp64": {"rays_per_s": rays / (ms_d * 1e-3), "ms_per_step": ms_d, "trace_kernel_ms": trace_ms_d,
                  "steps_accepted": st_d[-1]["steps_accepted"], "steps_rejected": st_d[-1]["steps_rejected"],
-                 "achieved_tflops": fl_d / (trace_ms_d * 1e-3) / 1e12},
+                 "achieved_tflops": fl_d / (trace_ms_d * 1e-3) / 1e12, "traffic": traffic_from_profile("FP64")},
         "mixed_vs_fp64": ms_d / ms_m,
         "mixed": {"trace_kernel_ms": trace_ms_m, "steps_accepted": st_m[-1]["steps_accepted"],
                   "steps_rejected": st_m[-1]["steps_rejected"], "rhs_fp32": st_m[-1]["rhs_fp32"],

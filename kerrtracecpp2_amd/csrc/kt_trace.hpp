@@ -500,8 +500,7 @@ KT_HD int after_reject(const RunOpts& O, RayState& S) {
 }
 
 // ---------------------------------------------------------------------------
-//  DOPRI5 attempt (geodesic.hpp:312-414).  The 5th-order update and the embedded
-//  error are accumulated as the stages arrive so k2..k6 die early (register
+//  DOPRI5 attempt (geodesic.hpp:312-414), ordered for a small live set (register
 //  economy); NC = 5 also advances phi with the same weights.
 // ---------------------------------------------------------------------------
 template <int CHART, int PREC, int NC>
@@ -511,27 +510,32 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     const double h = S.h;
     const double* y = S.y;
     const double* k1 = S.k1;
-    double k2[NC], k3[NC], k4[NC], k5[NC], k6[NC];
+    // Live set kept small: k2..k4 stay raw until k5 exists (b2 = e2 = 0), the 5th-order
+    // and error combinations are formed after k5, so at most three stage vectors besides
+    // k1 are live while a stage is evaluated.
+    double k2[NC], k3[NC], k4[NC];
     double d[4], a5[NC], ae[4];
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
     stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2, S);
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
     stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3, S);
-    for (int i = 0; i < NC; ++i) a5[i] = C::b1*k1[i] + C::b3*k3[i];
-    for (int i = 0; i < 4; ++i) ae[i] = C::e1*k1[i] + C::e3*k3[i];
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
     stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4, S);
-    for (int i = 0; i < NC; ++i) a5[i] += C::b4*k4[i];
-    for (int i = 0; i < 4; ++i) ae[i] += C::e4*k4[i];
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
-    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5, S);
-    for (int i = 0; i < NC; ++i) a5[i] += C::b5*k5[i];
-    for (int i = 0; i < 4; ++i) ae[i] += C::e5*k5[i];
-    for (int i = 0; i < 4; ++i)
-        d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
-    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6, S);
-    for (int i = 0; i < NC; ++i) a5[i] += C::b6*k6[i];
-    for (int i = 0; i < 4; ++i) ae[i] += C::e6*k6[i];
+    {
+        double k5[NC];
+        for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
+        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5, S);
+        for (int i = 0; i < 4; ++i)
+            d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
+        for (int i = 0; i < NC; ++i) a5[i] = C::b1*k1[i] + C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i];
+        for (int i = 0; i < 4; ++i) ae[i] = C::e1*k1[i] + C::e3*k3[i] + C::e4*k4[i] + C::e5*k5[i];
+    }
+    {
+        double k6[NC];
+        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6, S);
+        for (int i = 0; i < NC; ++i) a5[i] += C::b6*k6[i];
+        for (int i = 0; i < 4; ++i) ae[i] += C::e6*k6[i];
+    }
     for (int i = 0; i < 4; ++i) d[i] = h*a5[i];
     // FSAL stage at y5 in the stage precision; its sin/cos becomes the next base
     double k7[5];
