@@ -65,6 +65,27 @@ def gpu_run(ctx, kt, prec, steps, warmup):
     return wall, stats, out["geo"], out["info"][0]
 
 
+def c4_run(ctx, kt, steps):
+    """The uneven batched workload: 64 seeded frames (24.2 M rays), KS DOPRI5 mixed, rendered by
+    one graph launch of an autotuned plan; timed with the outputs left on the device."""
+    from kerrtracecpp2_amd import workloads
+    frames = workloads.c4_frames()
+    npix = sum(f.width * f.height for f in frames)
+    o = kt.default_opts()
+    o.chart, o.integrator, o.precision = kt.KT_CHART_KS, kt.KT_INTEGRATOR_DOPRI5, kt.KT_PRECISION_MIXED
+    o.rtol, o.atol = RTOL, ATOL
+    plan = ctx.plan(o, len(frames), npix)
+    plan.render_device(frames)          # autotune + capture + first launch
+    t0 = time.perf_counter()
+    sts = [plan.render_device(frames) for _ in range(steps)]
+    wall = (time.perf_counter() - t0) / steps
+    tune = plan.tuning()
+    plan.close()
+    return {"workload": f"{len(frames)} frames, sides 64..1024, seed {workloads.C4_SEED}, KS DOPRI5 mixed, one graph launch",
+            "rays": npix, "ms_per_batch": 1e3 * wall, "graph_ms": float(np.mean([s["trace_ms"] for s in sts])),
+            "rays_per_s": npix / wall, "steps_accepted": sts[-1]["steps_accepted"], "autotune": tune}
+
+
 def accuracy(geo, ref, px, py, band_geo):
     """Per-pixel agreement with the oracle outside the photon-ring band (fate boundaries + the
     steep inner rim, |g - g_neighbour| > 0.05), the band taken from the full FP64 GPU image."""
@@ -141,6 +162,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--copy-calibration", action="store_true")
     a = ap.parse_args()
@@ -198,6 +220,8 @@ def main():
                      "frac": achieved / PEAK_VALU_TFLOPS, "traffic": traffic_from_profile("MIXED"),
                      "kernel": "kt_trace_kernel<KS,DOPRI5,MIXED>", "flops_per_launch": fl_m},
     }
+    if not a.no_c4:
+        result["batched_c4"] = c4_run(ctx, kt, 3)
     if not a.no_cpu:
         cpu, ref, px, py = cpu_baseline(a.cpu_budget)
         result["cpu_baseline"] = cpu

@@ -119,7 +119,19 @@ typedef struct kt_stats {
     uint64_t events;          /* Hermite disk searches run                                  */
 } kt_stats;
 
+/* Launch configuration chosen by the autotuner (kt_hip_autotune). */
+typedef struct kt_tune {
+    int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu x CUs                        */
+    int32_t chunk;            /* rays per work-queue grab                                      */
+    int32_t occupancy;        /* waves per SIMD of the register budget                         */
+    int32_t candidates;       /* configurations timed                                          */
+    double best_ms;           /* trace time of the chosen configuration on the probe           */
+    double default_ms;        /* trace time of the built-in configuration on the probe         */
+    int64_t probe_rays;       /* rays of the probe workload                                    */
+} kt_tune;
+
 typedef struct kt_hip_ctx kt_hip_ctx;
+typedef struct kt_hip_plan kt_hip_plan;
 
 /* Library / context life cycle. */
 int  kt_hip_abi_version(void);
@@ -169,6 +181,31 @@ int kt_make_checker_background(uint64_t seed, int32_t width, int32_t height, int
  * the host with the same pow).  rgb_out: sum(W*H*3) bytes, frames back to back.               */
 int kt_hip_render_spp(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, int spp, const kt_opts* opts,
                       uint8_t* rgb_out, kt_stats* stats);
+
+/* Autotuner (no reference counterpart; the reference has no GPU launch parameters to tune).
+ * Times the trace kernel for blocks_per_cu in {2,3,4,6} x chunk in {32,64,128} on a probe
+ * workload: the given frames scaled down (same fov and aspect) to about 2 M rays, or the
+ * frames themselves when smaller.  The fastest configuration is returned and remembered by
+ * the context for this (chart, integrator, precision, bundles) class: later renders whose
+ * kt_opts tuning fields are 0 use it.  Results do not depend on the configuration (FP64). */
+int kt_hip_autotune(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const kt_opts* opts, kt_tune* out);
+
+/* Batched render plan (no reference counterpart; the reference renders animation frames one
+ * at a time, main.cpp:2994-3041).  A plan owns device buffers for up to max_frames frames and
+ * max_pixels pixels and, on its first render, captures the whole render -- frame-table upload
+ * from pinned staging, counter fill, setup, persistent trace, shade -- in one HIP graph
+ * (autotuning first when the tuning fields of opts are 0).  Every render, with any frame list
+ * that fits, is then a single graph launch.  Frames must share the effective chart.
+ * stats->trace_ms and total_ms are the graph's execution time.                              */
+int kt_hip_plan_create(kt_hip_ctx* ctx, const kt_opts* opts, int max_frames, int64_t max_pixels,
+                       kt_hip_plan** out);
+int kt_hip_plan_render(kt_hip_plan* plan, const kt_frame* frames, int n_frames, uint32_t* rgba_out,
+                       kt_geopixel* geo_out, kt_frame_info* info_out, kt_stats* stats);
+/* Outputs stay on the device (valid until the next render of the plan). */
+int kt_hip_plan_render_device(kt_hip_plan* plan, const kt_frame* frames, int n_frames, void** d_rgba,
+                              void** d_geo, kt_stats* stats);
+int kt_hip_plan_tuning(const kt_hip_plan* plan, kt_tune* out);
+void kt_hip_plan_destroy(kt_hip_plan* plan);
 
 /* Host utilities (no device work; callable without a GPU). */
 /* Offsets of the spp jittered passes, (x, y) pairs in pixels (build_offsets, main.cpp:2240-2279). */

@@ -127,12 +127,78 @@ class Context:
                                                ctypes.byref(st)), "kt_hip_render_spp")
         return {"rgb": rgb, "stats": st.as_dict()}
 
+    def autotune(self, frames, opts):
+        """Time the trace kernel's launch configurations on a probe of these frames; the context
+        then uses the fastest one for this option class.  Returns the kt_tune fields."""
+        frames = list(frames)
+        arr = (_abi.KtFrame * len(frames))(*frames)
+        t = _abi.KtTune()
+        self._check(self.lib.kt_hip_autotune(self.h, arr, len(frames), ctypes.byref(opts), ctypes.byref(t)),
+                    "kt_hip_autotune")
+        return t.as_dict()
+
+    def plan(self, opts, max_frames, max_pixels):
+        """A graph-captured batched render plan (see Plan)."""
+        return Plan(self, opts, max_frames, max_pixels)
+
     def colorize(self, geo, M, r_isco, opts):
         geo = np.ascontiguousarray(geo, dtype=GEOPIXEL_DTYPE)
         rgba = np.zeros(geo.size, np.uint32)
         self._check(self.lib.kt_hip_colorize(self.h, _ptr(geo), geo.size, float(M), float(r_isco),
                                              ctypes.byref(opts), _ptr(rgba)), "kt_hip_colorize")
         return rgba
+
+
+class Plan:
+    """kt_hip_plan: device buffers for up to max_frames / max_pixels and, from the first render
+    on, the whole render captured in one HIP graph; each render is a single graph launch."""
+
+    def __init__(self, ctx, opts, max_frames, max_pixels):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.opts = opts
+        h = ctypes.c_void_p()
+        ctx._check(self.lib.kt_hip_plan_create(ctx.h, ctypes.byref(opts), int(max_frames), int(max_pixels),
+                                               ctypes.byref(h)), "kt_hip_plan_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.kt_hip_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, frames, want_rgba=True, want_geo=True):
+        frames = list(frames)
+        arr = (_abi.KtFrame * len(frames))(*frames)
+        npix = sum(f.width * f.height for f in frames)
+        rgba = np.zeros(npix, np.uint32) if want_rgba else None
+        geo = np.zeros(npix, GEOPIXEL_DTYPE) if want_geo else None
+        info = (_abi.KtFrameInfo * len(frames))()
+        st = _abi.KtStats()
+        self.ctx._check(self.lib.kt_hip_plan_render(self.h, arr, len(frames), _ptr(rgba), _ptr(geo), info,
+                                                    ctypes.byref(st)), "kt_hip_plan_render")
+        return {"rgba": rgba, "geo": geo, "info": [_info_dict(i) for i in info], "stats": st.as_dict()}
+
+    def render_device(self, frames):
+        frames = list(frames)
+        arr = (_abi.KtFrame * len(frames))(*frames)
+        st = _abi.KtStats()
+        d_rgba, d_geo = ctypes.c_void_p(), ctypes.c_void_p()
+        self.ctx._check(self.lib.kt_hip_plan_render_device(self.h, arr, len(frames), ctypes.byref(d_rgba),
+                                                           ctypes.byref(d_geo), ctypes.byref(st)),
+                        "kt_hip_plan_render_device")
+        return st.as_dict()
+
+    def tuning(self):
+        t = _abi.KtTune()
+        self.ctx._check(self.lib.kt_hip_plan_tuning(self.h, ctypes.byref(t)), "kt_hip_plan_tuning")
+        return t.as_dict()
 
 
 def _info_dict(i):

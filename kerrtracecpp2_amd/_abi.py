@@ -56,6 +56,15 @@ class KtKgeoMeta(ctypes.Structure):
                 ("phi_deg", ctypes.c_double), ("r_obs", ctypes.c_double)]
 
 
+class KtTune(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_int32), ("chunk", ctypes.c_int32), ("occupancy", ctypes.c_int32),
+                ("candidates", ctypes.c_int32), ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double),
+                ("probe_rays", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class KtStats(ctypes.Structure):
     _fields_ = [("trace_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("rays", ctypes.c_uint64),
                 ("steps_accepted", ctypes.c_uint64), ("steps_rejected", ctypes.c_uint64),
@@ -82,6 +91,12 @@ PROTOTYPES = {
     "kt_make_checker_background": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                    ctypes.c_int32, _P]),
     "kt_hip_render_spp": (ctypes.c_int, [_P, _P, _I, _I, _P, _P, _P]),
+    "kt_hip_autotune": (ctypes.c_int, [_P, _P, _I, _P, _P]),
+    "kt_hip_plan_create": (ctypes.c_int, [_P, _P, _I, ctypes.c_int64, _P]),
+    "kt_hip_plan_render": (ctypes.c_int, [_P, _P, _I, _P, _P, _P, _P]),
+    "kt_hip_plan_render_device": (ctypes.c_int, [_P, _P, _I, _P, _P, _P]),
+    "kt_hip_plan_tuning": (ctypes.c_int, [_P, _P]),
+    "kt_hip_plan_destroy": (None, [_P]),
     "kt_spp_offsets": (ctypes.c_int, [ctypes.c_int32, _P]),
     "kt_spp_resolve": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64, _D, _P]),
     "kt_spp_tables": (ctypes.c_int, [_D, _P]),

@@ -45,6 +45,16 @@ enum {
     W_COUNT = 8
 };
 
+// Frame-list descriptor, uploaded in front of the frame table and read by every
+// kernel of a render, so that one captured graph serves any frame list that fits.
+struct LaunchDesc {
+    int32_t n_frames, pad;
+    int64_t total_rays;      // tile-padded ray count over all frames
+    int64_t n_pix;           // output pixels over all frames
+    int64_t pad2;
+};
+static_assert(sizeof(LaunchDesc) == 32, "frame table header");
+
 struct WaveVote {
     KT_HD static bool any(bool b) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -78,7 +88,8 @@ __device__ double bisect_ref(const FrameDev& F, double lo, double hi) {
     return 0.5*(lo + hi);
 }
 
-__global__ __launch_bounds__(kSetupThreads) void kt_setup_kernel(FrameDev* frames) {
+__global__ __launch_bounds__(kSetupThreads) void kt_setup_kernel(FrameDev* frames, const LaunchDesc* desc) {
+    if ((int)blockIdx.x >= desc->n_frames) return;       // grid sized for a capacity (graphs)
     FrameDev& F = frames[blockIdx.x];
     const int t = threadIdx.x;
     __shared__ double s_rh;
@@ -229,11 +240,10 @@ __global__ __launch_bounds__(kSetupThreads) void kt_setup_kernel(FrameDev* frame
 // ---------------------------------------------------------------------------
 //  persistent trace kernel
 // ---------------------------------------------------------------------------
+
 struct TraceArgs {
     const FrameDev* frames;
-    int32_t n_frames;
-    int32_t single;          // all rays in frame 0
-    int64_t total_rays;      // tile-padded ray count over all frames
+    const LaunchDesc* desc;
     uint32_t* geo;           // 7 dwords per pixel
     unsigned long long* words;
     RunOpts O;
@@ -301,6 +311,8 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
     // single frame: the constants are wave-uniform (scalar registers)
     Knds<double> P = A.frames[0].P;
     Knds<float> Pf = knds_cast<float>(P);
+    const int n_frames = A.desc->n_frames;
+    const int64_t total_rays = A.desc->total_rays;
     int64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
     uint32_t c_acc = 0, c_rej = 0, c_r32 = 0, c_r64 = 0, c_herm = 0, c_rays = 0;
@@ -320,11 +332,11 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(&A.words[W_QUEUE], (unsigned long long)A.chunk);
                 base = __shfl(base, 0, 64);
-                if ((int64_t)base >= A.total_rays) {
+                if ((int64_t)base >= total_rays) {
                     exhausted = true;
                 } else {
                     pool_next = (int64_t)base;
-                    pool_end = (int64_t)base + A.chunk < A.total_rays ? (int64_t)base + A.chunk : A.total_rays;
+                    pool_end = (int64_t)base + A.chunk < total_rays ? (int64_t)base + A.chunk : total_rays;
                     const int avail2 = (int)(pool_end - pool_next);
                     const int rest = need - taken;
                     const int take2 = avail2 < rest ? avail2 : rest;
@@ -333,7 +345,7 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
                 }
             }
             if (mine >= 0) {
-                frame = SINGLE ? 0 : find_frame(A.frames, A.n_frames, mine);
+                frame = SINGLE ? 0 : find_frame(A.frames, n_frames, mine);
                 const FrameDev& F = A.frames[frame];
                 const int64_t loc = mine - F.ray_base;
                 const int64_t tile = loc >> 6;
@@ -402,6 +414,7 @@ void kt_bundle_kernel(TraceArgs A) {
 // ---------------------------------------------------------------------------
 struct ShadeArgs {
     const FrameDev* frames;
+    const LaunchDesc* desc;  // frame count and pixel count when frames != nullptr
     int32_t n_frames;
     int64_t n_pix;
     const uint32_t* geo;
@@ -412,12 +425,10 @@ struct ShadeArgs {
     double M_override, risco_override;   // used when frames == nullptr (colour-only)
 };
 
-__global__ __launch_bounds__(256) void kt_shade_kernel(ShadeArgs A) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n_pix) return;
+__device__ __forceinline__ void shade_one(const ShadeArgs& A, int n_frames, int64_t i) {
     ShadeParams p;
     if (A.frames) {
-        int lo = 0, hi = A.n_frames - 1;
+        int lo = 0, hi = n_frames - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (A.frames[mid].pix_base <= i) lo = mid; else hi = mid - 1;
@@ -433,6 +444,14 @@ __global__ __launch_bounds__(256) void kt_shade_kernel(ShadeArgs A) {
     union { uint32_t u; float f; } r, red, mag, th, ph;
     r.u = g[1]; red.u = g[2]; mag.u = g[3]; th.u = g[5]; ph.u = g[6];
     A.rgba[i] = shade_pixel((uint8_t)(g[0] & 0xFF), r.f, red.f, mag.f, th.f, ph.f, p, A.bg, A.bw, A.bh);
+}
+
+// grid-stride, so that a captured launch covers any pixel count within its capacity
+__global__ __launch_bounds__(256) void kt_shade_kernel(ShadeArgs A) {
+    const int64_t n_pix = A.desc ? A.desc->n_pix : A.n_pix;
+    const int n_frames = A.desc ? A.desc->n_frames : A.n_frames;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pix; i += (int64_t)gridDim.x * blockDim.x)
+        shade_one(A, n_frames, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -513,6 +532,18 @@ TraceKernelFn pick_kernel(int chart, int method, int prec, bool phi, int occ, bo
 // ============================================================================
 //  context + C ABI
 // ============================================================================
+// device frame table: [LaunchDesc][FrameDev x cap], filled from pinned staging of the same layout
+struct FrameTable {
+    unsigned char* dev = nullptr;
+    unsigned char* host = nullptr;       // hipHostMalloc
+    size_t cap = 0;                      // frames
+    LaunchDesc* desc() const { return reinterpret_cast<LaunchDesc*>(dev); }
+    FrameDev* frames() const { return reinterpret_cast<FrameDev*>(dev + sizeof(LaunchDesc)); }
+    LaunchDesc* hdesc() const { return reinterpret_cast<LaunchDesc*>(host); }
+    FrameDev* hframes() const { return reinterpret_cast<FrameDev*>(host + sizeof(LaunchDesc)); }
+    size_t bytes(size_t n) const { return sizeof(LaunchDesc) + n * sizeof(FrameDev); }
+};
+
 struct kt_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -521,14 +552,17 @@ struct kt_hip_ctx {
     char name[256] = {0};
     int n_cu = 0;
     int has_fp64 = 1;
-    FrameDev* d_frames = nullptr; size_t cap_frames = 0;
+    FrameTable table;
+    FrameDev* d_frames = nullptr;                        // = table.frames()
     uint32_t* d_geo = nullptr; size_t cap_pix = 0;
     uint32_t* d_rgba = nullptr;
     uint8_t* d_bg = nullptr; int bg_w = 0, bg_h = 0; size_t cap_bg = 0;
     unsigned long long* d_words = nullptr;
     uint8_t* d_rgb = nullptr; size_t cap_rgb = 0;       // camera-spp resolved RGB8
     double* d_tab = nullptr; double tab_gamma = -1.0;  // camera-spp linear table + thresholds
-    std::vector<FrameDev> h_frames;
+    // autotuner results per (chart, integrator, precision, bundles)
+    kt_tune tune[2][2][2][2];
+    bool tuned[2][2][2][2] = {};
 };
 
 static int fail(kt_hip_ctx* c, int code, const char* fmt, ...) {
@@ -603,7 +637,8 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(c->d_frames);
+    (void)hipFree(c->table.dev);
+    if (c->table.host) (void)hipHostFree(c->table.host);
     (void)hipFree(c->d_geo);
     (void)hipFree(c->d_rgba);
     (void)hipFree(c->d_bg);
@@ -663,10 +698,25 @@ static int frame_chart(const kt_opts* o, const kt_frame& f) {
     return o->chart;
 }
 
-static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o,
-                         int64_t* total_rays, int64_t* total_pix, int* chart_out) {
+// (Re)allocate a frame table for n frames (device + pinned staging).
+static int table_reserve(kt_hip_ctx* c, FrameTable& t, size_t n) {
+    if (n <= t.cap) return KT_OK;
+    (void)hipFree(t.dev);
+    if (t.host) (void)hipHostFree(t.host);
+    t.dev = nullptr; t.host = nullptr; t.cap = 0;
+    if (hipMalloc(&t.dev, t.bytes(n)) != hipSuccess) return fail(c, KT_E_ALLOC, "frame table allocation failed");
+    if (hipHostMalloc(&t.host, t.bytes(n), hipHostMallocDefault) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "pinned frame staging allocation failed");
+    t.cap = n;
+    return KT_OK;
+}
+
+// Fill the staging copy of the table (descriptor + frames) for a frame list.
+static int prepare_frames(kt_hip_ctx* c, FrameTable& t, const kt_frame* frames, int n_frames, const kt_opts* o,
+                          int64_t* total_rays, int64_t* total_pix, int* chart_out) {
     if (!frames || n_frames <= 0) return fail(c, KT_E_ARG, "no frames");
-    c->h_frames.assign(n_frames, FrameDev());
+    if ((size_t)n_frames > t.cap) return fail(c, KT_E_ARG, "frame table capacity %zu < %d frames", t.cap, n_frames);
+    FrameDev* hf = t.hframes();
     int64_t rays = 0, pix = 0;
     int chart = -1;
     for (int i = 0; i < n_frames; ++i) {
@@ -676,7 +726,7 @@ static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, co
         const int ch = frame_chart(o, f);
         if (chart < 0) chart = ch;
         else if (chart != ch) return fail(c, KT_E_UNSUPPORTED, "frames of one call must share the effective chart");
-        FrameDev& F = c->h_frames[i];
+        FrameDev& F = hf[i];
         std::memset(&F, 0, sizeof(F));
         F.M = f.M; F.a = f.a; F.Q = f.Q; F.Lambda = f.Lambda;
         F.r_obs = f.r_obs;
@@ -697,27 +747,39 @@ static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, co
         rays += (int64_t)F.tiles_x * F.tiles_y * 64;
         pix += (int64_t)f.width * f.height;
     }
-    if ((size_t)n_frames > c->cap_frames) {
-        (void)hipFree(c->d_frames);
-        c->d_frames = nullptr;
-        c->cap_frames = 0;
-        if (hipMalloc(&c->d_frames, n_frames * sizeof(FrameDev)) != hipSuccess)
-            return fail(c, KT_E_ALLOC, "frame table allocation failed");
-        c->cap_frames = n_frames;
-    }
-    if ((size_t)pix > c->cap_pix) {
-        (void)(void)hipFree(c->d_geo); (void)hipFree(c->d_rgba);
-        c->d_geo = nullptr; c->d_rgba = nullptr; c->cap_pix = 0;
-        if (hipMalloc(&c->d_geo, (size_t)pix * 28) != hipSuccess ||
-            hipMalloc(&c->d_rgba, (size_t)pix * 4) != hipSuccess)
-            return fail(c, KT_E_ALLOC, "output allocation (%lld pixels) failed", (long long)pix);
-        c->cap_pix = pix;
-    }
-    KT_HIP(c, hipMemcpyAsync(c->d_frames, c->h_frames.data(), n_frames * sizeof(FrameDev), hipMemcpyHostToDevice,
-                             c->stream));
+    LaunchDesc* d = t.hdesc();
+    std::memset(d, 0, sizeof(*d));
+    d->n_frames = n_frames;
+    d->total_rays = rays;
+    d->n_pix = pix;
     *total_rays = rays;
     *total_pix = pix;
     *chart_out = chart;
+    return KT_OK;
+}
+
+static int reserve_pixels(kt_hip_ctx* c, uint32_t** geo, uint32_t** rgba, size_t* cap, int64_t pix) {
+    if ((size_t)pix <= *cap) return KT_OK;
+    (void)hipFree(*geo); (void)hipFree(*rgba);
+    *geo = nullptr; *rgba = nullptr; *cap = 0;
+    if (hipMalloc(geo, (size_t)pix * 28) != hipSuccess || hipMalloc(rgba, (size_t)pix * 4) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "output allocation (%lld pixels) failed", (long long)pix);
+    *cap = pix;
+    return KT_OK;
+}
+
+// Frame table upload + output capacity of the context's own buffers.
+static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o,
+                         int64_t* total_rays, int64_t* total_pix, int* chart_out) {
+    if (!frames || n_frames <= 0) return fail(c, KT_E_ARG, "no frames");
+    int rc = table_reserve(c, c->table, (size_t)n_frames);
+    if (rc) return rc;
+    c->d_frames = c->table.frames();
+    rc = prepare_frames(c, c->table, frames, n_frames, o, total_rays, total_pix, chart_out);
+    if (rc) return rc;
+    rc = reserve_pixels(c, &c->d_geo, &c->d_rgba, &c->cap_pix, *total_pix);
+    if (rc) return rc;
+    KT_HIP(c, hipMemcpyAsync(c->table.dev, c->table.host, c->table.bytes(n_frames), hipMemcpyHostToDevice, c->stream));
     return KT_OK;
 }
 
@@ -733,9 +795,94 @@ static RunOpts run_opts(const kt_opts* o) {
     return R;
 }
 
+// launch configuration of the trace kernel: explicit tuning fields, else the
+// autotuner's result for this option class, else the built-in defaults
+struct Launch { int bpc, chunk, occ; };
+
+static Launch launch_config(const kt_hip_ctx* c, const kt_opts* o, int chart) {
+    Launch L{kDefaultBlocksPerCu, kDefaultChunk, 0};
+    const int b = o->bundles ? 1 : 0, m = o->integrator ? 1 : 0, pr = o->precision ? 1 : 0, ch = chart ? 1 : 0;
+    if (c->tuned[ch][m][pr][b]) {
+        const kt_tune& t = c->tune[ch][m][pr][b];
+        L.bpc = t.blocks_per_cu; L.chunk = t.chunk; L.occ = t.occupancy;
+    }
+    if (o->tuning_blocks_per_cu > 0) L.bpc = o->tuning_blocks_per_cu;
+    if (o->tuning_chunk > 0) L.chunk = o->tuning_chunk;
+    if (o->tuning_occupancy > 0) L.occ = o->tuning_occupancy;
+    return L;
+}
+
+struct RenderBufs {
+    LaunchDesc* desc;
+    FrameDev* frames;
+    uint32_t* geo;
+    uint32_t* rgba;
+    unsigned long long* words;
+};
+
+// Enqueue fill + setup + persistent trace (+ shade) on the context stream.  Grid
+// sizes depend only on capacities, so the same sequence can be captured in a graph.
+// ev: 4 events (setup start, trace start, trace end, end) or nullptr.
+static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, int chart, int setup_grid,
+                          int64_t max_rays, bool single, bool shade, const Launch& L, hipEvent_t* ev) {
+    const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
+    TraceKernelFn fn = o->bundles ? (single ? kt_bundle_kernel<true> : kt_bundle_kernel<false>)
+                                  : pick_kernel(chart, o->integrator, o->precision, phi, L.occ, single);
+    if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");
+    // persistent grid: blocks beyond the resident limit start when others retire and
+    // find the queue drained, so over-subscription is harmless
+    int64_t grid = (int64_t)c->n_cu * (L.bpc > 0 ? L.bpc : kDefaultBlocksPerCu);
+    const int64_t max_useful = (max_rays + kBlock - 1) / kBlock;
+    if (grid > max_useful) grid = max_useful;
+    if (grid < 1) grid = 1;
+
+    TraceArgs A;
+    A.frames = B.frames;
+    A.desc = B.desc;
+    A.geo = B.geo;
+    A.words = B.words;
+    A.O = run_opts(o);
+    A.chunk = L.chunk > 0 ? L.chunk : kDefaultChunk;
+
+    hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, B.words, (int)W_COUNT);
+    if (ev) KT_HIP(c, hipEventRecord(ev[0], c->stream));
+    hipLaunchKernelGGL(kt_setup_kernel, dim3(setup_grid), dim3(kSetupThreads), 0, c->stream, B.frames, B.desc);
+    if (ev) KT_HIP(c, hipEventRecord(ev[1], c->stream));
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(kBlock), 0, c->stream, A);
+    KT_HIP(c, hipGetLastError());
+    if (ev) KT_HIP(c, hipEventRecord(ev[2], c->stream));
+    if (shade) {
+        ShadeArgs S;
+        S.frames = B.frames; S.desc = B.desc; S.n_frames = 0; S.n_pix = 0;
+        S.geo = B.geo; S.rgba = B.rgba;
+        S.bg = c->bg_w > 0 ? c->d_bg : nullptr; S.bw = c->bg_w; S.bh = c->bg_h;
+        S.exposure = o->exposure; S.gamma = o->gamma; S.temp_scale = o->temp_scale;
+        S.M_override = 1.0; S.risco_override = 6.0;
+        hipLaunchKernelGGL(kt_shade_kernel, dim3((unsigned)(c->n_cu * 8)), dim3(256), 0, c->stream, S);
+        KT_HIP(c, hipGetLastError());
+    }
+    if (ev) KT_HIP(c, hipEventRecord(ev[3], c->stream));
+    return KT_OK;
+}
+
+static int read_stats(kt_hip_ctx* c, const unsigned long long* d_words, float t_trace, float t_total, kt_stats* st) {
+    if (!st) return KT_OK;
+    unsigned long long w[W_COUNT];
+    KT_HIP(c, hipMemcpy(w, d_words, sizeof(w), hipMemcpyDeviceToHost));
+    st->trace_ms = t_trace;
+    st->total_ms = t_total;
+    st->rays = w[W_RAYS];
+    st->steps_accepted = w[W_ACC];
+    st->steps_rejected = w[W_REJ];
+    st->rhs_fp32 = w[W_RHS32];
+    st->rhs_fp64 = w[W_RHS64];
+    st->events = w[W_HERM];
+    return KT_OK;
+}
+
 // setup + trace + shade on the context stream; outputs stay on the device
 static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o, bool shade,
-                            kt_stats* st, int64_t* n_pix_out) {
+                            kt_stats* st, int64_t* n_pix_out, const Launch* force = nullptr) {
     int rc = check_opts(c, o);
     if (rc) return rc;
     KT_HIP(c, hipSetDevice(c->device));
@@ -743,64 +890,16 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     int chart = 0;
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
-    const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
-    const bool single = n_frames == 1;
-    TraceKernelFn fn = o->bundles ? (single ? kt_bundle_kernel<true> : kt_bundle_kernel<false>)
-                                  : pick_kernel(chart, o->integrator, o->precision, phi, o->tuning_occupancy, single);
-    if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");
-
-    // persistent grid: blocks beyond the resident limit simply start when others retire
-    // and drain the queue tail, so over-subscription is harmless
-    int per_cu = o->tuning_blocks_per_cu > 0 ? o->tuning_blocks_per_cu : kDefaultBlocksPerCu;
-    int64_t grid = (int64_t)c->n_cu * per_cu;
-    const int64_t max_useful = (rays + kBlock - 1) / kBlock;
-    if (grid > max_useful) grid = max_useful;
-    if (grid < 1) grid = 1;
-
-    TraceArgs A;
-    A.frames = c->d_frames;
-    A.n_frames = n_frames;
-    A.single = (n_frames == 1) ? 1 : 0;
-    A.total_rays = rays;
-    A.geo = c->d_geo;
-    A.words = c->d_words;
-    A.O = run_opts(o);
-    A.chunk = o->tuning_chunk > 0 ? o->tuning_chunk : kDefaultChunk;
-
-    hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
-    KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(kt_setup_kernel, dim3(n_frames), dim3(kSetupThreads), 0, c->stream, c->d_frames);
-    KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(kBlock), 0, c->stream, A);
-    KT_HIP(c, hipGetLastError());
-    KT_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    if (shade) {
-        ShadeArgs S;
-        S.frames = c->d_frames; S.n_frames = n_frames; S.n_pix = pix;
-        S.geo = c->d_geo; S.rgba = c->d_rgba;
-        S.bg = c->bg_w > 0 ? c->d_bg : nullptr; S.bw = c->bg_w; S.bh = c->bg_h;
-        S.exposure = o->exposure; S.gamma = o->gamma; S.temp_scale = o->temp_scale;
-        S.M_override = 1.0; S.risco_override = 6.0;
-        hipLaunchKernelGGL(kt_shade_kernel, dim3((unsigned)((pix + 255) / 256)), dim3(256), 0, c->stream, S);
-        KT_HIP(c, hipGetLastError());
-    }
-    KT_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    const Launch L = force ? *force : launch_config(c, o, chart);
+    RenderBufs B{c->table.desc(), c->table.frames(), c->d_geo, c->d_rgba, c->d_words};
+    rc = enqueue_render(c, B, o, chart, n_frames, rays, n_frames == 1, shade, L, c->ev);
+    if (rc) return rc;
     KT_HIP(c, hipStreamSynchronize(c->stream));
-    if (st) {
-        unsigned long long w[W_COUNT];
-        KT_HIP(c, hipMemcpy(w, c->d_words, sizeof(w), hipMemcpyDeviceToHost));
-        float t_trace = 0.f, t_total = 0.f;
-        (void)hipEventElapsedTime(&t_trace, c->ev[1], c->ev[2]);
-        (void)hipEventElapsedTime(&t_total, c->ev[0], c->ev[3]);
-        st->trace_ms = t_trace;
-        st->total_ms = t_total;
-        st->rays = w[W_RAYS];
-        st->steps_accepted = w[W_ACC];
-        st->steps_rejected = w[W_REJ];
-        st->rhs_fp32 = w[W_RHS32];
-        st->rhs_fp64 = w[W_RHS64];
-        st->events = w[W_HERM];
-    }
+    float t_trace = 0.f, t_total = 0.f;
+    (void)hipEventElapsedTime(&t_trace, c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&t_total, c->ev[0], c->ev[3]);
+    rc = read_stats(c, c->d_words, t_trace, t_total, st);
+    if (rc) return rc;
     *n_pix_out = pix;
     return KT_OK;
 }
@@ -850,7 +949,8 @@ int kt_hip_frame_info(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const
     int chart = 0;
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
-    hipLaunchKernelGGL(kt_setup_kernel, dim3(n_frames), dim3(kSetupThreads), 0, c->stream, c->d_frames);
+    hipLaunchKernelGGL(kt_setup_kernel, dim3(n_frames), dim3(kSetupThreads), 0, c->stream, c->d_frames,
+                       c->table.desc());
     KT_HIP(c, hipGetLastError());
     KT_HIP(c, hipStreamSynchronize(c->stream));
     std::vector<FrameDev> fr(n_frames);
@@ -880,7 +980,7 @@ int kt_hip_colorize(kt_hip_ctx* c, const kt_geopixel* geo, int64_t n, double M, 
     }
     KT_HIP(c, hipMemcpyAsync(c->d_geo, geo, (size_t)n * 28, hipMemcpyHostToDevice, c->stream));
     ShadeArgs S;
-    S.frames = nullptr; S.n_frames = 0; S.n_pix = n;
+    S.frames = nullptr; S.desc = nullptr; S.n_frames = 0; S.n_pix = n;
     S.geo = c->d_geo; S.rgba = c->d_rgba;
     S.bg = c->bg_w > 0 ? c->d_bg : nullptr; S.bw = c->bg_w; S.bh = c->bg_h;
     S.exposure = o->exposure; S.gamma = o->gamma; S.temp_scale = o->temp_scale;
@@ -1104,6 +1204,247 @@ int kt_read_image_rgb(const char* path, uint8_t* rgb_out, int64_t cap, int32_t* 
         std::memcpy(rgb_out, rgb.data(), rgb.size());
     }
     return KT_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+//  autotuner and graph-captured batched plans
+// ============================================================================
+static void tune_slot(const kt_opts* o, int chart, int idx[4]) {
+    idx[0] = chart ? 1 : 0; idx[1] = o->integrator ? 1 : 0; idx[2] = o->precision ? 1 : 0; idx[3] = o->bundles ? 1 : 0;
+}
+
+static int copy_frame_info(kt_hip_ctx* c, const FrameDev* d_frames, int n_frames, int integrator, kt_frame_info* out) {
+    std::vector<FrameDev> fr(n_frames);
+    KT_HIP(c, hipMemcpy(fr.data(), d_frames, n_frames * sizeof(FrameDev), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n_frames; ++i) {
+        out[i].r_horizon = fr[i].r_h;
+        out[i].r_isco = fr[i].r_isco;
+        out[i].r_disk_in = fr[i].r_in;
+        out[i].r_disk_out = fr[i].r_out;
+        out[i].r_escape = fr[i].r_esc;
+        out[i].eff_chart = fr[i].chart;
+        out[i].eff_integrator = integrator;
+    }
+    return KT_OK;
+}
+
+struct kt_hip_plan {
+    kt_hip_ctx* c = nullptr;
+    kt_opts o;
+    int max_frames = 0;
+    int64_t max_pix = 0;
+    FrameTable table;
+    uint32_t* d_geo = nullptr;
+    uint32_t* d_rgba = nullptr;
+    size_t cap_pix = 0;
+    unsigned long long* d_words = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int chart = -1;
+    Launch L{0, 0, 0};
+    kt_tune tune;
+    const uint8_t* bg = nullptr;     // background baked into the captured shade launch
+    int bg_w = 0, bg_h = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static void plan_drop_graph(kt_hip_plan* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+static int plan_capture(kt_hip_plan* p) {
+    kt_hip_ctx* c = p->c;
+    RenderBufs B{p->table.desc(), p->table.frames(), p->d_geo, p->d_rgba, p->d_words};
+    KT_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = KT_OK;
+    if (hipMemcpyAsync(p->table.dev, p->table.host, p->table.bytes(p->max_frames), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+        rc = fail(c, KT_E_DEVICE, "frame table copy could not be captured");
+    if (!rc)
+        rc = enqueue_render(c, B, &p->o, p->chart, p->max_frames, (int64_t)1 << 40, p->max_frames == 1, true, p->L,
+                            nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess || !g) return fail(c, KT_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    p->graph = g;
+    KT_HIP(c, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    p->bg = c->bg_w > 0 ? c->d_bg : nullptr;
+    p->bg_w = c->bg_w;
+    p->bg_h = c->bg_h;
+    return KT_OK;
+}
+
+static int plan_run(kt_hip_plan* p, const kt_frame* frames, int n_frames, kt_stats* st, int64_t* pix_out) {
+    kt_hip_ctx* c = p->c;
+    KT_HIP(c, hipSetDevice(c->device));
+    if (n_frames > p->max_frames) return fail(c, KT_E_ARG, "%d frames exceed the plan's %d", n_frames, p->max_frames);
+    int64_t rays = 0, pix = 0;
+    int chart = 0;
+    int rc = prepare_frames(c, p->table, frames, n_frames, &p->o, &rays, &pix, &chart);
+    if (rc) return rc;
+    if (pix > p->max_pix) return fail(c, KT_E_ARG, "%lld pixels exceed the plan's %lld", (long long)pix,
+                                      (long long)p->max_pix);
+    if (p->exec && chart != p->chart)
+        return fail(c, KT_E_UNSUPPORTED, "the plan was captured for another effective chart");
+    const uint8_t* bg = c->bg_w > 0 ? c->d_bg : nullptr;
+    if (p->exec && (bg != p->bg || c->bg_w != p->bg_w || c->bg_h != p->bg_h)) plan_drop_graph(p);
+    if (!p->exec) {
+        p->chart = chart;
+        int idx[4];
+        tune_slot(&p->o, chart, idx);
+        const bool explicit_tuning = p->o.tuning_blocks_per_cu > 0 && p->o.tuning_chunk > 0;
+        if (!explicit_tuning && !c->tuned[idx[0]][idx[1]][idx[2]][idx[3]]) {
+            rc = kt_hip_autotune(c, frames, n_frames, &p->o, &p->tune);
+            if (rc) return rc;
+        }
+        p->L = launch_config(c, &p->o, chart);
+        if (c->tuned[idx[0]][idx[1]][idx[2]][idx[3]] && !explicit_tuning) p->tune = c->tune[idx[0]][idx[1]][idx[2]][idx[3]];
+        else { std::memset(&p->tune, 0, sizeof(p->tune)); }
+        p->tune.blocks_per_cu = p->L.bpc; p->tune.chunk = p->L.chunk; p->tune.occupancy = p->L.occ > 0 ? p->L.occ : kDefaultOcc;
+        // the autotuner rendered through the context's own table; refill the plan's staging
+        rc = prepare_frames(c, p->table, frames, n_frames, &p->o, &rays, &pix, &chart);
+        if (rc) return rc;
+        rc = plan_capture(p);
+        if (rc) return rc;
+    }
+    KT_HIP(c, hipEventRecord(p->ev[0], c->stream));
+    KT_HIP(c, hipGraphLaunch(p->exec, c->stream));
+    KT_HIP(c, hipEventRecord(p->ev[1], c->stream));
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p->ev[0], p->ev[1]);
+    rc = read_stats(c, p->d_words, ms, ms, st);
+    if (rc) return rc;
+    *pix_out = pix;
+    return KT_OK;
+}
+
+extern "C" {
+
+int kt_hip_autotune(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o, kt_tune* out) {
+    if (!c) return KT_E_ARG;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    if (!frames || n_frames <= 0) return fail(c, KT_E_ARG, "no frames");
+    int64_t total = 0;
+    for (int i = 0; i < n_frames; ++i) total += (int64_t)frames[i].width * frames[i].height;
+    const double target = 2097152.0;
+    const double sc = (double)total > target ? std::sqrt(target / (double)total) : 1.0;
+    std::vector<kt_frame> pf(frames, frames + n_frames);
+    int64_t probe = 0;
+    for (kt_frame& f : pf) {       // same fov and aspect: the scaled frame sees the same sky
+        f.width = std::max(8, (int)std::lround(f.width * sc));
+        f.height = std::max(8, (int)std::lround(f.height * sc));
+        probe += (int64_t)f.width * f.height;
+    }
+    static const int bpcs[4] = {2, 3, 4, 6};
+    static const int chunks[3] = {32, 64, 128};
+    kt_stats st;
+    int64_t pix = 0;
+    Launch def{kDefaultBlocksPerCu, kDefaultChunk, 0};
+    rc = render_on_device(c, pf.data(), n_frames, o, false, &st, &pix, &def);   // warm-up
+    if (rc) return rc;
+    kt_tune best;
+    std::memset(&best, 0, sizeof(best));
+    best.best_ms = 1e300;
+    best.default_ms = -1.0;
+    for (int b : bpcs)
+        for (int ch : chunks) {
+            Launch L{b, ch, 0};
+            double ms = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {
+                rc = render_on_device(c, pf.data(), n_frames, o, false, &st, &pix, &L);
+                if (rc) return rc;
+                ms = std::min(ms, st.trace_ms);
+            }
+            ++best.candidates;
+            if (b == kDefaultBlocksPerCu && ch == kDefaultChunk) best.default_ms = ms;
+            if (ms < best.best_ms) { best.best_ms = ms; best.blocks_per_cu = b; best.chunk = ch; }
+        }
+    best.occupancy = kDefaultOcc;
+    best.probe_rays = probe;
+    int idx[4];
+    tune_slot(o, frame_chart(o, pf[0]), idx);
+    c->tune[idx[0]][idx[1]][idx[2]][idx[3]] = best;
+    c->tuned[idx[0]][idx[1]][idx[2]][idx[3]] = true;
+    if (out) *out = best;
+    return KT_OK;
+}
+
+int kt_hip_plan_create(kt_hip_ctx* c, const kt_opts* o, int max_frames, int64_t max_pixels, kt_hip_plan** out) {
+    if (!c || !out) return KT_E_ARG;
+    *out = nullptr;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    if (max_frames <= 0 || max_pixels <= 0) return fail(c, KT_E_ARG, "plan capacity must be positive");
+    KT_HIP(c, hipSetDevice(c->device));
+    kt_hip_plan* p = new kt_hip_plan();
+    p->c = c;
+    p->o = *o;
+    p->max_frames = max_frames;
+    p->max_pix = max_pixels;
+    std::memset(&p->tune, 0, sizeof(p->tune));
+    rc = table_reserve(c, p->table, (size_t)max_frames);
+    if (!rc) rc = reserve_pixels(c, &p->d_geo, &p->d_rgba, &p->cap_pix, max_pixels);
+    if (!rc && hipMalloc(&p->d_words, W_COUNT * sizeof(unsigned long long)) != hipSuccess)
+        rc = fail(c, KT_E_ALLOC, "plan counter allocation failed");
+    for (int i = 0; i < 2 && !rc; ++i)
+        if (hipEventCreate(&p->ev[i]) != hipSuccess) rc = fail(c, KT_E_DEVICE, "event creation failed");
+    if (rc) { kt_hip_plan_destroy(p); return rc; }
+    std::memset(p->table.host, 0, p->table.bytes((size_t)max_frames));
+    *out = p;
+    return KT_OK;
+}
+
+int kt_hip_plan_render(kt_hip_plan* p, const kt_frame* frames, int n_frames, uint32_t* rgba_out, kt_geopixel* geo_out,
+                       kt_frame_info* info_out, kt_stats* st) {
+    if (!p) return KT_E_ARG;
+    kt_hip_ctx* c = p->c;
+    int64_t pix = 0;
+    int rc = plan_run(p, frames, n_frames, st, &pix);
+    if (rc) return rc;
+    if (rgba_out) KT_HIP(c, hipMemcpy(rgba_out, p->d_rgba, (size_t)pix * 4, hipMemcpyDeviceToHost));
+    if (geo_out) KT_HIP(c, hipMemcpy(geo_out, p->d_geo, (size_t)pix * 28, hipMemcpyDeviceToHost));
+    if (info_out) return copy_frame_info(c, p->table.frames(), n_frames, p->o.integrator, info_out);
+    return KT_OK;
+}
+
+int kt_hip_plan_render_device(kt_hip_plan* p, const kt_frame* frames, int n_frames, void** d_rgba, void** d_geo,
+                              kt_stats* st) {
+    if (!p) return KT_E_ARG;
+    int64_t pix = 0;
+    const int rc = plan_run(p, frames, n_frames, st, &pix);
+    if (rc) return rc;
+    if (d_rgba) *d_rgba = p->d_rgba;
+    if (d_geo) *d_geo = p->d_geo;
+    return KT_OK;
+}
+
+int kt_hip_plan_tuning(const kt_hip_plan* p, kt_tune* out) {
+    if (!p || !out) return KT_E_ARG;
+    *out = p->tune;
+    return KT_OK;
+}
+
+void kt_hip_plan_destroy(kt_hip_plan* p) {
+    if (!p) return;
+    kt_hip_ctx* c = p->c;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    plan_drop_graph(p);
+    (void)hipFree(p->table.dev);
+    if (p->table.host) (void)hipHostFree(p->table.host);
+    (void)hipFree(p->d_geo);
+    (void)hipFree(p->d_rgba);
+    (void)hipFree(p->d_words);
+    for (int i = 0; i < 2; ++i) if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
+    delete p;
 }
 
 }  // extern "C"
