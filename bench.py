@@ -86,6 +86,23 @@ def c4_run(ctx, kt, steps):
             "rays_per_s": npix / wall, "steps_accepted": sts[-1]["steps_accepted"], "autotune": tune}
 
 
+def c5_run(ctx, kt, reps):
+    """Differentiable render: 256x256 Kerr-Newman a=0.7 Q=0.2 KS theta=60 DOPRI5; one adjoint call =
+    counting forward + recording forward (step sizes, checkpoints every 16 steps) + reverse pass."""
+    fr = kt.make_frame(256, 256, a=0.7, Q=0.2, Lambda=0.0, theta=60.0, r_obs=40.0, fov=30.0, disk_out=12.0)
+    o = kt.default_opts()
+    o.chart, o.integrator, o.rtol, o.atol = kt.KT_CHART_KS, kt.KT_INTEGRATOR_DOPRI5, RTOL, ATOL
+    ctx.render_adjoint(fr, o)
+    t0 = time.perf_counter()
+    outs = [ctx.render_adjoint(fr, o) for _ in range(reps)]
+    wall = (time.perf_counter() - t0) / reps
+    st = outs[-1]["stats"]
+    return {"workload": "256x256 KN a=0.7 Q=0.2 KS theta=60 DOPRI5, dI/d(a,Q,Lambda,theta,r_obs) per pixel",
+            "ms_per_call": 1e3 * wall, "forward_ms": st["forward_ms"], "record_ms": st["record_ms"],
+            "backward_ms": st["backward_ms"], "rays_differentiated": st["rays_differentiated"],
+            "steps_recorded": st["steps"], "checkpoints": st["checkpoints"], "record_bytes": st["record_bytes"]}
+
+
 def accuracy(geo, ref, px, py, band_geo):
     """Per-pixel agreement with the oracle outside the photon-ring band (fate boundaries + the
     steep inner rim, |g - g_neighbour| > 0.05), the band taken from the full FP64 GPU image."""
@@ -222,6 +239,7 @@ def main():
     }
     if not a.no_c4:
         result["batched_c4"] = c4_run(ctx, kt, 3)
+        result["adjoint_c5"] = c5_run(ctx, kt, 3)
     if not a.no_cpu:
         cpu, ref, px, py = cpu_baseline(a.cpu_budget)
         result["cpu_baseline"] = cpu

@@ -207,6 +207,35 @@ int kt_hip_plan_render_device(kt_hip_plan* plan, const kt_frame* frames, int n_f
 int kt_hip_plan_tuning(const kt_hip_plan* plan, kt_tune* out);
 void kt_hip_plan_destroy(kt_hip_plan* plan);
 
+/* Differentiable rendering (no reference counterpart).  The per-pixel disk intensity of the
+ * shading, I = PT(r_hit)/PT(3 r_isco) g^4 (1 + 0.85 max(0, 1 - g)) with g clamped to [0.1, 10]
+ * (main.cpp:1799-1810, magnification 1; 0 off the disk), of the binary64 DOPRI5 path (one frame;
+ * opts->integrator must be KT_INTEGRATOR_DOPRI5, precision is forced to FP64, no bundles).     */
+int kt_hip_render_intensity(kt_hip_ctx* ctx, const kt_frame* frame, const kt_opts* opts, double* intensity_out,
+                            kt_stats* stats);
+
+typedef struct kt_adjoint_stats {
+    double forward_ms;        /* counting forward pass                                       */
+    double record_ms;         /* recording forward pass (step sizes + checkpoints)             */
+    double backward_ms;       /* reverse pass                                                  */
+    int64_t steps;            /* accepted steps recorded                                       */
+    int64_t checkpoints;      /* state checkpoints stored (one per 16 accepted steps)          */
+    int64_t record_bytes;     /* device bytes of the step sizes and checkpoints                */
+    int64_t rays_differentiated;
+    int64_t seed_failures;    /* disk pixels whose last step gave no root on replay            */
+} kt_adjoint_stats;
+
+/* Hand-written adjoint of the adaptive integrator + disk shading.  Gradients of the per-pixel
+ * intensity with respect to p = (a, Q, Lambda, theta_obs [degrees], r_obs): a counting forward
+ * pass sizes the records, a recording pass stores every accepted step size and a checkpoint
+ * every 16 accepted steps, and the reverse pass replays each segment from its checkpoint and
+ * propagates the adjoint back through the DOPRI5 stages (accepted step sizes held fixed).
+ *   weights:       W*H values w (dL/dI), NULL for all ones
+ *   intensity_out: W*H (may be NULL);  dI_dp_out: W*H*5, pixel-major (may be NULL)
+ *   grad_out:      5 values, sum_p w_p dI_p/dp (summed on the host in pixel order)          */
+int kt_hip_render_adjoint(kt_hip_ctx* ctx, const kt_frame* frame, const kt_opts* opts, const double* weights,
+                          double* intensity_out, double* dI_dp_out, double* grad_out, kt_adjoint_stats* stats);
+
 /* Host utilities (no device work; callable without a GPU). */
 /* Offsets of the spp jittered passes, (x, y) pairs in pixels (build_offsets, main.cpp:2240-2279). */
 int kt_spp_offsets(int32_t spp, double* xy_out);

@@ -137,6 +137,29 @@ class Context:
                     "kt_hip_autotune")
         return t.as_dict()
 
+    def render_intensity(self, frame, opts):
+        """Disk intensity image (H, W) float64 of the binary64 DOPRI5 path, the quantity the
+        adjoint differentiates."""
+        I = np.zeros(frame.width * frame.height)
+        st = _abi.KtStats()
+        self._check(self.lib.kt_hip_render_intensity(self.h, ctypes.byref(frame), ctypes.byref(opts), _ptr(I),
+                                                     ctypes.byref(st)), "kt_hip_render_intensity")
+        return I.reshape(frame.height, frame.width), st.as_dict()
+
+    def render_adjoint(self, frame, opts, weights=None, want_pixel_grads=False):
+        """Gradient of sum_p w_p I_p with respect to (a, Q, Lambda, theta_obs [deg], r_obs).
+        Returns dict(grad (5,), intensity (H, W), dI_dp (H, W, 5) or None, stats)."""
+        n = frame.width * frame.height
+        w = None if weights is None else np.ascontiguousarray(np.asarray(weights, np.float64).reshape(n))
+        I = np.zeros(n)
+        dI = np.zeros((n, 5)) if want_pixel_grads else None
+        g = np.zeros(5)
+        st = _abi.KtAdjointStats()
+        self._check(self.lib.kt_hip_render_adjoint(self.h, ctypes.byref(frame), ctypes.byref(opts), _ptr(w), _ptr(I),
+                                                   _ptr(dI), _ptr(g), ctypes.byref(st)), "kt_hip_render_adjoint")
+        return {"grad": g, "intensity": I.reshape(frame.height, frame.width),
+                "dI_dp": None if dI is None else dI.reshape(frame.height, frame.width, 5), "stats": st.as_dict()}
+
     def plan(self, opts, max_frames, max_pixels):
         """A graph-captured batched render plan (see Plan)."""
         return Plan(self, opts, max_frames, max_pixels)

@@ -65,6 +65,15 @@ class KtTune(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class KtAdjointStats(ctypes.Structure):
+    _fields_ = [("forward_ms", ctypes.c_double), ("record_ms", ctypes.c_double), ("backward_ms", ctypes.c_double),
+                ("steps", ctypes.c_int64), ("checkpoints", ctypes.c_int64), ("record_bytes", ctypes.c_int64),
+                ("rays_differentiated", ctypes.c_int64), ("seed_failures", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class KtStats(ctypes.Structure):
     _fields_ = [("trace_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("rays", ctypes.c_uint64),
                 ("steps_accepted", ctypes.c_uint64), ("steps_rejected", ctypes.c_uint64),
@@ -92,6 +101,8 @@ PROTOTYPES = {
                                                    ctypes.c_int32, _P]),
     "kt_hip_render_spp": (ctypes.c_int, [_P, _P, _I, _I, _P, _P, _P]),
     "kt_hip_autotune": (ctypes.c_int, [_P, _P, _I, _P, _P]),
+    "kt_hip_render_intensity": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "kt_hip_render_adjoint": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "kt_hip_plan_create": (ctypes.c_int, [_P, _P, _I, ctypes.c_int64, _P]),
     "kt_hip_plan_render": (ctypes.c_int, [_P, _P, _I, _P, _P, _P, _P]),
     "kt_hip_plan_render_device": (ctypes.c_int, [_P, _P, _I, _P, _P, _P]),

@@ -1,0 +1,500 @@
+// ============================================================================
+//  kt_adjoint.hpp -- reverse-mode derivatives of the per-pixel disk intensity
+//
+//    I(pixel) = PT(r_hit)/PT(3 r_isco) * gc^4 * (1 + 0.85 clamp(1 - gc, 0, 1)),
+//    gc = clamp(g, 0.1, 10)    (the shading intensity of main.cpp:1799-1810 with
+//                               magnification 1; 0 off the disk)
+//
+//  with respect to p = (a, Q, Lambda, theta_obs [deg], r_obs), for the binary64
+//  DOPRI5 path.  The forward pass records every accepted step size and a state
+//  checkpoint (r, theta, p_r, p_theta, sin theta, cos theta) every kCkptEvery
+//  accepted steps.  The reverse pass walks the checkpoints backwards: it replays
+//  one segment from its checkpoint (the same arithmetic as the forward step),
+//  then propagates the adjoint through each step in reverse:
+//
+//    y+ = y + h sum_i b_i k_i,   k_i = f(Y_i),   Y_i = y + h sum_{j<i} a_ij k_j
+//    mu_i   = h b_i lam + h sum_{m>i} a_mi nu_m        (adjoint of k_i)
+//    nu_i   = J_y(Y_i)^T mu_i                          (adjoint of Y_i)
+//    lam   <- lam + sum_i nu_i,  grad_c += J_c^T mu_i,  grad_P += J_P^T mu_i
+//
+//  with the accepted step sizes held fixed (the discrete adjoint of the taken
+//  steps; the controller's dependence on p enters only at the tolerance level).
+//  Local Jacobians come from the templated right-hand side evaluated in dual
+//  numbers.  The seed differentiates the disk event: the Hermite root alpha of
+//  theta(alpha) = pi/2 by the implicit function theorem, r_hit = Hermite r at
+//  alpha, then redshift and intensity.  The differentiable path converges that
+//  root (Q_EXACT_ROOT); the reference's 8 bisections stop 2^-12 short, which
+//  leaves r_hit piecewise constant in p.  The initial state's Jacobian (camera
+//  tetrad, p_t from H = 0, the Kerr-Schild covector) comes from the templated
+//  camera map in dual numbers.  r_isco, which normalises the emissivity, moves
+//  with p through r_+ (isco_gradient).
+// ============================================================================
+#pragma once
+#include "kt_dual.hpp"
+#include "kt_shade.hpp"
+#include "kt_trace.hpp"
+
+namespace kt {
+
+static constexpr int kCkptEvery = 16;
+enum { AP_A = 0, AP_Q, AP_L, AP_TH, AP_R, AP_N };
+
+// ---------------------------------------------------------------------------
+//  templated mirrors of the camera / covector / redshift maps (the binary64
+//  kernels' own functions are these with T = double; a host test pins that)
+// ---------------------------------------------------------------------------
+template <class T> struct ObsT {
+    T gtt, gtp, grr, gth, gpp, U00, U03, U11, U22, U33, sq_rr, sq_th, u0, ep, et;
+    bool ok;
+};
+
+template <class T> KT_HD Knds<T> knds_t(double M, const T& a, const T& Q, const T& L) {
+    Knds<T> P;
+    P.a = a; P.a2 = a*a; P.M = T(M); P.M2 = T(2.0*M); P.Q2 = Q*Q;
+    P.lam3 = L/3.0; P.lam3a2 = L*a*a/3.0;
+    P.xi = 1.0 + L*a*a/3.0; P.xi2 = P.xi*P.xi;
+    return P;
+}
+
+// static observer constants at (r, theta) (frame_camera_consts, camera.hpp:66-142)
+template <class T> KT_HD void observer_t(double M, const T& a, const T& Q, const T& L, const T& r, const T& th,
+                                         ObsT<T>& C) {
+    T st, ct;
+    kt_sincos(th, &st, &ct);
+    const T sg = r*r + a*a*ct*ct;
+    const T dr = (r*r + a*a) * (1.0 - L*r*r/3.0) - 2.0*M*r + Q*Q;
+    const T dt = 1.0 + L*a*a*ct*ct/3.0;
+    const T xi = 1.0 + L*a*a/3.0;
+    const T x2 = xi*xi, s2 = st*st, ra = r*r + a*a;
+    const T den = sg*x2;
+    C.gtt = (-dr + dt*a*a*s2) / den;
+    C.gtp = a*s2*(dr - dt*ra) / den;
+    C.grr = sg/dr;
+    C.gth = sg/dt;
+    C.gpp = s2*(dt*ra*ra - dr*a*a*s2) / den;
+    const T pre = sg*dr*dt;
+    C.U00 = -x2*(dt*ra*ra - dr*a*a*s2) / pre;
+    C.U03 = a*x2*(dr - dt*ra) / pre;
+    C.U11 = dr/sg;
+    C.U22 = dt/sg;
+    C.U33 = (val(s2) > 1e-14) ? x2*(dr - dt*a*a*s2) / (pre*s2) : T(0.0);
+    C.sq_rr = sqrt(fabs(C.grr));
+    C.sq_th = sqrt(fabs(C.gth));
+    const T dphi = C.gpp - (C.gtp*C.gtp)/C.gtt;
+    C.ok = (val(C.gtt) < -1e-14) && isfinite(val(dphi)) && (val(dphi) > 1e-14) && isfinite(val(C.sq_rr)) &&
+           (val(C.sq_rr) > 1e-14) && isfinite(val(C.sq_th)) && (val(C.sq_th) > 1e-14);
+    if (C.ok) {
+        C.u0 = 1.0 / sqrt(-C.gtt);
+        C.ep = 1.0 / sqrt(dphi);
+        C.et = -C.gtp/C.gtt * C.ep;
+    } else {
+        C.u0 = T(0.0); C.ep = T(0.0); C.et = T(0.0);
+    }
+}
+
+// screen angles -> BL state (camera_shoot): st = r, theta, phi(0), p_r, p_theta, p_t, p_phi
+template <class T> KT_HD void camera_t(const ObsT<T>& C, const T& r_obs, const T& th_obs, double al, double be,
+                                       T st[7]) {
+    double sa, ca, sb, cb;
+    kt_sincos(al, &sa, &ca);
+    kt_sincos(be, &sb, &cb);
+    T ut, ur, uth, uph;
+    if (C.ok) {
+        const double nr = -ca*cb, nt = -sb, np = -sa*cb;
+        ut = C.u0 + np*C.et;
+        ur = nr / C.sq_rr;
+        uth = nt / C.sq_th;
+        uph = np*C.ep;
+    } else {
+        ut = T(1.0);
+        ur = -ca*cb / vmax(C.sq_rr, T(1e-14));
+        uth = -sb / vmax(C.sq_th, T(1e-14));
+        uph = -sa*cb / sqrt(vmax(fabs(C.gpp), T(1e-14)));
+    }
+    T pt = C.gtt*ut + C.gtp*uph;
+    const T pr = C.grr*ur;
+    const T pth = C.gth*uth;
+    const T pp = C.gtp*ut + C.gpp*uph;
+    const T A = C.U00, B = 2.0*C.U03*pp;
+    const T Cq = C.U11*pr*pr + C.U22*pth*pth + C.U33*pp*pp;
+    const T disc = B*B - 4.0*A*Cq;
+    if (val(disc) >= 0.0 && fabs(val(A)) > 1e-15) {
+        const T sq = sqrt(disc);
+        const T p1 = (-B - sq) / (2.0*A), p2 = (-B + sq) / (2.0*A);
+        pt = (val(p1) < 0.0) ? p1 : p2;
+        if (val(pt) > 0.0) pt = vmin(p1, p2);
+    }
+    st[0] = r_obs; st[1] = th_obs; st[2] = T(0.0);
+    st[3] = pr; st[4] = pth; st[5] = pt; st[6] = pp;
+}
+
+// BL state -> Kerr-Schild covector (bl_to_ks_covector)
+template <class T> KT_HD void bl_to_ks_t(const Knds<T>& P, bool ref_init, T st[7]) {
+    const T r = st[0], th = st[1];
+    T s, c;
+    kt_sincos(th, &s, &c);
+    const T s2 = s*s, c2 = c*c, ra = r*r + P.a2;
+    const T dR = ra*(1.0 - P.lam3*r*r) - P.M2*r + P.Q2;
+    const T dT = 1.0 + P.lam3a2*c2;
+    const T pr = st[3], pth = st[4], pt = st[5], pp = st[6];
+    if (ref_init) {
+        const T Pr = pr;
+        const T A = dR - 2.0*P.xi*ra + P.xi2*P.a2*s2/dT;
+        const T B = -2.0*dR*Pr + 2.0*P.xi*Pr*ra - 2.0*P.xi*P.a*pp + 2.0*P.xi2*P.a*pp/dT;
+        const T C = dR*Pr*Pr + 2.0*P.xi*Pr*P.a*pp + dT*pth*pth + P.xi2*pp*pp/(dT*s2);
+        const T disc = B*B - 4.0*A*C;
+        if (val(disc) >= 0.0 && fabs(val(A)) > 1e-15) {
+            const T sq = sqrt(disc);
+            const T p1 = (-B - sq) / (2.0*A), p2 = (-B + sq) / (2.0*A);
+            T pT = (val(p1) < 0.0) ? p1 : p2;
+            if (val(pT) > 0.0) pT = vmin(p1, p2);
+            st[5] = pT;
+            st[3] = Pr - pT;
+            return;
+        }
+    }
+    const T K = ra*pt + P.a*pp;
+    st[3] = pr - P.xi*K/dR;
+}
+
+// thin-disk redshift (omega_k, lower_equator, disk_redshift), clamped to [0, 20]
+template <class T> KT_HD T disk_redshift_t(double M, const T& a, const T& Q, const T& L, const T& r, const T& pt,
+                                           const T& pp) {
+    const double s = (val(a) < 0.0) ? 1.0 : -1.0;
+    const T Me = M - Q*Q/(2.0*r) + L*a*r*r/3.0;
+    const T sqm = sqrt(vmax(Me, T(0.0)));
+    const T den = r*sqrt(r) + s*a*sqm;
+    const T Om = (fabs(val(den)) < 1e-14) ? T(0.0) : s*sqm/den;
+    const T b = -pp / (-pt);
+    const double ct = kCosHalfPi;
+    const T sg = r*r + a*a*ct*ct;
+    const T dr = (r*r + a*a) * (1.0 - L*r*r/3.0) - 2.0*M*r + Q*Q;
+    const T dt = 1.0 + L*a*a*ct*ct/3.0;
+    const T xi = 1.0 + L*a*a/3.0;
+    const T x2 = xi*xi, ra = r*r + a*a, den2 = sg*x2;
+    const T gtt = (-dr + dt*a*a) / den2;
+    const T gtp = a*(dr - dt*ra) / den2;
+    const T gpp = (dt*ra*ra - dr*a*a) / den2;
+    const T d2 = -(gtt + 2.0*gtp*Om + gpp*Om*Om);
+    T red;
+    if (val(d2) <= 0.0) red = T(1.0);
+    else {
+        const T dv = fabs(1.0 - Om*b);
+        if (val(dv) < 1e-8) red = T(20.0);
+        else {
+            red = sqrt(d2) / dv;
+            if (!isfinite(val(red))) red = T(1.0);
+        }
+    }
+    return val(red) < 0.0 ? T(0.0) : (val(red) > 20.0 ? T(20.0) : red);
+}
+
+// disk intensity of the shading (main.cpp:1799-1810), magnification 1
+template <class T> KT_HD T disk_intensity_t(const T& r, const T& g, const T& risco) {
+    if (!(val(r) > val(risco))) return T(0.0);
+    const T rp = 3.0*risco;
+    const T pk = (1.0 - sqrt(risco/rp)) / (rp*rp*rp);
+    if (!(val(pk) > 0.0)) return T(0.0);
+    const T x = sqrt(risco/r);
+    const T pt = (1.0 - x) / (r*r*r);
+    const T gc = val(g) < 0.1 ? T(0.1) : (val(g) > 10.0 ? T(10.0) : g);
+    const T g2 = gc*gc;
+    const T lift = val(gc) < 1.0 ? 1.0 + 0.85*(1.0 - gc) : T(1.0);
+    return pt / pk * (g2*g2) * lift;
+}
+
+// Bardeen ISCO (isco_bardeen) in the arithmetic type T
+template <class T> KT_HD T isco_bardeen_t(double M, const T& a) {
+    if (fabs(val(a)) < 1e-10) return T(6.0*M);
+    const T ap = a/M;
+    const T z1 = 1.0 + cbrt(1.0 - ap*ap) * (cbrt(1.0 + ap) + cbrt(1.0 - ap));
+    const T z2 = sqrt(3.0*ap*ap + z1*z1);
+    return M*(3.0 + z2 - sqrt((3.0 - z1)*(3.0 + z1 + 2.0*z2)));
+}
+
+// d r_isco / d(a, Q, Lambda) of the frame's ISCO.  The scan starts at 1.05 r_+ and moves in
+// fixed steps of 1e-4 M, so between its jumps r_isco follows 1.05 dr_+/dp, with dr_+/dp from
+// Delta_r(r_+) = 0; with Q = Lambda = 0 the Bardeen closed form is used (its a-derivative).
+KT_HD void isco_gradient(const FrameDev& F, double d[3]) {
+    if (fabs(F.Lambda) < 1e-15 && fabs(F.Q) < 1e-15) {
+        const DN<1> r = isco_bardeen_t<DN<1>>(F.M, DN<1>::var(F.a, 0));
+        d[0] = r.d[0]; d[1] = 0.0; d[2] = 0.0;
+        return;
+    }
+    const double r = F.r_h, a = F.a, L = F.Lambda, ra = r*r + a*a;
+    const double dDr = 2.0*r*(1.0 - L*r*r/3.0) - ra*(2.0*L*r/3.0) - 2.0*F.M;
+    if (!(fabs(dDr) > 0.0)) { d[0] = d[1] = d[2] = 0.0; return; }
+    d[0] = -1.05 * 2.0*a*(1.0 - L*r*r/3.0) / dDr;
+    d[1] = -1.05 * 2.0*F.Q / dDr;
+    d[2] = 1.05 * ra*r*r/3.0 / dDr;
+}
+
+// ---------------------------------------------------------------------------
+//  local Jacobians
+// ---------------------------------------------------------------------------
+// J[i][k] = d f_i / d x_k,  x = (r, theta, p_r, p_theta, p_t, p_phi, a, Q, Lambda), i < 4;
+// three passes of three tangent directions (register economy)
+template <int CHART>
+KT_HD void rhs_jacobian(double M, const double par[3], const double y[4], double pt, double pp, double J[4][9]) {
+    typedef DN<3> D;
+    for (int g = 0; g < 3; ++g) {
+        const int o = 3*g;
+        const D a = D::var(par[0], 6 - o), Q = D::var(par[1], 7 - o), L = D::var(par[2], 8 - o);
+        const D r = D::var(y[0], 0 - o), th = D::var(y[1], 1 - o);
+        const D pr = D::var(y[2], 2 - o), pth = D::var(y[3], 3 - o);
+        const D ptd = D::var(pt, 4 - o), ppd = D::var(pp, 5 - o);
+        const Knds<D> P = knds_t<D>(M, a, Q, L);
+        SinCos<D> sc;
+        kt_sincos(th, &sc.s, &sc.c);
+        D out[5];
+        rhs<CHART, D, D>(P, r, sc, pr, pth, ptd, ppd, out);
+        for (int i = 0; i < 4; ++i)
+            for (int k = 0; k < 3; ++k) J[i][o + k] = out[i].d[k];
+    }
+}
+
+// initial state y0 = (r, theta, p_r, p_theta), c = (p_t, p_phi) of a pixel and their
+// derivatives with respect to p (theta_obs in degrees)
+template <int CHART>
+KT_HD void init_jacobian(const FrameDev& F, double al, double be, double y0[4], double c[2], double Jy[4][AP_N],
+                         double Jc[2][AP_N]) {
+    typedef DN<AP_N> D;
+    const D a = D::var(F.a, AP_A), Q = D::var(F.Q, AP_Q), L = D::var(F.Lambda, AP_L);
+    const D th = D::var(F.th_obs * 180.0 / kPi, AP_TH) * kPi / 180.0;
+    const D r = D::var(F.r_obs, AP_R);
+    D thv = th;
+    thv.v = F.th_obs;                       // the frame's own rounding of the angle
+    ObsT<D> C;
+    observer_t<D>(F.M, a, Q, L, r, thv, C);
+    D st[7];
+    camera_t<D>(C, r, thv, al, be, st);
+    if (CHART == CHART_KS) bl_to_ks_t<D>(knds_t<D>(F.M, a, Q, L), F.ks_ref_init != 0, st);
+    const D* ys[4] = {&st[0], &st[1], &st[3], &st[4]};
+    for (int i = 0; i < 4; ++i) {
+        y0[i] = ys[i]->v;
+        for (int k = 0; k < AP_N; ++k) Jy[i][k] = ys[i]->d[k];
+    }
+    for (int k = 0; k < AP_N; ++k) { Jc[0][k] = st[5].d[k]; Jc[1][k] = st[6].d[k]; }
+    c[0] = st[5].v;
+    c[1] = st[6].v;
+}
+
+// ---------------------------------------------------------------------------
+//  replay of an accepted DOPRI5 step (the binary64 arithmetic of dp5_step, phi
+//  not carried): y, sc at the step start and the FSAL derivative k1 = f(y) ->
+//  the step end, its sin/cos and its derivative
+// ---------------------------------------------------------------------------
+template <int CHART>
+KT_HD void dp5_replay(const Knds<double>& P, double y[4], SinCos<double>& sc, double k1[4], double h, double pt,
+                      double pp) {
+    typedef DP5 C;
+    double k2[4], k3[4], k4[4], d[4], a5[4];
+    SinCos<double> scx;
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
+    stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k2, scx);
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
+    stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k3, scx);
+    for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
+    stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k4, scx);
+    {
+        double k5[4];
+        for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
+        stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k5, scx);
+        for (int i = 0; i < 4; ++i)
+            d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
+        for (int i = 0; i < 4; ++i) a5[i] = C::b1*k1[i] + C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i];
+    }
+    {
+        double k6[4];
+        stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k6, scx);
+        for (int i = 0; i < 4; ++i) a5[i] += C::b6*k6[i];
+    }
+    for (int i = 0; i < 4; ++i) d[i] = h*a5[i];
+    double k7[5];
+    SinCos<double> sc7;
+    stage_f64<CHART, 5>(P, y, d, sc, pt, pp, k7, sc7);
+    for (int i = 0; i < 4; ++i) { y[i] += d[i]; k1[i] = k7[i]; }
+    sc = sc7;
+}
+
+// Adjoint of one accepted step: lam (adjoint of the step end) -> adjoint of the step start;
+// accumulates the parameter adjoints gc (p_t, p_phi) and gP (a, Q, Lambda).
+template <int CHART>
+KT_HD void dp5_step_adjoint(double M, const double par[3], const Knds<double>& P, const double y[4],
+                            const SinCos<double>& sc, double h, double pt, double pp, double lam[4], double gc[2],
+                            double gP[3]) {
+    typedef DP5 C;
+    // stage points Y_1..Y_6 (forward arithmetic)
+    double Y[6][4], k[6][4], d[4];
+    SinCos<double> scx;
+    const double A[6][6] = {{0, 0, 0, 0, 0, 0},
+                            {C::a21, 0, 0, 0, 0, 0},
+                            {C::a31, C::a32, 0, 0, 0, 0},
+                            {C::a41, C::a42, C::a43, 0, 0, 0},
+                            {C::a51, C::a52, C::a53, C::a54, 0, 0},
+                            {C::a61, C::a62, C::a63, C::a64, C::a65, 0}};
+    const double B[6] = {C::b1, 0.0, C::b3, C::b4, C::b5, C::b6};
+    const double z[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < 4; ++i) Y[0][i] = y[i];
+    stage_f64<CHART, 4>(P, y, z, sc, pt, pp, k[0], scx);
+    for (int s = 1; s < 6; ++s) {
+        for (int i = 0; i < 4; ++i) {
+            double acc = 0.0;
+            for (int j = 0; j < s; ++j) acc += A[s][j]*k[j][i];
+            d[i] = h*acc;
+            Y[s][i] = y[i] + d[i];
+        }
+        stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k[s], scx);
+    }
+    // reverse over the stages
+    double nu[6][4];
+    double lam_out[4] = {lam[0], lam[1], lam[2], lam[3]};
+    for (int s = 5; s >= 0; --s) {
+        double mu[4];
+        for (int i = 0; i < 4; ++i) {
+            double acc = B[s]*lam[i];
+            for (int m = s + 1; m < 6; ++m) acc += A[m][s]*nu[m][i];
+            mu[i] = h*acc;
+        }
+        double J[4][9];
+        rhs_jacobian<CHART>(M, par, Y[s], pt, pp, J);
+        for (int k2 = 0; k2 < 4; ++k2) {
+            double acc = 0.0;
+            for (int i = 0; i < 4; ++i) acc += J[i][k2]*mu[i];
+            nu[s][k2] = acc;
+            lam_out[k2] += acc;
+        }
+        for (int i = 0; i < 4; ++i) {
+            gc[0] += J[i][4]*mu[i];
+            gc[1] += J[i][5]*mu[i];
+            gP[0] += J[i][6]*mu[i];
+            gP[1] += J[i][7]*mu[i];
+            gP[2] += J[i][8]*mu[i];
+        }
+    }
+    for (int i = 0; i < 4; ++i) lam[i] = lam_out[i];
+}
+
+// Hermite basis on [0,1] and its alpha-derivative (the interpolant of hermite())
+KT_HD void hermite_basis(double al, double hb[4], double db[4]) {
+    const double a2 = al*al, a3 = a2*al;
+    hb[0] = 2.0*a3 - 3.0*a2 + 1.0; hb[1] = a3 - 2.0*a2 + al; hb[2] = -2.0*a3 + 3.0*a2; hb[3] = a3 - a2;
+    db[0] = 6.0*a2 - 6.0*al; db[1] = 3.0*a2 - 4.0*al + 1.0; db[2] = -6.0*a2 + 6.0*al; db[3] = 3.0*a2 - 2.0*al;
+}
+
+// Seed of the disk event on the last step [y0 -> y1] of size h: the intensity and its
+// adjoints on y0, y1 (through the interpolation and the end-point derivatives),
+// on (p_t, p_phi) and on (a, Q, Lambda).  Returns false when the step has no disk root.
+template <int CHART>
+KT_HD bool event_seed(const FrameDev& F, const double y0[4], const double y1[4], double h, double pt, double pp,
+                      double lam0[4], double lam1[4], double gc[2], double gP[3], double* inten) {
+    const double par[3] = {F.a, F.Q, F.Lambda};
+    const Knds<double>& P = F.P;
+    // end-point derivatives (values) and Jacobians
+    double f0[5], f1[5];
+    SinCos<double> s0, s1;
+    kt_sincos(y0[1], &s0.s, &s0.c);
+    kt_sincos(y1[1], &s1.s, &s1.c);
+    rhs<CHART>(P, y0[0], s0, y0[2], y0[3], pt, pp, f0);
+    rhs<CHART>(P, y1[0], s1, y1[2], y1[3], pt, pp, f1);
+    const double halfpi = kPi / 2.0;
+    double al = 0.0;
+    if (!hermite_first_root(y0[1], y1[1], f0[1], f1[1], h, halfpi, al)) return false;
+    al = hermite_root_polish(y0[1], y1[1], f0[1], f1[1], h, halfpi, al);
+    double hb[4], db[4];
+    hermite_basis(al, hb, db);
+    const double rh = hb[0]*y0[0] + hb[1]*(h*f0[0]) + hb[2]*y1[0] + hb[3]*(h*f1[0]);
+    // redshift and intensity in dual numbers over (r_hit, p_t, p_phi, a, Q, Lambda, r_isco)
+    typedef DN<7> D;
+    const D r = D::var(rh, 0), ptd = D::var(pt, 1), ppd = D::var(pp, 2);
+    const D a = D::var(F.a, 3), Q = D::var(F.Q, 4), L = D::var(F.Lambda, 5);
+    const D g = disk_redshift_t<D>(F.M, a, Q, L, r, ptd, ppd);
+    const D I = disk_intensity_t<D>(r, g, D::var(F.r_isco, 6));
+    double dri[3];
+    isco_gradient(F, dri);
+    *inten = I.v;
+    gc[0] += I.d[1];
+    gc[1] += I.d[2];
+    gP[0] += I.d[3] + I.d[6]*dri[0];
+    gP[1] += I.d[4] + I.d[6]*dri[1];
+    gP[2] += I.d[5] + I.d[6]*dri[2];
+    const double lr = I.d[0];                       // dI / dr_hit
+    // r_hit(alpha; r0, r1, f0r, f1r), alpha from theta(alpha) = pi/2
+    const double th_al = db[0]*y0[1] + db[1]*(h*f0[1]) + db[2]*y1[1] + db[3]*(h*f1[1]);
+    const double r_al = db[0]*y0[0] + db[1]*(h*f0[0]) + db[2]*y1[0] + db[3]*(h*f1[0]);
+    const double la = (fabs(th_al) > 1e-300) ? -lr*r_al/th_al : 0.0;   // d alpha = -(dTheta)/Theta_alpha
+    // adjoints of the interpolation inputs
+    double e0[4] = {lr*hb[0], la*hb[0], 0.0, 0.0};          // direct on y0 (r, theta)
+    double e1[4] = {lr*hb[2], la*hb[2], 0.0, 0.0};          // direct on y1
+    const double df0[2] = {lr*hb[1]*h, la*hb[1]*h};          // on f0 (r, theta components)
+    const double df1[2] = {lr*hb[3]*h, la*hb[3]*h};
+    double J0[4][9], J1[4][9];
+    rhs_jacobian<CHART>(F.M, par, y0, pt, pp, J0);
+    rhs_jacobian<CHART>(F.M, par, y1, pt, pp, J1);
+    for (int k = 0; k < 4; ++k) {
+        lam0[k] = e0[k] + J0[0][k]*df0[0] + J0[1][k]*df0[1];
+        lam1[k] = e1[k] + J1[0][k]*df1[0] + J1[1][k]*df1[1];
+    }
+    for (int k = 0; k < 2; ++k)
+        gc[k] += J0[0][4 + k]*df0[0] + J0[1][4 + k]*df0[1] + J1[0][4 + k]*df1[0] + J1[1][4 + k]*df1[1];
+    for (int k = 0; k < 3; ++k)
+        gP[k] += J0[0][6 + k]*df0[0] + J0[1][6 + k]*df0[1] + J1[0][6 + k]*df1[0] + J1[1][6 + k]*df1[1];
+    return true;
+}
+
+// Reverse pass of one pixel from its records: N >= 1 accepted steps of sizes hs[0..N-1]
+// and checkpoints ck (6 doubles each, the state before steps 0, 16, 32, ...).  The ray
+// must have ended on the disk in step N-1.  grad = dI/dp; false if the replayed last
+// step has no disk root.
+template <int CHART>
+KT_HD bool adjoint_pixel(const FrameDev& F, int px, int py, int N, const double* hs, const double* ck,
+                         double grad[AP_N]) {
+    const double par[3] = {F.a, F.Q, F.Lambda};
+    const Knds<double>& P = F.P;
+    double al, be;
+    pixel_angles(F, (double)px, (double)py, al, be);
+    double y0[4], cc[2], Jy[4][AP_N], Jc[2][AP_N];
+    init_jacobian<CHART>(F, al, be, y0, cc, Jy, Jc);
+    const double pt = cc[0], pp = cc[1];
+    double lam[4] = {0.0, 0.0, 0.0, 0.0}, gc[2] = {0.0, 0.0}, gP[3] = {0.0, 0.0, 0.0};
+    double segy[kCkptEvery + 1][4];
+    SinCos<double> segsc[kCkptEvery + 1];
+    for (int k = (N - 1) / kCkptEvery; k >= 0; --k) {
+        const int n0 = k*kCkptEvery;
+        const int n1 = (n0 + kCkptEvery < N) ? n0 + kCkptEvery : N;
+        double y[4] = {ck[6*k + 0], ck[6*k + 1], ck[6*k + 2], ck[6*k + 3]};
+        SinCos<double> sc;
+        sc.s = ck[6*k + 4];
+        sc.c = ck[6*k + 5];
+        double k1[5];
+        rhs<CHART>(P, y[0], sc, y[2], y[3], pt, pp, k1);
+        for (int i = 0; i < 4; ++i) segy[0][i] = y[i];
+        segsc[0] = sc;
+        for (int n = n0; n < n1; ++n) {
+            dp5_replay<CHART>(P, y, sc, k1, hs[n], pt, pp);
+            for (int i = 0; i < 4; ++i) segy[n - n0 + 1][i] = y[i];
+            segsc[n - n0 + 1] = sc;
+        }
+        int n = n1 - 1;
+        if (n1 == N) {                      // last segment: seed at the disk event of step N-1
+            double lam0[4], lam1[4], inten = 0.0;
+            if (!event_seed<CHART>(F, segy[n - n0], segy[n - n0 + 1], hs[n], pt, pp, lam0, lam1, gc, gP, &inten))
+                return false;
+            for (int i = 0; i < 4; ++i) lam[i] = lam1[i];
+            dp5_step_adjoint<CHART>(F.M, par, P, segy[n - n0], segsc[n - n0], hs[n], pt, pp, lam, gc, gP);
+            for (int i = 0; i < 4; ++i) lam[i] += lam0[i];
+            --n;
+        }
+        for (; n >= n0; --n)
+            dp5_step_adjoint<CHART>(F.M, par, P, segy[n - n0], segsc[n - n0], hs[n], pt, pp, lam, gc, gP);
+    }
+    for (int q = 0; q < AP_N; ++q) {
+        double g = gc[0]*Jc[0][q] + gc[1]*Jc[1][q];
+        for (int i = 0; i < 4; ++i) g += lam[i]*Jy[i][q];
+        if (q < 3) g += gP[q];
+        grad[q] = g;
+    }
+    return true;
+}
+
+}  // namespace kt

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/kt_hip.h"
+#include "kt_adjoint.hpp"
 #include "kt_bundle.hpp"
 #include "kt_image.hpp"
 #include "kt_setup.hpp"
@@ -241,9 +242,21 @@ __global__ __launch_bounds__(kSetupThreads) void kt_setup_kernel(FrameDev* frame
 //  persistent trace kernel
 // ---------------------------------------------------------------------------
 
+// per-pixel records of the adjoint's forward passes
+struct AdjRecord {
+    int32_t* n_acc;          // accepted steps of the ray (written when it ends)
+    double* inten;           // disk intensity, 0 off the disk
+    const int32_t* n_cap;    // recording pass: step count of the counting pass
+    const int64_t* hoff;     // recording pass: first step-size slot of the pixel (nullptr: counting)
+    double* hs;              // accepted step sizes
+    const int64_t* coff;     // first checkpoint slot of the pixel
+    double* ckpt;            // r, theta, p_r, p_theta, sin theta, cos theta per checkpoint
+};
+
 struct TraceArgs {
     const FrameDev* frames;
     const LaunchDesc* desc;
+    AdjRecord rec;           // adjoint forward passes only (zero otherwise)
     uint32_t* geo;           // 7 dwords per pixel
     unsigned long long* words;
     RunOpts O;
@@ -273,29 +286,66 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 template <int CHART, int METHOD, int PREC, bool PHI>
 struct GeodesicPolicy {
     typedef RayState State;
-    __device__ static void init(const FrameDev& F, const RunOpts& O, int px, int py, State& S) {
-        ray_init<CHART, PREC>(F, O, px, py, S);
+    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, State& S) {
+        ray_init<CHART, PREC>(F, A.O, px, py, S);
     }
-    __device__ static bool advance(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf,
-                                   const RunOpts& O, State& S, RayOut& o, double& magnif) {
+    __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
+                                   const Knds<float>& Pf, State& S, RayOut& o, double& magnif) {
         bool use64 = true;
-        if (PREC == PREC_MIXED) use64 = WaveVote::any(wants_fp64(F, O, S));
+        if (PREC == PREC_MIXED) use64 = WaveVote::any(wants_fp64(F, A.O, S));
         magnif = 1.0;
-        return ray_advance<CHART, METHOD, PREC, PHI>(F, P, Pf, O, use64, S, o);
+        return ray_advance<CHART, METHOD, PREC, PHI>(F, P, Pf, A.O, use64, S, o);
     }
     __device__ static const RayState& geo(const State& S) { return S; }
 };
 
 struct BundlePolicy {
     typedef BundleState State;
-    __device__ static void init(const FrameDev& F, const RunOpts& O, int px, int py, State& B) {
-        bundle_init(F, O, px, py, B);
+    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, State& B) {
+        bundle_init(F, A.O, px, py, B);
     }
-    __device__ static bool advance(const FrameDev& F, const Knds<double>&, const Knds<float>&, const RunOpts& O,
+    __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>&, const Knds<float>&,
                                    State& B, RayOut& o, double& magnif) {
-        return bundle_step(F, O, B, o, magnif) == STEP_DONE;
+        return bundle_step(F, A.O, B, o, magnif) == STEP_DONE;
     }
     __device__ static const RayState& geo(const State& B) { return B.g; }
+};
+
+// Forward pass of the adjoint: the binary64 DOPRI5 geodesic, plus per-pixel records.
+// Counting pass (rec->hoff == nullptr): accepted-step count and disk intensity.
+// Recording pass: also every accepted step size and a checkpoint every kCkptEvery steps.
+template <int CHART>
+struct AdjointForwardPolicy {
+    struct State { RayState g; int64_t pix; };
+    __device__ static void checkpoint(const AdjRecord& R, int64_t slot, const RayState& g) {
+        double* c = R.ckpt + 6*slot;
+        c[0] = g.y[0]; c[1] = g.y[1]; c[2] = g.y[2]; c[3] = g.y[3]; c[4] = g.sc.s; c[5] = g.sc.c;
+    }
+    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, State& S) {
+        ray_init<CHART, PREC_FP64>(F, A.O, px, py, S.g);
+        S.pix = F.pix_base + (int64_t)py*F.W + px;
+        const AdjRecord& R = A.rec;
+        if (R.hoff) checkpoint(R, R.coff[S.pix], S.g);
+    }
+    __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
+                                   const Knds<float>& Pf, State& S, RayOut& o, double& magnif) {
+        const double h = S.g.h;
+        const int n0 = S.g.n_acc;
+        magnif = 1.0;
+        const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false>(F, P, Pf, A.O, true, S.g, o);
+        const AdjRecord& R = A.rec;
+        if (R.hoff && S.g.n_acc > n0 && n0 < R.n_cap[S.pix]) {
+            R.hs[R.hoff[S.pix] + n0] = h;
+            const int n1 = S.g.n_acc;
+            if (!done && n1 % kCkptEvery == 0 && n1 < R.n_cap[S.pix]) checkpoint(R, R.coff[S.pix] + n1 / kCkptEvery, S.g);
+        }
+        if (done) {
+            R.n_acc[S.pix] = S.g.n_acc;
+            R.inten[S.pix] = (o.fate == 1) ? disk_intensity_t<double>(o.r, o.g, F.r_isco) : 0.0;
+        }
+        return done;
+    }
+    __device__ static const RayState& geo(const State& S) { return S.g; }
 };
 
 template <class Policy, bool SINGLE>
@@ -354,7 +404,7 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
                 py = (int)(tile / F.tiles_x) * 8 + (w >> 3);
                 if (px < F.W && py < F.H) {
                     if (!SINGLE) { P = F.P; Pf = knds_cast<float>(P); }
-                    Policy::init(F, A.O, px, py, S);
+                    Policy::init(A, F, px, py, S);
                     active = true;
                 }
             }
@@ -365,7 +415,7 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
         }
         if (active) {
             const FrameDev& F = A.frames[SINGLE ? 0 : frame];
-            if (Policy::advance(F, P, Pf, A.O, S, o, magnif)) {
+            if (Policy::advance(A, F, P, Pf, S, o, magnif)) {
                 uint32_t wv[7];
                 pack_geopixel_m(o, magnif, wv);
                 uint32_t* dst = A.geo + 7 * (F.pix_base + (int64_t)py * F.W + px);
@@ -407,6 +457,47 @@ template <bool SINGLE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void kt_bundle_kernel(TraceArgs A) {
     trace_loop<BundlePolicy, SINGLE>(A);
+}
+
+// Adjoint forward passes (counting / recording), one frame, binary64 DOPRI5.
+template <int CHART>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void kt_adjoint_forward_kernel(TraceArgs A) {
+    trace_loop<AdjointForwardPolicy<CHART>, true>(A);
+}
+
+struct AdjBackArgs {
+    const FrameDev* frame;
+    int64_t n_pix;
+    const int32_t* n_acc;
+    const double* inten;
+    const int64_t* hoff;
+    const double* hs;
+    const int64_t* coff;
+    const double* ckpt;
+    double* grad;                  // AP_N per pixel: dI/dp
+    unsigned long long* words;     // [0] rays differentiated, [1] seeds without a disk root
+};
+
+// Reverse pass: one lane per pixel (adjoint_pixel: segments replayed from their
+// checkpoints, last first, into a local buffer, then walked backwards).
+template <int CHART>
+__global__ __launch_bounds__(256) void kt_adjoint_backward_kernel(AdjBackArgs B) {
+    const FrameDev& F = *B.frame;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < B.n_pix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        double* gout = B.grad + AP_N*p;
+        for (int q = 0; q < AP_N; ++q) gout[q] = 0.0;
+        const int N = B.n_acc[p];
+        if (!(B.inten[p] > 0.0) || N < 1) continue;
+        double g[AP_N];
+        if (!adjoint_pixel<CHART>(F, (int)(p % F.W), (int)(p / F.W), N, B.hs + B.hoff[p], B.ckpt + 6*B.coff[p], g)) {
+            atomicAdd(&B.words[1], 1ull);
+            continue;
+        }
+        for (int q = 0; q < AP_N; ++q) gout[q] = g[q];
+        atomicAdd(&B.words[0], 1ull);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -839,6 +930,7 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
     TraceArgs A;
     A.frames = B.frames;
     A.desc = B.desc;
+    std::memset(&A.rec, 0, sizeof(A.rec));
     A.geo = B.geo;
     A.words = B.words;
     A.O = run_opts(o);
@@ -1445,6 +1537,195 @@ void kt_hip_plan_destroy(kt_hip_plan* p) {
     (void)hipFree(p->d_words);
     for (int i = 0; i < 2; ++i) if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
     delete p;
+}
+
+}  // extern "C"
+
+// ============================================================================
+//  differentiable rendering: intensity and its adjoint
+// ============================================================================
+namespace {
+
+// device allocations released on scope exit
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+int dev_alloc(kt_hip_ctx* c, DevBuf& b, size_t bytes) {
+    if (hipMalloc(&b.p, bytes ? bytes : 8) != hipSuccess) return fail(c, KT_E_ALLOC, "allocation of %zu bytes failed", bytes);
+    return KT_OK;
+}
+
+typedef void (*AdjFwdFn)(TraceArgs);
+
+// One forward pass of the adjoint (counting when rec.hoff is null); times it with HIP events.
+int adjoint_forward(kt_hip_ctx* c, const kt_opts* o, int chart, int64_t rays, const AdjRecord& rec, float* ms) {
+    Launch L = launch_config(c, o, chart);
+    int64_t grid = (int64_t)c->n_cu * (L.bpc > 0 ? L.bpc : kDefaultBlocksPerCu);
+    const int64_t max_useful = (rays + kBlock - 1) / kBlock;
+    if (grid > max_useful) grid = max_useful;
+    if (grid < 1) grid = 1;
+    TraceArgs A;
+    A.frames = c->table.frames();
+    A.desc = c->table.desc();
+    A.rec = rec;
+    A.geo = c->d_geo;
+    A.words = c->d_words;
+    A.O = run_opts(o);
+    A.chunk = L.chunk > 0 ? L.chunk : kDefaultChunk;
+    AdjFwdFn fn = chart == KT_CHART_KS ? kt_adjoint_forward_kernel<CHART_KS> : kt_adjoint_forward_kernel<CHART_BL>;
+    hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
+    KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(kBlock), 0, c->stream, A);
+    KT_HIP(c, hipGetLastError());
+    KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    (void)hipEventElapsedTime(ms, c->ev[0], c->ev[1]);
+    return KT_OK;
+}
+
+// common front: options, one frame uploaded, setup done, per-pixel buffers
+int adjoint_prepare(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, kt_opts* o, int* chart, int64_t* rays,
+                    int64_t* pix) {
+    if (!c) return KT_E_ARG;
+    if (!f || !oin) return fail(c, KT_E_ARG, "frame and opts are required");
+    *o = *oin;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    if (o->integrator != KT_INTEGRATOR_DOPRI5 || o->bundles)
+        return fail(c, KT_E_UNSUPPORTED, "the adjoint differentiates the DOPRI5 single-ray path only");
+    o->precision = KT_PRECISION_FP64;
+    o->quirks |= Q_EXACT_ROOT;            // smooth disk hit point
+    KT_HIP(c, hipSetDevice(c->device));
+    rc = upload_frames(c, f, 1, o, rays, pix, chart);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kt_setup_kernel, dim3(1), dim3(kSetupThreads), 0, c->stream, c->table.frames(), c->table.desc());
+    KT_HIP(c, hipGetLastError());
+    return KT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kt_hip_render_intensity(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, double* I_out, kt_stats* st) {
+    kt_opts o;
+    int chart = 0;
+    int64_t rays = 0, pix = 0;
+    int rc = adjoint_prepare(c, f, oin, &o, &chart, &rays, &pix);
+    if (rc) return rc;
+    if (!I_out) return fail(c, KT_E_ARG, "intensity_out is NULL");
+    DevBuf n_acc, inten;
+    if ((rc = dev_alloc(c, n_acc, pix * 4)) || (rc = dev_alloc(c, inten, pix * 8))) return rc;
+    AdjRecord rec;
+    std::memset(&rec, 0, sizeof(rec));
+    rec.n_acc = n_acc.as<int32_t>();
+    rec.inten = inten.as<double>();
+    float ms = 0.f;
+    rc = adjoint_forward(c, &o, chart, rays, rec, &ms);
+    if (rc) return rc;
+    rc = read_stats(c, c->d_words, ms, ms, st);
+    if (rc) return rc;
+    KT_HIP(c, hipMemcpy(I_out, rec.inten, pix * 8, hipMemcpyDeviceToHost));
+    return KT_OK;
+}
+
+int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, const double* w, double* I_out,
+                          double* dI_out, double* grad_out, kt_adjoint_stats* ast) {
+    kt_opts o;
+    int chart = 0;
+    int64_t rays = 0, pix = 0;
+    int rc = adjoint_prepare(c, f, oin, &o, &chart, &rays, &pix);
+    if (rc) return rc;
+    if (!grad_out && !dI_out) return fail(c, KT_E_ARG, "grad_out and dI_dp_out are both NULL");
+    DevBuf n_acc, n_cap, inten, hoff, coff, grad, hs, ckpt;
+    if ((rc = dev_alloc(c, n_acc, pix * 4)) || (rc = dev_alloc(c, n_cap, pix * 4)) ||
+        (rc = dev_alloc(c, inten, pix * 8)) || (rc = dev_alloc(c, hoff, pix * 8)) ||
+        (rc = dev_alloc(c, coff, pix * 8)) || (rc = dev_alloc(c, grad, pix * 8 * AP_N)))
+        return rc;
+    AdjRecord rec;
+    std::memset(&rec, 0, sizeof(rec));
+    rec.n_acc = n_acc.as<int32_t>();
+    rec.inten = inten.as<double>();
+    kt_adjoint_stats S;
+    std::memset(&S, 0, sizeof(S));
+    float ms = 0.f;
+    // 1. counting pass: accepted steps per ray -> exact record layout
+    rc = adjoint_forward(c, &o, chart, rays, rec, &ms);
+    if (rc) return rc;
+    S.forward_ms = ms;
+    std::vector<int32_t> cnt((size_t)pix);
+    KT_HIP(c, hipMemcpy(cnt.data(), rec.n_acc, pix * 4, hipMemcpyDeviceToHost));
+    std::vector<int64_t> ho((size_t)pix), co((size_t)pix);
+    int64_t nh = 0, nc = 0;
+    for (int64_t i = 0; i < pix; ++i) {
+        ho[i] = nh;
+        co[i] = nc;
+        const int64_t n = cnt[i] > 0 ? cnt[i] : 0;
+        nh += n;
+        nc += (n + kCkptEvery - 1) / kCkptEvery > 0 ? (n + kCkptEvery - 1) / kCkptEvery : 1;
+    }
+    S.steps = nh;
+    S.checkpoints = nc;
+    S.record_bytes = nh * 8 + nc * 48;
+    if ((rc = dev_alloc(c, hs, (size_t)nh * 8)) || (rc = dev_alloc(c, ckpt, (size_t)nc * 48))) return rc;
+    KT_HIP(c, hipMemcpy(hoff.p, ho.data(), pix * 8, hipMemcpyHostToDevice));
+    KT_HIP(c, hipMemcpy(coff.p, co.data(), pix * 8, hipMemcpyHostToDevice));
+    KT_HIP(c, hipMemcpy(n_cap.p, rec.n_acc, pix * 4, hipMemcpyDeviceToDevice));
+    // 2. recording pass (the binary64 path is deterministic: the same steps as pass 1)
+    rec.n_cap = n_cap.as<int32_t>();
+    rec.hoff = hoff.as<int64_t>();
+    rec.hs = hs.as<double>();
+    rec.coff = coff.as<int64_t>();
+    rec.ckpt = ckpt.as<double>();
+    rc = adjoint_forward(c, &o, chart, rays, rec, &ms);
+    if (rc) return rc;
+    S.record_ms = ms;
+    // 3. reverse pass
+    AdjBackArgs B;
+    B.frame = c->table.frames();
+    B.n_pix = pix;
+    B.n_acc = rec.n_cap;
+    B.inten = rec.inten;
+    B.hoff = rec.hoff;
+    B.hs = rec.hs;
+    B.coff = rec.coff;
+    B.ckpt = rec.ckpt;
+    B.grad = grad.as<double>();
+    B.words = c->d_words;
+    hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
+    KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    const int64_t blocks = std::min<int64_t>((pix + 255) / 256, (int64_t)c->n_cu * 8);
+    if (chart == KT_CHART_KS)
+        hipLaunchKernelGGL(kt_adjoint_backward_kernel<CHART_KS>, dim3((unsigned)blocks), dim3(256), 0, c->stream, B);
+    else
+        hipLaunchKernelGGL(kt_adjoint_backward_kernel<CHART_BL>, dim3((unsigned)blocks), dim3(256), 0, c->stream, B);
+    KT_HIP(c, hipGetLastError());
+    KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    S.backward_ms = ms;
+    unsigned long long words[W_COUNT];
+    KT_HIP(c, hipMemcpy(words, c->d_words, sizeof(words), hipMemcpyDeviceToHost));
+    S.rays_differentiated = (int64_t)words[0];
+    S.seed_failures = (int64_t)words[1];
+    std::vector<double> g((size_t)pix * AP_N);
+    KT_HIP(c, hipMemcpy(g.data(), grad.p, pix * 8 * AP_N, hipMemcpyDeviceToHost));
+    if (I_out) KT_HIP(c, hipMemcpy(I_out, rec.inten, pix * 8, hipMemcpyDeviceToHost));
+    if (dI_out) std::memcpy(dI_out, g.data(), g.size() * 8);
+    if (grad_out) {
+        double acc[AP_N] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int64_t i = 0; i < pix; ++i) {
+            const double wi = w ? w[i] : 1.0;
+            if (wi == 0.0) continue;
+            for (int q = 0; q < AP_N; ++q) acc[q] += wi * g[(size_t)i*AP_N + q];
+        }
+        for (int q = 0; q < AP_N; ++q) grad_out[q] = acc[q];
+    }
+    if (ast) *ast = S;
+    return KT_OK;
 }
 
 }  // extern "C"

@@ -28,6 +28,8 @@ enum : uint32_t {
     Q_BUNDLE_HORIZON_ESC = 1u << 3,
     Q_BUNDLE_ROOT_SKIP = 1u << 4,
     Q_KS_LAMBDA_TO_BL = 1u << 5,
+    // internal mode of the differentiable path: converge the disk root (below)
+    Q_EXACT_ROOT = 1u << 30,
 };
 
 // Per-frame constants, produced by the setup kernel (device memory, 64-bit aligned).
@@ -230,6 +232,25 @@ KT_HD bool hermite_first_root(double y0, double y1, double d0, double d1, double
     return false;
 }
 
+// Newton steps on the Hermite cubic from the bisection result al, kept inside its final
+// bracket (half-width 2^-12): the root to rounding, so that the hit point is a smooth
+// function of the ray (the 8 bisections alone leave it piecewise constant).
+KT_HD double hermite_root_polish(double y0, double y1, double d0, double d1, double h, double tgt, double al) {
+    const double lo = al - 1.0/4096.0 > 0.0 ? al - 1.0/4096.0 : 0.0;
+    const double hi = al + 1.0/4096.0 < 1.0 ? al + 1.0/4096.0 : 1.0;
+    for (int it = 0; it < 4; ++it) {
+        const double a2 = al*al;
+        const double f = hermite(y0, y1, d0, d1, h, al) - tgt;
+        const double df = (6.0*a2 - 6.0*al)*y0 + (3.0*a2 - 4.0*al + 1.0)*(h*d0) + (-6.0*a2 + 6.0*al)*y1 +
+                          (3.0*a2 - 2.0*al)*(h*d1);
+        if (!(fabs(df) > 0.0)) break;
+        const double an = al - f/df;
+        if (!(an >= lo && an <= hi)) break;
+        al = an;
+    }
+    return al;
+}
+
 // Keplerian angular velocity, reference sign convention   knds_metric.hpp:354-363
 KT_HD double omega_k(const FrameDev& F, double r) {
     const double s = (F.a < 0.0) ? 1.0 : -1.0;
@@ -399,6 +420,7 @@ KT_HD bool step_events(const FrameDev& F, double r0, double th0, double ph0, dou
         hermites++;
         double al = 0.0;
         if (hermite_first_root(th0, th1, dth0, dth1, hu, halfpi, al)) {
+            if (quirks & Q_EXACT_ROOT) al = hermite_root_polish(th0, th1, dth0, dth1, hu, halfpi, al);
             const double rh = hermite(r0, r1, dr0, dr1, hu, al);
             if (rh >= F.r_in && rh <= F.r_out) {
                 d_r = rh;

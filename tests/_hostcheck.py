@@ -32,6 +32,13 @@ def lib():
         L.ktc_isco_exact.argtypes = [ctypes.c_double] * 4 + [P, P]
         L.ktc_trace_bundle_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
         L.ktc_sincos.argtypes = [P, ctypes.c_int64, P, P]
+        L.ktc_adjoint_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P, P, P]
+        L.ktc_camera_pair.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
+        D = ctypes.c_double
+        L.ktc_rhs_jacobian.argtypes = [ctypes.c_int, D, P, P, D, D, P]
+        L.ktc_rhs.argtypes = [ctypes.c_int, D, P, P, D, D, P]
+        L.ktc_replay_intensity.argtypes = [P, P, P, P, D, D, P, ctypes.c_int, P, P]
+        L.ktc_ray_steps.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P, P]
         _lib = L
     return _lib
 
@@ -86,3 +93,57 @@ def sincos(x):
     s, c = np.zeros_like(x), np.zeros_like(x)
     lib().ktc_sincos(_p(x), x.size, _p(s), _p(c))
     return s, c
+
+
+def adjoint_pixels(frame, opts, px, py, want_grad=True):
+    """Intensity, dI/d(a, Q, Lambda, theta_deg, r_obs), accepted steps and (r_h, r_isco, r_esc,
+    chart, ks_ref_init) of the host build of the adjoint."""
+    px = np.ascontiguousarray(px, np.int32)
+    py = np.ascontiguousarray(py, np.int32)
+    n = px.size
+    inten = np.zeros(n)
+    grad = np.zeros((n, 5)) if want_grad else None
+    nacc = np.zeros(n, np.int32)
+    info = np.zeros(5)
+    lib().ktc_adjoint_pixels(ctypes.byref(frame), ctypes.byref(opts), _p(px), _p(py), n, _p(inten), _p(grad),
+                             _p(nacc), _p(info))
+    return inten, grad, nacc, info
+
+
+def camera_pair(frame, opts, px, py):
+    px = np.ascontiguousarray(px, np.int32)
+    py = np.ascontiguousarray(py, np.int32)
+    a, b = np.zeros((px.size, 7)), np.zeros((px.size, 7))
+    lib().ktc_camera_pair(ctypes.byref(frame), ctypes.byref(opts), _p(px), _p(py), px.size, _p(a), _p(b))
+    return a, b
+
+
+def rhs_jacobian(chart, M, par, y, pt, pp):
+    J = np.zeros((4, 9))
+    lib().ktc_rhs_jacobian(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), pt, pp, _p(J))
+    return J
+
+
+def rhs(chart, M, par, y, pt, pp):
+    f = np.zeros(4)
+    lib().ktc_rhs(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), pt, pp, _p(f))
+    return f
+
+
+def ray_steps(frame, opts, px, py, cap=200000):
+    hs = np.zeros(cap)
+    n = ctypes.c_int()
+    fate = ctypes.c_int()
+    y0c = np.zeros(6)
+    lib().ktc_ray_steps(ctypes.byref(frame), ctypes.byref(opts), int(px), int(py), _p(hs), cap, ctypes.byref(n),
+                        _p(y0c), ctypes.byref(fate))
+    return hs[:n.value].copy(), y0c, fate.value
+
+
+def replay_intensity(frame, opts, par, y0, pt, pp, hs, adjoint=False):
+    inten = np.zeros(1)
+    lam = np.zeros(9) if adjoint else None
+    rc = lib().ktc_replay_intensity(ctypes.byref(frame), ctypes.byref(opts), _p(np.asarray(par, np.float64)),
+                                    _p(np.asarray(y0, np.float64)), pt, pp, _p(np.ascontiguousarray(hs)), hs.size,
+                                    _p(inten), _p(lam))
+    return (None if rc else inten[0]), lam

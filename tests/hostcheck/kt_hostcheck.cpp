@@ -1,10 +1,13 @@
 // Host build of the product's per-ray tracer (kerrtracecpp2_amd/csrc/kt_trace.hpp),
 // for CPU-only tests of the kernel logic against the oracle.  Test
 // infrastructure: not part of the product path, never loaded by it.
+#include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
 #include "../../include/kt_hip.h"
+#include "../../kerrtracecpp2_amd/csrc/kt_adjoint.hpp"
 #include "../../kerrtracecpp2_amd/csrc/kt_bundle.hpp"
 #include "../../kerrtracecpp2_amd/csrc/kt_setup.hpp"
 #include "../../kerrtracecpp2_amd/csrc/kt_shade.hpp"
@@ -144,5 +147,180 @@ extern "C" int ktc_trace_bundle_pixels(const kt_frame* f, const kt_opts* o, cons
 // the inline binary64 sin/cos of the kernels, on n arguments
 extern "C" int ktc_sincos(const double* x, int64_t n, double* s, double* c) {
     for (int64_t i = 0; i < n; ++i) kt_sincos(x[i], &s[i], &c[i]);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+//  adjoint on the host: forward with records, then the kernels' reverse pass
+// ---------------------------------------------------------------------------
+template <int CHART>
+static void adjoint_run(const FrameDev& F, const RunOpts& O, const int32_t* px, const int32_t* py, int64_t n,
+                        double* inten, double* grad, int32_t* nacc) {
+    const Knds<float> Pf = knds_cast<float>(F.P);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = 0; i < n; ++i) {
+        RayState S;
+        RayOut o;
+        std::vector<double> hs, ck;
+        ray_init<CHART, PREC_FP64>(F, O, px[i], py[i], S);
+        auto push_ck = [&]() { ck.insert(ck.end(), {S.y[0], S.y[1], S.y[2], S.y[3], S.sc.s, S.sc.c}); };
+        push_ck();
+        for (;;) {
+            const double h = S.h;
+            const int n0 = S.n_acc;
+            const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false>(F, F.P, Pf, O, true, S, o);
+            if (S.n_acc > n0) {
+                hs.push_back(h);
+                if (!done && S.n_acc % kCkptEvery == 0) push_ck();
+            }
+            if (done) break;
+        }
+        const double I = (o.fate == 1) ? disk_intensity_t<double>(o.r, o.g, F.r_isco) : 0.0;
+        if (inten) inten[i] = I;
+        if (nacc) nacc[i] = S.n_acc;
+        double g[AP_N] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (grad && I > 0.0 && S.n_acc >= 1)
+            if (!adjoint_pixel<CHART>(F, px[i], py[i], S.n_acc, hs.data(), ck.data(), g))
+                for (int q = 0; q < AP_N; ++q) g[q] = std::nan("");
+        if (grad) for (int q = 0; q < AP_N; ++q) grad[i*AP_N + q] = g[q];
+    }
+}
+
+// intensity (and dI/dp when grad != NULL) of n pixels, binary64 DOPRI5
+extern "C" int ktc_adjoint_pixels(const kt_frame* f, const kt_opts* o, const int32_t* px, const int32_t* py, int64_t n,
+                                  double* inten, double* grad, int32_t* nacc, double* info5) {
+    FrameDev F;
+    fill_frame(*f, *o, F);
+    RunOpts O;
+    O.max_steps = o->max_steps < 1 ? 1 : o->max_steps;
+    O.quirks = o->quirks;
+    O.step_init = o->step_init < 1e-10 ? 1e-10 : o->step_init;
+    O.rtol = o->rtol; O.atol = o->atol;
+    O.isl_noise = kMixedNoiseDefault;
+    O.isl_r = kMixedRIslandDefault;
+    if (info5) { info5[0] = F.r_h; info5[1] = F.r_isco; info5[2] = F.r_esc; info5[3] = F.chart; info5[4] = F.ks_ref_init; }
+    if (F.chart == CHART_KS) adjoint_run<CHART_KS>(F, O, px, py, n, inten, grad, nacc);
+    else adjoint_run<CHART_BL>(F, O, px, py, n, inten, grad, nacc);
+    return 0;
+}
+
+// templated camera map (T = double) against the kernels' camera_ray + covector: 7 values per pixel each
+extern "C" int ktc_camera_pair(const kt_frame* f, const kt_opts* o, const int32_t* px, const int32_t* py, int64_t n,
+                               double* st_kernel, double* st_template) {
+    FrameDev F;
+    fill_frame(*f, *o, F);
+    for (int64_t i = 0; i < n; ++i) {
+        double st[7], al, be;
+        camera_ray(F, (double)px[i], (double)py[i], st);
+        if (F.chart == CHART_KS) bl_to_ks_covector(F, st);
+        for (int k = 0; k < 7; ++k) st_kernel[7*i + k] = st[k];
+        pixel_angles(F, (double)px[i], (double)py[i], al, be);
+        ObsT<double> C;
+        observer_t<double>(F.M, F.a, F.Q, F.Lambda, F.r_obs, F.th_obs, C);
+        double s2[7];
+        camera_t<double>(C, F.r_obs, F.th_obs, al, be, s2);
+        s2[2] = st[2];
+        if (F.chart == CHART_KS) bl_to_ks_t<double>(knds_t<double>(F.M, F.a, F.Q, F.Lambda), F.ks_ref_init != 0, s2);
+        for (int k = 0; k < 7; ++k) st_template[7*i + k] = s2[k];
+    }
+    return 0;
+}
+
+// rhs Jacobian (dual numbers) at one point: J[4][9]
+extern "C" int ktc_rhs_jacobian(int chart, double M, const double* par3, const double* y4, double pt, double pp,
+                                double* J) {
+    double Jl[4][9];
+    if (chart == CHART_KS) rhs_jacobian<CHART_KS>(M, par3, y4, pt, pp, Jl);
+    else rhs_jacobian<CHART_BL>(M, par3, y4, pt, pp, Jl);
+    for (int i = 0; i < 4; ++i) for (int k = 0; k < 9; ++k) J[9*i + k] = Jl[i][k];
+    return 0;
+}
+
+// rhs value at one point with sin/cos from the argument (for finite differences)
+extern "C" int ktc_rhs(int chart, double M, const double* par3, const double* y4, double pt, double pp, double* f4) {
+    const Knds<double> P = knds_t<double>(M, par3[0], par3[1], par3[2]);
+    SinCos<double> sc;
+    kt_sincos(y4[1], &sc.s, &sc.c);
+    double o5[5];
+    if (chart == CHART_KS) rhs<CHART_KS>(P, y4[0], sc, y4[2], y4[3], pt, pp, o5);
+    else rhs<CHART_BL>(P, y4[0], sc, y4[2], y4[3], pt, pp, o5);
+    for (int i = 0; i < 4; ++i) f4[i] = o5[i];
+    return 0;
+}
+
+// Fixed-step pieces of the adjoint (no step-size control): replay N steps of sizes hs from
+// (y0, p_t, p_phi) with parameters par = (a, Q, Lambda) and return the intensity of the disk
+// event in the last step; with lam_out != NULL also the reverse pass: adjoints of y0 (4),
+// (p_t, p_phi) (2) and (a, Q, Lambda) (3).
+extern "C" int ktc_replay_intensity(const kt_frame* f, const kt_opts* o, const double* par, const double* y0,
+                                    double pt, double pp, const double* hs, int N, double* inten, double* lam_out) {
+    FrameDev F;
+    kt_frame f2 = *f;
+    f2.a = par[0]; f2.Q = par[1]; f2.Lambda = par[2];
+    fill_frame(f2, *o, F);
+    const int chart = F.chart;
+    auto go = [&](auto chart_tag) -> int {
+        constexpr int CH = decltype(chart_tag)::value;
+        std::vector<double> ys((size_t)(N + 1) * 4);
+        std::vector<SinCos<double>> scs((size_t)N + 1);
+        double y[4] = {y0[0], y0[1], y0[2], y0[3]};
+        SinCos<double> sc;
+        kt_sincos(y[1], &sc.s, &sc.c);
+        double k1[5];
+        rhs<CH>(F.P, y[0], sc, y[2], y[3], pt, pp, k1);
+        for (int i = 0; i < 4; ++i) ys[i] = y[i];
+        scs[0] = sc;
+        for (int n = 0; n < N; ++n) {
+            dp5_replay<CH>(F.P, y, sc, k1, hs[n], pt, pp);
+            for (int i = 0; i < 4; ++i) ys[(size_t)(n + 1)*4 + i] = y[i];
+            scs[n + 1] = sc;
+        }
+        double lam0[4], lam1[4], gc[2] = {0, 0}, gP[3] = {0, 0, 0};
+        if (!event_seed<CH>(F, &ys[(size_t)(N - 1)*4], &ys[(size_t)N*4], hs[N - 1], pt, pp, lam0, lam1, gc, gP, inten))
+            return 1;
+        if (!lam_out) return 0;
+        double lam[4] = {lam1[0], lam1[1], lam1[2], lam1[3]};
+        const double prm[3] = {F.a, F.Q, F.Lambda};
+        for (int n = N - 1; n >= 0; --n) {
+            dp5_step_adjoint<CH>(F.M, prm, F.P, &ys[(size_t)n*4], scs[n], hs[n], pt, pp, lam, gc, gP);
+            if (n == N - 1) for (int i = 0; i < 4; ++i) lam[i] += lam0[i];
+        }
+        for (int i = 0; i < 4; ++i) lam_out[i] = lam[i];
+        lam_out[4] = gc[0]; lam_out[5] = gc[1];
+        lam_out[6] = gP[0]; lam_out[7] = gP[1]; lam_out[8] = gP[2];
+        return 0;
+    };
+    if (chart == CHART_KS) return go(std::integral_constant<int, CHART_KS>());
+    return go(std::integral_constant<int, CHART_BL>());
+}
+
+// the accepted step sizes and the initial state of one pixel's binary64 DOPRI5 ray
+extern "C" int ktc_ray_steps(const kt_frame* f, const kt_opts* o, int px, int py, double* hs, int cap, int* n_out,
+                             double* y0c, int* fate) {
+    FrameDev F;
+    fill_frame(*f, *o, F);
+    RunOpts O;
+    O.max_steps = o->max_steps; O.quirks = o->quirks; O.step_init = o->step_init; O.rtol = o->rtol; O.atol = o->atol;
+    O.isl_noise = kMixedNoiseDefault; O.isl_r = kMixedRIslandDefault;
+    const Knds<float> Pf = knds_cast<float>(F.P);
+    RayState S;
+    RayOut out;
+    auto go = [&](auto chart_tag) {
+        constexpr int CH = decltype(chart_tag)::value;
+        ray_init<CH, PREC_FP64>(F, O, px, py, S);
+        y0c[0] = S.y[0]; y0c[1] = S.y[1]; y0c[2] = S.y[2]; y0c[3] = S.y[3]; y0c[4] = S.pt; y0c[5] = S.pp;
+        int n = 0;
+        for (;;) {
+            const double h = S.h;
+            const int n0 = S.n_acc;
+            const bool done = ray_advance<CH, METHOD_DP5, PREC_FP64, false>(F, F.P, Pf, O, true, S, out);
+            if (S.n_acc > n0 && n < cap) hs[n++] = h;
+            if (done) break;
+        }
+        *n_out = n;
+        *fate = out.fate;
+    };
+    if (F.chart == CHART_KS) go(std::integral_constant<int, CHART_KS>());
+    else go(std::integral_constant<int, CHART_BL>());
     return 0;
 }

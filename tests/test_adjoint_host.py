@@ -1,0 +1,117 @@
+"""The adjoint's arithmetic on the CPU (the kernels' per-ray code built for the host).
+
+  local Jacobian ........... dual-number J of the right-hand side vs central differences
+  templated camera map ..... bit-identical to the kernels' camera + covector (T = double)
+  fixed-step reverse pass .. vs central differences of the same fixed-step replay (all 9
+                             inputs: state, conserved p_t / p_phi, a, Q, Lambda)
+  full adaptive adjoint .... dI/dp vs central differences of the binary64 forward, h = 1e-6,
+                             at the differentiable-render configuration (KS, a=0.7, Q=0.2,
+                             theta=60) and at Lambda != 0, BL and Kerr (Bardeen ISCO) frames;
+                             bar: 1e-3 relative
+"""
+import numpy as np
+import pytest
+
+import kerrtracecpp2_amd as kt
+from tests import _hostcheck as hc
+
+EXACT_ROOT = 1 << 30          # internal mode of the differentiable path (converged disk root)
+PX = np.array([128, 100, 150, 60, 200, 128, 90, 170, 140, 110, 180])
+PY = np.array([128, 140, 110, 128, 120, 160, 100, 150, 170, 125, 135])
+
+
+def _opts(chart=1, rtol=1e-9, atol=1e-12):
+    o = kt.default_opts()
+    o.chart, o.integrator, o.rtol, o.atol = chart, kt.KT_INTEGRATOR_DOPRI5, rtol, atol
+    o.quirks |= EXACT_ROOT
+    return o
+
+
+@pytest.mark.parametrize("chart", [0, 1])
+def test_rhs_jacobian_vs_central_differences(chart):
+    rng = np.random.default_rng(chart)
+    for _ in range(5):
+        par = [rng.uniform(-0.9, 0.9), rng.uniform(0, 0.3), rng.uniform(0, 1e-5)]
+        y = [rng.uniform(3, 30), rng.uniform(0.3, 2.8), rng.uniform(-2, 2), rng.uniform(-5, 5)]
+        pt, pp = -1.0, rng.uniform(-4, 4)
+        J = hc.rhs_jacobian(chart, 1.0, par, y, pt, pp)
+        x0 = np.array(y + [pt, pp] + par)
+        fd = np.zeros((4, 9))
+        for k in range(9):
+            e = max(abs(x0[k]), 1e-3) * 1e-6
+            xp, xm = x0.copy(), x0.copy()
+            xp[k] += e
+            xm[k] -= e
+            fd[:, k] = (hc.rhs(chart, 1.0, xp[6:], xp[:4], xp[4], xp[5]) -
+                        hc.rhs(chart, 1.0, xm[6:], xm[:4], xm[4], xm[5])) / (2 * e)
+        assert np.abs(J - fd).max() <= 1e-6 * np.abs(J).max()
+
+
+@pytest.mark.parametrize("chart,Lam", [(0, 0.0), (1, 0.0), (1, 1e-5)])
+def test_templated_camera_is_the_kernel_camera(chart, Lam):
+    fr = kt.make_frame(64, 36, a=0.9, Q=0.3, Lambda=Lam, theta=70, fov=40)
+    px, py = np.arange(0, 64, 5), np.arange(0, 36, 3)
+    px, py = np.meshgrid(px, py)
+    a, b = hc.camera_pair(fr, _opts(chart), px.ravel(), py.ravel())
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("pix", [(60, 128), (128, 160), (200, 120)])
+def test_fixed_step_reverse_pass(pix):
+    o = _opts()
+    fr = kt.make_frame(256, 256, a=0.7, Q=0.2, theta=60)
+    hs, y0c, fate = hc.ray_steps(fr, o, *pix)
+    assert fate == 1
+    par = np.array([0.7, 0.2, 0.0])
+    I, lam = hc.replay_intensity(fr, o, par, y0c[:4], y0c[4], y0c[5], hs, adjoint=True)
+    x0 = np.concatenate([y0c, par])
+    fd = np.zeros(9)
+    for k in range(9):
+        e = max(abs(x0[k]), 1e-2) * 1e-7
+        xp, xm = x0.copy(), x0.copy()
+        xp[k] += e
+        xm[k] -= e
+        Ip, _ = hc.replay_intensity(fr, o, xp[6:], xp[:4], xp[4], xp[5], hs)
+        Im, _ = hc.replay_intensity(fr, o, xm[6:], xm[:4], xm[4], xm[5], hs)
+        fd[k] = (Ip - Im) / (2 * e)
+    np.testing.assert_allclose(lam, fd, rtol=1e-5, atol=1e-9 * abs(I))
+
+
+def _adjoint_vs_fd(base, keys, h, chart=1, px=PX, py=PY):
+    o = _opts(chart)
+    W = H = 256
+
+    def frame(**kw):
+        p = dict(base)
+        p.update(kw)
+        return kt.make_frame(W, H, **p)
+
+    I, G, N, _ = hc.adjoint_pixels(frame(), o, px, py)
+    m = I > 0
+    assert m.sum() >= 4
+    names = ["a", "Q", "Lambda", "theta", "r_obs"]
+    worst = 0.0
+    for k in keys:
+        q = names.index(k)
+        Ip, _, _, _ = hc.adjoint_pixels(frame(**{k: base[k] + h[k]}), o, px, py, False)
+        Im, _, _, _ = hc.adjoint_pixels(frame(**{k: base[k] - h[k]}), o, px, py, False)
+        fd = (Ip - Im) / (2 * h[k])
+        rel = np.abs(G[m, q] - fd[m]) / np.maximum(np.abs(fd[m]), 1e-6 * I[m])
+        worst = max(worst, float(rel.max()))
+    return worst
+
+
+def test_adjoint_vs_fd_kerr_newman_ks():
+    base = dict(a=0.7, Q=0.2, Lambda=0.0, theta=60.0, r_obs=40.0)
+    assert _adjoint_vs_fd(base, ["a", "Q", "theta", "r_obs"], dict(a=1e-6, Q=1e-6, theta=1e-6, r_obs=1e-6)) <= 1e-3
+
+
+def test_adjoint_vs_fd_lambda():
+    base = dict(a=0.7, Q=0.2, Lambda=5e-6, theta=60.0, r_obs=40.0)
+    assert _adjoint_vs_fd(base, ["Lambda", "a"], dict(Lambda=1e-8, a=1e-6)) <= 1e-3
+
+
+def test_adjoint_vs_fd_bl_and_kerr_isco():
+    # BL chart; Q = Lambda = 0: the ISCO is the Bardeen closed form, differentiated in a
+    base = dict(a=0.9, Q=0.0, Lambda=0.0, theta=75.0, r_obs=35.0)
+    assert _adjoint_vs_fd(base, ["a", "theta", "r_obs"], dict(a=1e-6, theta=1e-6, r_obs=1e-6), chart=0) <= 1e-3

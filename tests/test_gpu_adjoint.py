@@ -1,0 +1,144 @@
+"""GPU: the hand-written adjoint (checkpointed reverse pass) and its torch.autograd wrapper, at
+the differentiable-render configuration: 256x256, Kerr-Newman a=0.7, Q=0.2, Lambda=0, KS,
+theta_obs=60 deg, DOPRI5 (rtol 1e-9, atol 1e-12).
+
+  GPU adjoint == host build of the same reverse pass (per pixel)
+  per-pixel dI/dp vs central differences (h = 1e-6) of the GPU binary64 forward
+  loss gradient: L = sum over interior disk pixels of (I(p) - I_target)^2, target rendered from
+      seeded perturbed parameters (da, dQ ~ U(+-0.02), dtheta ~ U(+-2 deg)); adjoint vs central
+      differences, 1e-3 relative
+  torch: KerrIntensity.apply(p).backward() == the adjoint's gradient
+
+"Interior disk pixels": I > 0 at the pixel and its 8 neighbours in the base image, so the fixed
+mask holds no disk edge or photon-ring pixel (where I jumps and no derivative exists).
+"""
+import numpy as np
+import pytest
+
+import kerrtracecpp2_amd as kt
+from tests import _hostcheck as hc
+
+pytestmark = pytest.mark.gpu
+
+C5 = dict(a=0.7, Q=0.2, Lambda=0.0, theta=60.0, r_obs=40.0)
+W = H = 256
+NAMES = ["a", "Q", "Lambda", "theta", "r_obs"]
+
+
+def _opts():
+    o = kt.default_opts()
+    o.chart, o.integrator, o.rtol, o.atol = kt.KT_CHART_KS, kt.KT_INTEGRATOR_DOPRI5, 1e-9, 1e-12
+    return o
+
+
+def _frame(**kw):
+    p = dict(C5)
+    p.update(kw)
+    return kt.make_frame(W, H, fov=30.0, disk_out=12.0, **p)
+
+
+def _interior(I):
+    m = I > 0
+    out = m.copy()
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            out &= np.roll(np.roll(m, dy, 0), dx, 1)
+    out[0, :] = out[-1, :] = out[:, 0] = out[:, -1] = False
+    return out
+
+
+@pytest.fixture(scope="module")
+def base(gpu_ctx):
+    out = gpu_ctx.render_adjoint(_frame(), _opts(), want_pixel_grads=True)
+    print("C5 adjoint stats", out["stats"], flush=True)
+    return out
+
+
+def test_adjoint_records_and_counts(gpu_ctx, base):
+    st = base["stats"]
+    I = base["intensity"]
+    assert st["seed_failures"] == 0
+    assert st["rays_differentiated"] == int((I > 0).sum()) > 10000
+    assert st["checkpoints"] * 16 >= st["steps"] and st["record_bytes"] == st["steps"] * 8 + st["checkpoints"] * 48
+    I2, _ = gpu_ctx.render_intensity(_frame(), _opts())
+    assert np.array_equal(I2, I)                  # the intensity pass and the adjoint's forward agree
+
+
+def test_gpu_adjoint_equals_host_adjoint(base):
+    o = _opts()
+    o.quirks |= 1 << 30
+    rng = np.random.default_rng(11)
+    idx = rng.choice(np.flatnonzero(base["intensity"].ravel() > 0), size=60, replace=False)
+    px, py = (idx % W).astype(np.int32), (idx // W).astype(np.int32)
+    I, G, _, _ = hc.adjoint_pixels(_frame(), o, px, py)
+    gd = base["dI_dp"].reshape(-1, 5)[idx]
+    np.testing.assert_allclose(base["intensity"].ravel()[idx], I, rtol=1e-7)
+    scale = np.abs(G).max(axis=0) + 1e-30
+    assert np.all(np.abs(gd - G) <= 1e-5 * scale + 1e-5 * np.abs(G))
+
+
+def test_per_pixel_adjoint_vs_central_differences(gpu_ctx, base):
+    o = _opts()
+    G = base["dI_dp"]
+    mask = _interior(base["intensity"])
+    report = {}
+    for q, k in enumerate(NAMES):
+        if k == "Lambda":
+            continue     # Lambda = 0 switches the KS initial covector form (reference quirk): no two-sided FD
+        h = 1e-6
+        Ip, _ = gpu_ctx.render_intensity(_frame(**{k: C5[k] + h}), o)
+        Im, _ = gpu_ctx.render_intensity(_frame(**{k: C5[k] - h}), o)
+        fd = (Ip - Im) / (2 * h)
+        m = mask & (Ip > 0) & (Im > 0)
+        err = np.abs(G[..., q] - fd)[m]
+        rel = err / np.maximum(np.abs(fd[m]), 1e-4 * base["intensity"][m])
+        report[k] = (float(np.quantile(rel, 0.99)), float(np.mean(rel <= 1e-3)))
+    print("per-pixel rel err (q99, frac <= 1e-3)", report, flush=True)
+    for k, (q99, frac) in report.items():
+        assert frac >= 0.99, (k, frac)
+
+
+def _loss_setup(gpu_ctx):
+    rng = np.random.default_rng(2025)
+    d = rng.uniform(-1, 1, 3) * np.array([0.02, 0.02, 2.0])
+    target, _ = gpu_ctx.render_intensity(_frame(a=C5["a"] + d[0], Q=C5["Q"] + d[1], theta=C5["theta"] + d[2]),
+                                         _opts())
+    I0, _ = gpu_ctx.render_intensity(_frame(), _opts())
+    return target, _interior(I0)
+
+
+def test_c5_loss_gradient_vs_central_differences(gpu_ctx):
+    target, mask = _loss_setup(gpu_ctx)
+
+    def loss(**kw):
+        I, _ = gpu_ctx.render_intensity(_frame(**kw), _opts())
+        return float((((I - target) ** 2) * mask).sum()), I
+
+    L0, I0 = loss()
+    w = 2.0 * (I0 - target) * mask
+    g = gpu_ctx.render_adjoint(_frame(), _opts(), weights=w)["grad"]
+    rel = {}
+    for q, k in enumerate(NAMES):
+        if k == "Lambda":
+            continue
+        h = 1e-6
+        fd = (loss(**{k: C5[k] + h})[0] - loss(**{k: C5[k] - h})[0]) / (2 * h)
+        rel[k] = abs(g[q] - fd) / abs(fd)
+    print("C5 loss", L0, "grad", g, "rel err vs FD", rel, flush=True)
+    assert max(rel.values()) <= 1e-3
+
+
+def test_torch_autograd_function(gpu_ctx):
+    torch = pytest.importorskip("torch")
+    from kerrtracecpp2_amd.autograd import KerrIntensity
+    target, mask = _loss_setup(gpu_ctx)
+    spec = dict(width=W, height=H, fov=30.0, disk_out=12.0)
+    p = torch.tensor([C5[k] for k in NAMES], dtype=torch.float64, requires_grad=True)
+    img = KerrIntensity.apply(p, gpu_ctx, spec, _opts())
+    tt, mm = torch.from_numpy(target), torch.from_numpy(mask.astype(np.float64))
+    loss = (((img - tt) ** 2) * mm).sum()
+    loss.backward()
+    w = 2.0 * (img.detach().numpy() - target) * mask
+    want = gpu_ctx.render_adjoint(_frame(), _opts(), weights=w)["grad"]
+    np.testing.assert_array_equal(p.grad.numpy(), want)
+    assert np.all(np.isfinite(p.grad.numpy())) and abs(p.grad.numpy()[3]) > 0
