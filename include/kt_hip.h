@@ -40,6 +40,8 @@ extern "C" {
 #define KT_INTEGRATOR_DOPRI5 1 /* Dormand-Prince 5(4) with FSAL (geodesic.hpp:312-414)           */
 #define KT_PRECISION_FP64 0    /* every operation in binary64                                     */
 #define KT_PRECISION_MIXED 1   /* binary32 stage derivatives, binary64 state/error control/islands */
+#define KT_PALETTE_BLACKBODY 0     /* disk_colour (main.cpp:1799-1814)                              */
+#define KT_PALETTE_INTERSTELLAR 1  /* disk_colour_interstellar (main.cpp:1689-1797)                 */
 
 /* reference behaviours that a drop-in must reproduce (all set by default, KT_QUIRKS_REFERENCE) */
 #define KT_QUIRK_DOPRI5_PHI_FROZEN   (1u << 0) /* DOPRI5 never advances phi (geodesic.hpp:300-310)        */
@@ -93,6 +95,12 @@ typedef struct kt_opts {
                                0 selects the built-in default                               */
     double mixed_r_island;  /* MIXED: binary64 steps below r < mixed_r_island * r_horizon;
                                0 selects the built-in default                               */
+    int32_t palette;        /* KT_PALETTE_*: disk colouring (ColorParams::palette, main.cpp:70-81) */
+    int32_t disk_rings;     /* Interstellar palette: rings x sectors cells (main.cpp:76-77)   */
+    int32_t disk_sectors;
+    int32_t pad2;
+    double disk_sigma;      /* Interstellar Gaussian blend width in cells (main.cpp:78)       */
+    double disk_hue_offset; /* Interstellar hue offset (main.cpp:79)                          */
 } kt_opts;
 
 /* Background: RGB8 equirectangular image (BackgroundImage, main.cpp:140-171). */
@@ -160,9 +168,10 @@ int kt_hip_render(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const k
 int kt_hip_render_device(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const kt_opts* opts,
                          void** d_rgba, void** d_geo, kt_stats* stats);
 
-/* Phase 2 on the device: GeoPixel records -> ABGR (colorize_buffer, main.cpp:1817-1866). */
+/* Phase 2 on the device: GeoPixel records -> ABGR (colorize_buffer, main.cpp:1817-1866), with the
+ * same scalars colorize_buffer takes (M, r_isco, r_disk_in, r_disk_out) and the palette of opts. */
 int kt_hip_colorize(kt_hip_ctx* ctx, const kt_geopixel* geo, int64_t n, double M, double r_isco,
-                    const kt_opts* opts, uint32_t* rgba_out);
+                    double r_disk_in, double r_disk_out, const kt_opts* opts, uint32_t* rgba_out);
 
 /* Per-frame scalars only (setup kernel): horizon, ISCO, escape radius. */
 int kt_hip_frame_info(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const kt_opts* opts,

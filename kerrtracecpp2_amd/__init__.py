@@ -11,8 +11,8 @@ import numpy as np
 
 from . import _abi
 from ._abi import (GEOPIXEL_DTYPE, KT_CHART_BL, KT_CHART_KS, KT_INTEGRATOR_DOPRI5, KT_INTEGRATOR_RK4,  # noqa: F401
-                   KT_PRECISION_FP64, KT_PRECISION_MIXED, KT_QUIRKS_REFERENCE, KtFrame, KtOpts, default_opts,
-                   make_frame)
+                   KT_PALETTE_BLACKBODY, KT_PALETTE_INTERSTELLAR, KT_PRECISION_FP64, KT_PRECISION_MIXED,
+                   KT_QUIRKS_REFERENCE, KtFrame, KtOpts, default_opts, make_frame)
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libkt_hip.so")
@@ -164,11 +164,13 @@ class Context:
         """A graph-captured batched render plan (see Plan)."""
         return Plan(self, opts, max_frames, max_pixels)
 
-    def colorize(self, geo, M, r_isco, opts):
+    def colorize(self, geo, M, r_isco, opts, r_disk_in=None, r_disk_out=12.0):
+        """Phase 2 (colorize_buffer): r_disk_in defaults to r_isco as the reference writes it."""
         geo = np.ascontiguousarray(geo, dtype=GEOPIXEL_DTYPE)
         rgba = np.zeros(geo.size, np.uint32)
-        self._check(self.lib.kt_hip_colorize(self.h, _ptr(geo), geo.size, float(M), float(r_isco),
-                                             ctypes.byref(opts), _ptr(rgba)), "kt_hip_colorize")
+        r_in = float(r_isco if r_disk_in is None else r_disk_in)
+        self._check(self.lib.kt_hip_colorize(self.h, _ptr(geo), geo.size, float(M), float(r_isco), r_in,
+                                             float(r_disk_out), ctypes.byref(opts), _ptr(rgba)), "kt_hip_colorize")
         return rgba
 
 

@@ -8,6 +8,7 @@ KT_E_ARG, KT_E_DEVICE, KT_E_NOFP64, KT_E_ALLOC, KT_E_LAUNCH, KT_E_UNSUPPORTED = 
 KT_CHART_BL, KT_CHART_KS = 0, 1
 KT_INTEGRATOR_RK4, KT_INTEGRATOR_DOPRI5 = 0, 1
 KT_PRECISION_FP64, KT_PRECISION_MIXED = 0, 1
+KT_PALETTE_BLACKBODY, KT_PALETTE_INTERSTELLAR = 0, 1
 KT_QUIRK_DOPRI5_PHI_FROZEN = 1 << 0
 KT_QUIRK_ESCAPE_POSITIONAL = 1 << 1
 KT_QUIRK_KS_INIT_NO_SHIFT = 1 << 2
@@ -36,7 +37,9 @@ class KtOpts(ctypes.Structure):
                 ("exposure", ctypes.c_double), ("gamma", ctypes.c_double), ("temp_scale", ctypes.c_double),
                 ("tuning_blocks_per_cu", ctypes.c_int32), ("tuning_occupancy", ctypes.c_int32),
                 ("tuning_chunk", ctypes.c_int32), ("tuning_pad", ctypes.c_int32),
-                ("mixed_noise", ctypes.c_double), ("mixed_r_island", ctypes.c_double)]
+                ("mixed_noise", ctypes.c_double), ("mixed_r_island", ctypes.c_double),
+                ("palette", ctypes.c_int32), ("disk_rings", ctypes.c_int32), ("disk_sectors", ctypes.c_int32),
+                ("pad2", ctypes.c_int32), ("disk_sigma", ctypes.c_double), ("disk_hue_offset", ctypes.c_double)]
 
 
 class KtBackground(ctypes.Structure):
@@ -95,7 +98,7 @@ PROTOTYPES = {
     "kt_hip_set_background": (ctypes.c_int, [_P, _P]),
     "kt_hip_render": (ctypes.c_int, [_P, _P, _I, _P, _P, _P, _P, _P]),
     "kt_hip_render_device": (ctypes.c_int, [_P, _P, _I, _P, _P, _P, _P]),
-    "kt_hip_colorize": (ctypes.c_int, [_P, _P, ctypes.c_int64, _D, _D, _P, _P]),
+    "kt_hip_colorize": (ctypes.c_int, [_P, _P, ctypes.c_int64, _D, _D, _D, _D, _P, _P]),
     "kt_hip_frame_info": (ctypes.c_int, [_P, _P, _I, _P, _P]),
     "kt_make_checker_background": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                    ctypes.c_int32, _P]),
@@ -141,6 +144,8 @@ def default_opts(lib=None):
     o.exposure = 1.0
     o.gamma = 2.2
     o.temp_scale = 1.0
+    o.palette = KT_PALETTE_BLACKBODY
+    o.disk_rings, o.disk_sectors, o.disk_sigma, o.disk_hue_offset = 7, 14, 0.5, 0.0   # main.cpp:76-79
     return o
 
 

@@ -46,6 +46,9 @@ struct RenderOptions {
     unsigned quirks = KT_QUIRKS_REFERENCE;
     double pixel_offset_x = 0.0, pixel_offset_y = 0.0;
     double exposure = 1.0, gamma = 2.2, temp_scale = 1.0;
+    int palette = KT_PALETTE_BLACKBODY;   // ColorParams (main.cpp:70-81)
+    int disk_rings = 7, disk_sectors = 14;
+    double disk_sigma = 0.5, disk_hue_offset = 0.0;
     const uint8_t* bg_rgb = nullptr;  // equirect RGB8 background
     int bg_w = 0, bg_h = 0;
 };
@@ -113,6 +116,8 @@ inline kt_opts make_opts(int chart, const RenderOptions& ro, bool fp64) {
     o.rtol = ro.tol < 1e-10 ? 1e-10 : ro.tol;
     o.atol = ro.atol > 0.0 ? ro.atol : o.rtol;
     o.exposure = ro.exposure; o.gamma = ro.gamma; o.temp_scale = ro.temp_scale;
+    o.palette = ro.palette; o.disk_rings = ro.disk_rings; o.disk_sectors = ro.disk_sectors;
+    o.disk_sigma = ro.disk_sigma; o.disk_hue_offset = ro.disk_hue_offset;
     return o;
 }
 

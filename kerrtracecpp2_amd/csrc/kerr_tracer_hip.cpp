@@ -17,6 +17,8 @@
 //    * --quirks fixed turns off the reference-compatibility behaviours.
 //  Solver modes other than "standard", the wormhole metric, linear
 //  intersection and the Metal/debug switches are not provided by this backend.
+//  Both disk palettes are (--disk-blackbody, --disk-interstellar with
+//  --disk-rings/--disk-sectors/--disk-sigma/--disk-hue-offset).
 // ============================================================================
 #include <sys/stat.h>
 #include <unistd.h>
@@ -100,6 +102,8 @@ int main(int argc, char** argv) {
     double arg_a = 0.998, arg_disk_out = 12.0, arg_theta = 80.0, arg_phi = 0.0, arg_r_obs = -1.0;
     double arg_Q = 0.0, arg_Lam = 0.0, arg_fov = 30.0;
     double exposure = 1.0, gamma = 2.2, temp_scale = 1.0;
+    int palette = KT_PALETTE_BLACKBODY, disk_rings = 7, disk_sectors = 14;
+    double disk_sigma = 0.5, disk_hue = 0.0;
     bool geo_only = false, color_only = false;
     std::string geo_file;
     bool anim_mode = false, anim_ease = false, anim_keep_frames = false, anim_resume = false, anim_no_encode = false;
@@ -154,7 +158,12 @@ int main(int argc, char** argv) {
         else if (arg == "--exposure" && has1) exposure = num();
         else if (arg == "--gamma" && has1) gamma = num();
         else if (arg == "--temp-scale" && has1) temp_scale = num();
-        else if (arg == "--disk-blackbody") {}
+        else if (arg == "--disk-blackbody") palette = KT_PALETTE_BLACKBODY;
+        else if (arg == "--disk-interstellar") palette = KT_PALETTE_INTERSTELLAR;
+        else if (arg == "--disk-rings" && has1) disk_rings = inum();
+        else if (arg == "--disk-sectors" && has1) disk_sectors = inum();
+        else if (arg == "--disk-sigma" && has1) disk_sigma = num();
+        else if (arg == "--disk-hue-offset" && has1) disk_hue = num();
         else if (arg == "--geo-only") geo_only = true;
         else if (arg == "--geo-file" && has1) geo_file = argv[++i];
         else if (arg == "--color-only" && has1) { color_only = true; geo_file = argv[++i]; }
@@ -183,15 +192,14 @@ int main(int argc, char** argv) {
         else if (arg == "--disk-out-start" && has1) d0 = num();
         else if (arg == "--disk-out-end" && has1) d1 = num();
         else if (arg == "--semi-analytic" || arg == "--elliptic" || arg == "--elliptic-closed" ||
-                 arg == "--wormhole" || arg == "--disk-interstellar" || arg == "--intersection-linear" ||
+                 arg == "--wormhole" || arg == "--intersection-linear" ||
                  arg == "--linear-intersection" || arg == "--debug-elliptic") unsupported.push_back(arg);
         else if ((arg == "--solver-mode" || arg == "--intersection") && has1) {
             const std::string v = argv[++i];
             if (v != "standard" && v != "hermite" && v != "cubic" && v != "event-hermite")
                 unsupported.push_back(arg + " " + v);
         } else if ((arg == "--metal-kernel" || arg == "--wh-throat" || arg == "--wh-lensing" || arg == "--wh-tunnel" ||
-                    arg == "--bg-b" || arg == "--disk-rings" || arg == "--disk-sectors" || arg == "--disk-sigma" ||
-                    arg == "--disk-hue-offset") && has1) ++i;  // accepted and ignored
+                    arg == "--bg-b") && has1) ++i;  // accepted and ignored
         else if (arg == "--intersection-hermite" || arg == "--hermite-intersection" || arg == "--anti-fireflies" ||
                  arg == "--no-anti-fireflies" || arg == "--elliptic-fallback-black") {}
         else std::cerr << "Warning: unknown argument '" << arg << "'\n";
@@ -199,8 +207,8 @@ int main(int argc, char** argv) {
     if (!unsupported.empty()) {
         std::string s;
         for (auto& u : unsupported) s += (s.empty() ? "" : ", ") + u;
-        die("not provided by the HIP backend: " + s + " (it implements the standard solver, Kerr-Newman-de Sitter, "
-            "Hermite disk intersection and the blackbody palette)");
+        die("not provided by the HIP backend: " + s + " (it implements the standard solver, Kerr-Newman-de Sitter "
+            "and Hermite disk intersection)");
     }
     if (precision != "fp64" && precision != "mixed") die("--precision must be fp64 or mixed");
     max_steps = std::max(1, max_steps);
@@ -217,6 +225,8 @@ int main(int argc, char** argv) {
     ro.mixed_precision = (precision == "mixed");
     ro.quirks = quirks;
     ro.exposure = exposure; ro.gamma = gamma; ro.temp_scale = temp_scale;
+    ro.palette = palette; ro.disk_rings = disk_rings; ro.disk_sectors = disk_sectors;
+    ro.disk_sigma = disk_sigma; ro.disk_hue_offset = disk_hue;
 
     std::vector<uint8_t> bg_rgb;
     int bg_w = 0, bg_h = 0;
@@ -254,7 +264,8 @@ int main(int argc, char** argv) {
             ctx.check(kt_hip_set_background(ctx.get(), ro.bg_rgb ? &bg : nullptr), "kt_hip_set_background");
             const kt_opts o = kthip::make_opts(KT_CHART_BL, ro, true);
             std::vector<uint32_t> px(geo.size());
-            ctx.check(kt_hip_colorize(ctx.get(), geo.data(), (int64_t)geo.size(), meta.M_bh, meta.r_isco, &o, px.data()),
+            ctx.check(kt_hip_colorize(ctx.get(), geo.data(), (int64_t)geo.size(), meta.M_bh, meta.r_isco, meta.r_disk_in,
+                                      meta.r_disk_out, &o, px.data()),
                       "kt_hip_colorize");
             std::vector<uint8_t> rgb(px.size() * 3);
             kthip::abgr_to_rgb(px.data(), px.size(), rgb.data());

@@ -27,7 +27,7 @@
 namespace kt {
 using std::isfinite; using std::fma; using std::sqrt; using std::fabs; using std::fmin; using std::fmax;
 using std::floor; using std::frexp; using std::ldexp; using std::cbrt; using std::exp; using std::log;
-using std::log10; using std::pow; using std::fmod; using std::fmaf; using std::atan2; using std::rint;
+using std::log10; using std::pow; using std::fmod; using std::fmaf; using std::atan2; using std::rint; using std::ceil;
 KT_HD float sqrtf(float x) { return std::sqrt(x); }
 }
 #endif

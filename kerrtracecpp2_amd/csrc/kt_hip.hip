@@ -256,6 +256,7 @@ struct AdjRecord {
 struct TraceArgs {
     const FrameDev* frames;
     const LaunchDesc* desc;
+    const RayInit* init;     // per-pixel initial states (geodesic kernels)
     AdjRecord rec;           // adjoint forward passes only (zero otherwise)
     uint32_t* geo;           // 7 dwords per pixel
     unsigned long long* words;
@@ -287,7 +288,7 @@ template <int CHART, int METHOD, int PREC, bool PHI>
 struct GeodesicPolicy {
     typedef RayState State;
     __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, State& S) {
-        ray_init<CHART, PREC>(F, A.O, px, py, S);
+        ray_load_init(A.init[F.pix_base + (int64_t)py*F.W + px], A.O, S);
     }
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
                                    const Knds<float>& Pf, State& S, RayOut& o, double& magnif) {
@@ -459,6 +460,27 @@ void kt_bundle_kernel(TraceArgs A) {
     trace_loop<BundlePolicy, SINGLE>(A);
 }
 
+// Initial states of every pixel (camera ray, covector of the chart, sin/cos, first RHS).
+template <int CHART>
+__global__ __launch_bounds__(256) void kt_init_kernel(const FrameDev* frames, const LaunchDesc* desc, RayInit* out) {
+    const int n_frames = desc->n_frames;
+    const int64_t n_pix = desc->n_pix;
+    RunOpts O;
+    O.step_init = 1.0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n_pix; p += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = n_frames - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (frames[mid].pix_base <= p) lo = mid; else hi = mid - 1;
+        }
+        const FrameDev& F = frames[lo];
+        const int64_t loc = p - F.pix_base;
+        RayState S;
+        ray_init<CHART, PREC_FP64>(F, O, (int)(loc % F.W), (int)(loc / F.W), S);
+        ray_store_init(S, out[p]);
+    }
+}
+
 // Adjoint forward passes (counting / recording), one frame, binary64 DOPRI5.
 template <int CHART>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
@@ -514,6 +536,9 @@ struct ShadeArgs {
     int32_t bw, bh;
     double exposure, gamma, temp_scale;
     double M_override, risco_override;   // used when frames == nullptr (colour-only)
+    double rin_override, rout_override;
+    int32_t palette, rings, sectors;
+    double sigma, hue_offset;
 };
 
 __device__ __forceinline__ void shade_one(const ShadeArgs& A, int n_frames, int64_t i) {
@@ -526,15 +551,20 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& A, int n_frames, int6
         }
         p.M = A.frames[lo].M;
         p.r_isco = A.frames[lo].r_isco;
+        p.r_in = A.frames[lo].r_in;
+        p.r_out = A.frames[lo].r_out;
     } else {
         p.M = A.M_override;
         p.r_isco = A.risco_override;
+        p.r_in = A.rin_override;
+        p.r_out = A.rout_override;
     }
     p.exposure = A.exposure; p.gamma = A.gamma; p.temp_scale = A.temp_scale;
+    p.palette = A.palette; p.rings = A.rings; p.sectors = A.sectors; p.sigma = A.sigma; p.hue_offset = A.hue_offset;
     const uint32_t* g = A.geo + 7*i;
-    union { uint32_t u; float f; } r, red, mag, th, ph;
-    r.u = g[1]; red.u = g[2]; mag.u = g[3]; th.u = g[5]; ph.u = g[6];
-    A.rgba[i] = shade_pixel((uint8_t)(g[0] & 0xFF), r.f, red.f, mag.f, th.f, ph.f, p, A.bg, A.bw, A.bh);
+    union { uint32_t u; float f; } r, red, mag, pd, th, ph;
+    r.u = g[1]; red.u = g[2]; mag.u = g[3]; pd.u = g[4]; th.u = g[5]; ph.u = g[6];
+    A.rgba[i] = shade_pixel((uint8_t)(g[0] & 0xFF), r.f, red.f, mag.f, pd.f, th.f, ph.f, p, A.bg, A.bw, A.bh);
 }
 
 // grid-stride, so that a captured launch covers any pixel count within its capacity
@@ -647,6 +677,7 @@ struct kt_hip_ctx {
     FrameDev* d_frames = nullptr;                        // = table.frames()
     uint32_t* d_geo = nullptr; size_t cap_pix = 0;
     uint32_t* d_rgba = nullptr;
+    RayInit* d_init = nullptr; size_t cap_init = 0;     // per-pixel initial states
     uint8_t* d_bg = nullptr; int bg_w = 0, bg_h = 0; size_t cap_bg = 0;
     unsigned long long* d_words = nullptr;
     uint8_t* d_rgb = nullptr; size_t cap_rgb = 0;       // camera-spp resolved RGB8
@@ -692,6 +723,11 @@ void kt_hip_default_opts(kt_opts* o) {
     o->exposure = 1.0;              // main.cpp:73-75
     o->gamma = 2.2;
     o->temp_scale = 1.0;
+    o->palette = KT_PALETTE_BLACKBODY;
+    o->disk_rings = 7;              // main.cpp:76-79
+    o->disk_sectors = 14;
+    o->disk_sigma = 0.5;
+    o->disk_hue_offset = 0.0;
 }
 
 int kt_hip_create(int device, kt_hip_ctx** out) {
@@ -732,6 +768,7 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     if (c->table.host) (void)hipHostFree(c->table.host);
     (void)hipFree(c->d_geo);
     (void)hipFree(c->d_rgba);
+    (void)hipFree(c->d_init);
     (void)hipFree(c->d_bg);
     (void)hipFree(c->d_words);
     (void)hipFree(c->d_rgb);
@@ -859,6 +896,17 @@ static int reserve_pixels(kt_hip_ctx* c, uint32_t** geo, uint32_t** rgba, size_t
     return KT_OK;
 }
 
+static int reserve_init(kt_hip_ctx* c, RayInit** init, size_t* cap, int64_t pix) {
+    if ((size_t)pix <= *cap) return KT_OK;
+    (void)hipFree(*init);
+    *init = nullptr;
+    *cap = 0;
+    if (hipMalloc(init, (size_t)pix * sizeof(RayInit)) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "initial-state allocation (%lld pixels) failed", (long long)pix);
+    *cap = pix;
+    return KT_OK;
+}
+
 // Frame table upload + output capacity of the context's own buffers.
 static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o,
                          int64_t* total_rays, int64_t* total_pix, int* chart_out) {
@@ -870,6 +918,10 @@ static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, co
     if (rc) return rc;
     rc = reserve_pixels(c, &c->d_geo, &c->d_rgba, &c->cap_pix, *total_pix);
     if (rc) return rc;
+    if (!o->bundles) {
+        rc = reserve_init(c, &c->d_init, &c->cap_init, *total_pix);
+        if (rc) return rc;
+    }
     KT_HIP(c, hipMemcpyAsync(c->table.dev, c->table.host, c->table.bytes(n_frames), hipMemcpyHostToDevice, c->stream));
     return KT_OK;
 }
@@ -909,6 +961,7 @@ struct RenderBufs {
     uint32_t* geo;
     uint32_t* rgba;
     unsigned long long* words;
+    RayInit* init;           // per-pixel initial states (capacity >= pixels)
 };
 
 // Enqueue fill + setup + persistent trace (+ shade) on the context stream.  Grid
@@ -930,6 +983,7 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
     TraceArgs A;
     A.frames = B.frames;
     A.desc = B.desc;
+    A.init = B.init;
     std::memset(&A.rec, 0, sizeof(A.rec));
     A.geo = B.geo;
     A.words = B.words;
@@ -939,6 +993,15 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
     hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, B.words, (int)W_COUNT);
     if (ev) KT_HIP(c, hipEventRecord(ev[0], c->stream));
     hipLaunchKernelGGL(kt_setup_kernel, dim3(setup_grid), dim3(kSetupThreads), 0, c->stream, B.frames, B.desc);
+    if (!o->bundles) {
+        if (chart == KT_CHART_KS)
+            hipLaunchKernelGGL(kt_init_kernel<CHART_KS>, dim3((unsigned)(c->n_cu * 8)), dim3(256), 0, c->stream,
+                               B.frames, B.desc, B.init);
+        else
+            hipLaunchKernelGGL(kt_init_kernel<CHART_BL>, dim3((unsigned)(c->n_cu * 8)), dim3(256), 0, c->stream,
+                               B.frames, B.desc, B.init);
+        KT_HIP(c, hipGetLastError());
+    }
     if (ev) KT_HIP(c, hipEventRecord(ev[1], c->stream));
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(kBlock), 0, c->stream, A);
     KT_HIP(c, hipGetLastError());
@@ -949,7 +1012,9 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
         S.geo = B.geo; S.rgba = B.rgba;
         S.bg = c->bg_w > 0 ? c->d_bg : nullptr; S.bw = c->bg_w; S.bh = c->bg_h;
         S.exposure = o->exposure; S.gamma = o->gamma; S.temp_scale = o->temp_scale;
-        S.M_override = 1.0; S.risco_override = 6.0;
+        S.M_override = 1.0; S.risco_override = 6.0; S.rin_override = 6.0; S.rout_override = 12.0;
+        S.palette = o->palette; S.rings = o->disk_rings; S.sectors = o->disk_sectors;
+        S.sigma = o->disk_sigma; S.hue_offset = o->disk_hue_offset;
         hipLaunchKernelGGL(kt_shade_kernel, dim3((unsigned)(c->n_cu * 8)), dim3(256), 0, c->stream, S);
         KT_HIP(c, hipGetLastError());
     }
@@ -983,7 +1048,7 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
     const Launch L = force ? *force : launch_config(c, o, chart);
-    RenderBufs B{c->table.desc(), c->table.frames(), c->d_geo, c->d_rgba, c->d_words};
+    RenderBufs B{c->table.desc(), c->table.frames(), c->d_geo, c->d_rgba, c->d_words, c->d_init};
     rc = enqueue_render(c, B, o, chart, n_frames, rays, n_frames == 1, shade, L, c->ev);
     if (rc) return rc;
     KT_HIP(c, hipStreamSynchronize(c->stream));
@@ -1059,8 +1124,8 @@ int kt_hip_frame_info(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const
     return KT_OK;
 }
 
-int kt_hip_colorize(kt_hip_ctx* c, const kt_geopixel* geo, int64_t n, double M, double r_isco, const kt_opts* o,
-                    uint32_t* rgba_out) {
+int kt_hip_colorize(kt_hip_ctx* c, const kt_geopixel* geo, int64_t n, double M, double r_isco, double r_in,
+                    double r_out, const kt_opts* o, uint32_t* rgba_out) {
     if (!c || !geo || !rgba_out || n <= 0 || !o) return KT_E_ARG;
     KT_HIP(c, hipSetDevice(c->device));
     if ((size_t)n > c->cap_pix) {
@@ -1076,7 +1141,9 @@ int kt_hip_colorize(kt_hip_ctx* c, const kt_geopixel* geo, int64_t n, double M, 
     S.geo = c->d_geo; S.rgba = c->d_rgba;
     S.bg = c->bg_w > 0 ? c->d_bg : nullptr; S.bw = c->bg_w; S.bh = c->bg_h;
     S.exposure = o->exposure; S.gamma = o->gamma; S.temp_scale = o->temp_scale;
-    S.M_override = M; S.risco_override = r_isco;
+    S.M_override = M; S.risco_override = r_isco; S.rin_override = r_in; S.rout_override = r_out;
+    S.palette = o->palette; S.rings = o->disk_rings; S.sectors = o->disk_sectors;
+    S.sigma = o->disk_sigma; S.hue_offset = o->disk_hue_offset;
     hipLaunchKernelGGL(kt_shade_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S);
     KT_HIP(c, hipGetLastError());
     KT_HIP(c, hipMemcpyAsync(rgba_out, c->d_rgba, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1331,6 +1398,8 @@ struct kt_hip_plan {
     uint32_t* d_geo = nullptr;
     uint32_t* d_rgba = nullptr;
     size_t cap_pix = 0;
+    RayInit* d_init = nullptr;
+    size_t cap_init = 0;
     unsigned long long* d_words = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1351,7 +1420,7 @@ static void plan_drop_graph(kt_hip_plan* p) {
 
 static int plan_capture(kt_hip_plan* p) {
     kt_hip_ctx* c = p->c;
-    RenderBufs B{p->table.desc(), p->table.frames(), p->d_geo, p->d_rgba, p->d_words};
+    RenderBufs B{p->table.desc(), p->table.frames(), p->d_geo, p->d_rgba, p->d_words, p->d_init};
     KT_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = KT_OK;
     if (hipMemcpyAsync(p->table.dev, p->table.host, p->table.bytes(p->max_frames), hipMemcpyHostToDevice,
@@ -1484,6 +1553,7 @@ int kt_hip_plan_create(kt_hip_ctx* c, const kt_opts* o, int max_frames, int64_t 
     std::memset(&p->tune, 0, sizeof(p->tune));
     rc = table_reserve(c, p->table, (size_t)max_frames);
     if (!rc) rc = reserve_pixels(c, &p->d_geo, &p->d_rgba, &p->cap_pix, max_pixels);
+    if (!rc && !o->bundles) rc = reserve_init(c, &p->d_init, &p->cap_init, max_pixels);
     if (!rc && hipMalloc(&p->d_words, W_COUNT * sizeof(unsigned long long)) != hipSuccess)
         rc = fail(c, KT_E_ALLOC, "plan counter allocation failed");
     for (int i = 0; i < 2 && !rc; ++i)
@@ -1534,6 +1604,7 @@ void kt_hip_plan_destroy(kt_hip_plan* p) {
     if (p->table.host) (void)hipHostFree(p->table.host);
     (void)hipFree(p->d_geo);
     (void)hipFree(p->d_rgba);
+    (void)hipFree(p->d_init);
     (void)hipFree(p->d_words);
     for (int i = 0; i < 2; ++i) if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
     delete p;
@@ -1570,6 +1641,7 @@ int adjoint_forward(kt_hip_ctx* c, const kt_opts* o, int chart, int64_t rays, co
     TraceArgs A;
     A.frames = c->table.frames();
     A.desc = c->table.desc();
+    A.init = nullptr;
     A.rec = rec;
     A.geo = c->d_geo;
     A.words = c->d_words;

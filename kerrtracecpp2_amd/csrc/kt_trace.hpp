@@ -680,6 +680,33 @@ KT_HD void ray_init(const FrameDev& F, const RunOpts& O, int px, int py, RayStat
     S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0;
 }
 
+// Initial state of a ray as the init kernel leaves it in HBM: one 128-byte record per
+// pixel, so that the persistent loop refills a lane with loads instead of running the
+// camera (tetrad, two square-root solves, the Kerr-Schild covector, sin/cos, first RHS)
+// for the whole wave every time one lane needs a new ray.
+struct RayInit {
+    double y[4], ph, pt, pp, k1[5], s, c, pad[2];
+};
+static_assert(sizeof(RayInit) == 128, "init record");
+
+KT_HD void ray_store_init(const RayState& S, RayInit& R) {
+    for (int i = 0; i < 4; ++i) R.y[i] = S.y[i];
+    R.ph = S.ph; R.pt = S.pt; R.pp = S.pp;
+    for (int i = 0; i < 5; ++i) R.k1[i] = S.k1[i];
+    R.s = S.sc.s; R.c = S.sc.c;
+    R.pad[0] = R.pad[1] = 0.0;
+}
+
+KT_HD void ray_load_init(const RayInit& R, const RunOpts& O, RayState& S) {
+    for (int i = 0; i < 4; ++i) S.y[i] = R.y[i];
+    S.ph = R.ph; S.pt = R.pt; S.pp = R.pp;
+    for (int i = 0; i < 5; ++i) S.k1[i] = R.k1[i];
+    S.sc.s = R.s; S.sc.c = R.c;
+    S.h = O.step_init < 1e-10 ? 1e-10 : O.step_init;
+    S.n_acc = 0; S.n_rej = 0;
+    S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0;
+}
+
 // Does this lane need a binary64 step in the mixed-precision path?  The FP32
 // stage noise of the embedded error estimate, ~isl_noise * h |k|, must stay
 // well below the error scale atol + rtol |y|; near the horizon and the polar

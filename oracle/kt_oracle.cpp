@@ -21,6 +21,7 @@
 //    BL tracer ......... main.cpp:329-434
 //    KS tracer ......... main.cpp:1303-1660
 //    shading ........... main.cpp:140-171, 1663-1687, 1799-1866
+//    Interstellar ...... main.cpp:1689-1797
 //    frame driver ...... main.cpp:2014-2206 (GeoPixel fill 2098-2147)
 //    camera spp ........ main.cpp:2232-2322
 //
@@ -1154,6 +1155,74 @@ static RGB8 disk_rgb(double r, double red, double mag, double M, double rI, cons
             (uint8_t)(reinhard(c.b/255.0*I, cp.exposure, cp.gamma)*255)};
 }
 
+// Interstellar palette (main.cpp:1689-1797).  HSV -> RGB8 with round-half-up bytes (:1690-1708).
+static RGB8 hsv8(double hue, double sat, double v) {
+    hue -= std::floor(hue);
+    const double x = hue * 6.0;
+    const int sector = (int)x;
+    const double fr = x - sector;
+    const double lo = v * (1.0 - sat), fall = v * (1.0 - sat * fr), rise = v * (1.0 - sat * (1.0 - fr));
+    double c[3];
+    switch (sector % 6) {
+        case 0:  c[0] = v;    c[1] = rise; c[2] = lo;   break;
+        case 1:  c[0] = fall; c[1] = v;    c[2] = lo;   break;
+        case 2:  c[0] = lo;   c[1] = v;    c[2] = rise; break;
+        case 3:  c[0] = lo;   c[1] = fall; c[2] = v;    break;
+        case 4:  c[0] = rise; c[1] = lo;   c[2] = v;    break;
+        default: c[0] = v;    c[1] = lo;   c[2] = fall; break;
+    }
+    return {(uint8_t)(c[0]*255 + 0.5), (uint8_t)(c[1]*255 + 0.5), (uint8_t)(c[2]*255 + 0.5)};
+}
+// fract(43758.5453 * sin(id * a + b)) (:1711-1715)
+static double cell_fract(int id, double a, double b) {
+    const double v = std::sin((double)id * a + b) * 43758.5453;
+    return v - std::floor(v);
+}
+struct Cells { int rings, sectors; double sigma, hue_offset, r_in, r_out; };
+
+// (:1722-1797) Gaussian-weighted mean colour of the ring x sector cells around (r, phi):
+// weight exp(-d^2 / 2 sigma^2) in cell units, sector distance wrapped, rings clamped at
+// the edges, cells with a third hash below 0.3 left out; then the blackbody palette's
+// emissivity / redshift / magnification factor and tone map.
+static RGB8 disk_rgb_cells(double r, double phi, double red, double mag, double rI, const Cells& k,
+                           const Colour& cp) {
+    const int nr = k.rings > 0 ? k.rings : 1, ns = k.sectors > 0 ? k.sectors : 1;
+    const double sg = k.sigma > 0 ? k.sigma : 0.5;
+    const double two_pi = 2.0*PI;
+    const double u = clampv((r - k.r_in) / (k.r_out - k.r_in), 0.0, 1.0) * nr;
+    const double v = (phi - std::floor(phi / two_pi) * two_pi) / two_pi * ns;
+    const double q = 1.0 / (2.0 * sg * sg);
+    const int reach = (int)std::ceil(3.0 * sg) + 1;
+    double sum[3] = {0.0, 0.0, 0.0}, wsum = 0.0;
+    for (int i = -reach; i <= nr + reach; ++i) {
+        const double du = u - (double)i;
+        if (du * du * q > 16.0) continue;
+        const int ring = std::min(std::max(i, 0), nr - 1);
+        for (int j = -reach; j <= ns + reach; ++j) {
+            double dv = v - (double)j;
+            dv = dv - std::floor((dv + ns * 0.5) / ns) * ns;
+            const double w = std::exp(-(du * du + dv * dv) * q);
+            if (w < 1e-4) continue;
+            const int id = ring * ns + ((j % ns) + ns) % ns;
+            const double c1 = cell_fract(id, 127.1, 311.7), c2 = cell_fract(id, 269.5, 183.3);
+            if (cell_fract(id, 419.2, 371.9) < 0.30) continue;
+            const RGB8 c = hsv8(std::fmod(c1 * 0.14 + k.hue_offset, 1.0), 0.70 + c2 * 0.30, 0.50 + c1 * 0.50);
+            sum[0] += w * c.r; sum[1] += w * c.g; sum[2] += w * c.b;
+            wsum += w;
+        }
+    }
+    if (wsum < 1e-6) return {0, 0, 0};
+    const double pk = pt_emiss(3.0*rI, rI);
+    double I = (pk > 0.0) ? pt_emiss(r, rI)/pk : 0.0;
+    const double rc = clampv(red, 0.1, 10.0);
+    I *= std::pow(rc, 4.0) * (1.0 + 0.85 * clampv(1.0 - rc, 0.0, 1.0));
+    I *= clampv(1.0/mag, 0.05, 5.0);
+    const double nrm = wsum * 255.0;
+    return {(uint8_t)(reinhard(sum[0] / nrm * I, cp.exposure, cp.gamma)*255),
+            (uint8_t)(reinhard(sum[1] / nrm * I, cp.exposure, cp.gamma)*255),
+            (uint8_t)(reinhard(sum[2] / nrm * I, cp.exposure, cp.gamma)*255)};
+}
+
 static RGB8 sky(const uint8_t* px, int w, int h, double th, double ph) {   // :150-170
     if (!px || w <= 0 || h <= 0) return {0,0,0};
     double pw = std::fmod(ph, 2.0*PI);
@@ -1305,6 +1374,25 @@ int kto_colorize(const kto_geopixel* geo, int64_t n, double M, double r_isco, do
         RGB8 c{0, 0, 0};
         const kto_geopixel& p = geo[i];
         if (p.outcome == 1) c = disk_rgb(p.r, p.redshift, p.magnif, M, r_isco, cp);
+        else if ((p.outcome == 0 || p.outcome == 3) && bg) c = sky(bg, bg_w, bg_h, p.theta_esc, p.phi_esc);
+        rgb_out[3*i+0] = c.r; rgb_out[3*i+1] = c.g; rgb_out[3*i+2] = c.b;
+    }
+    return 0;
+}
+
+// Phase 2 with the Interstellar palette (colorize_buffer, main.cpp:1839-1845): disk pixels
+// take the cell colours over [r_disk_in, r_disk_out]; sky and black pixels as above.
+int kto_colorize_interstellar(const kto_geopixel* geo, int64_t n, double M, double r_isco, double r_disk_in,
+                              double r_disk_out, int rings, int sectors, double sigma, double hue_offset,
+                              double exposure, double gamma, const uint8_t* bg, int bg_w, int bg_h, uint8_t* rgb_out) {
+    if (!geo || !rgb_out || n < 0) return -1;
+    (void)M;
+    const Colour cp{exposure, gamma, 1.0};
+    const Cells k{rings, sectors, sigma, hue_offset, r_disk_in, r_disk_out};
+    for (int64_t i = 0; i < n; ++i) {
+        RGB8 c{0, 0, 0};
+        const kto_geopixel& p = geo[i];
+        if (p.outcome == 1) c = disk_rgb_cells(p.r, p.phi_disk, p.redshift, p.magnif, r_isco, k, cp);
         else if ((p.outcome == 0 || p.outcome == 3) && bg) c = sky(bg, bg_w, bg_h, p.theta_esc, p.phi_esc);
         rgb_out[3*i+0] = c.r; rgb_out[3*i+1] = c.g; rgb_out[3*i+2] = c.b;
     }

@@ -75,6 +75,7 @@ def lib():
         L.kto_hermite.argtypes = [D, D, D, D, D, D]
         L.kto_hermite.restype = D
         L.kto_blackbody.argtypes = [D, P]
+        L.kto_colorize_interstellar.argtypes = [P, ctypes.c_int64, D, D, D, D, I, I, D, D, D, D, P, I, I, P]
         _lib = L
     return _lib
 
@@ -135,6 +136,20 @@ def colorize(geo, M, r_isco, bg=None, exposure=1.0, gamma=2.2, temp_scale=1.0):
     else:
         bh = bw = 0
     lib().kto_colorize(_p(geo), geo.size, M, r_isco, exposure, gamma, temp_scale, _p(bg), bw, bh, _p(rgb))
+    return rgb
+
+
+def colorize_interstellar(geo, M, r_isco, r_disk_in, r_disk_out, rings=7, sectors=14, sigma=0.5, hue_offset=0.0,
+                          bg=None, exposure=1.0, gamma=2.2):
+    geo = np.ascontiguousarray(geo)
+    rgb = np.zeros((geo.size, 3), np.uint8)
+    if bg is not None:
+        bg = np.ascontiguousarray(bg, dtype=np.uint8)
+        bh, bw = bg.shape[:2]
+    else:
+        bh = bw = 0
+    lib().kto_colorize_interstellar(_p(geo), geo.size, M, r_isco, r_disk_in, r_disk_out, rings, sectors, sigma,
+                                    hue_offset, exposure, gamma, _p(bg), bw, bh, _p(rgb))
     return rgb
 
 

@@ -30,6 +30,8 @@ def lib():
         L.ktc_shade.argtypes = [P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_double, P, ctypes.c_int, ctypes.c_int, P]
         L.ktc_isco_exact.argtypes = [ctypes.c_double] * 4 + [P, P]
+        L.ktc_shade_interstellar.argtypes = [P, ctypes.c_int64] + [ctypes.c_double] * 4 + [ctypes.c_int] * 2 + \
+            [ctypes.c_double] * 4 + [P, ctypes.c_int, ctypes.c_int, P]
         L.ktc_trace_bundle_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
         L.ktc_sincos.argtypes = [P, ctypes.c_int64, P, P]
         L.ktc_adjoint_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P, P, P]
@@ -147,3 +149,14 @@ def replay_intensity(frame, opts, par, y0, pt, pp, hs, adjoint=False):
                                     _p(np.asarray(y0, np.float64)), pt, pp, _p(np.ascontiguousarray(hs)), hs.size,
                                     _p(inten), _p(lam))
     return (None if rc else inten[0]), lam
+
+
+def shade_interstellar(geo, M, r_isco, r_in, r_out, rings=7, sectors=14, sigma=0.5, hue=0.0, bg=None, exposure=1.0,
+                       gamma=2.2):
+    geo = np.ascontiguousarray(geo)
+    out = np.zeros(geo.size, np.uint32)
+    bgp = None if bg is None else np.ascontiguousarray(bg, np.uint8)
+    lib().ktc_shade_interstellar(_p(geo), geo.size, M, r_isco, r_in, r_out, rings, sectors, sigma, hue, exposure,
+                                 gamma, _p(bgp), 0 if bg is None else bg.shape[1], 0 if bg is None else bg.shape[0],
+                                 _p(out))
+    return out

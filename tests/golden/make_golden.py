@@ -2,6 +2,7 @@
 
 Run in the build container only (needs /root/reference):
     make -C oracle ref && python tests/golden/make_golden.py
+    python tests/golden/make_golden.py --images NAME...     (only those image fixtures)
 
 The reference CPU binary (oracle/_ref/kerr_tracer_ref, IEEE flags) produces
 .kgeo files and PNGs for small configurations, and oracle/_ref/ref_harness
@@ -100,6 +101,12 @@ IMAGES = {
     "c2_rgb_160x90": "--ks --custom-res 160 90 --a 0.9 --charge 0.3 --lambda 1e-5 --r-obs 40 --theta 85 "
                      "--disk-out 12 --dopri5 --tol 1e-9",
     "spp3_rgb_80x45": "--bl --custom-res 80 45 --a 0.5 --r-obs 40 --theta 80 --fov 45 --disk-out 12 --camera-spp 3",
+    # Interstellar palette: the c1 frame with default cells, and a KS frame with every cell option set
+    "c1_interstellar_rgb_320x180": "--bl --custom-res 320 180 --a 0.5 --r-obs 40 --theta 80 --phi 0 --fov 45 "
+                                   "--disk-out 12 --disk-interstellar",
+    "ks_interstellar_rgb_160x90": "--ks --custom-res 160 90 --a 0.5 --theta 80 --phi 0 --r-obs 40 --fov 45 "
+                                  "--disk-out 12 --disk-interstellar --disk-rings 5 --disk-sectors 9 "
+                                  "--disk-sigma 0.8 --disk-hue-offset 0.3",
 }
 
 
@@ -167,9 +174,28 @@ def function_vectors():
     return [dict(kind=c[0], inputs=c[2], output=o) for c, o in zip(cases, outs)]
 
 
+def images_only(names):
+    """Re-make only the named image fixtures (the rest of the manifest is kept)."""
+    bg_path = os.path.join(OUTD, "checker_bg.png")
+    os.makedirs(OUTD, exist_ok=True)
+    Image.fromarray(checker()).save(bg_path)
+    with open(os.path.join(HERE, "manifest.json")) as f:
+        manifest = json.load(f)
+    for name in names:
+        args = IMAGES[name]
+        img = run_ref(args.split() + ["--bg", bg_path], geo=False)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), rgb=img)
+        manifest["images"][name] = dict(args=args)
+        print(name, img.shape, flush=True)
+    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1)
+
+
 def main():
     if not (os.path.exists(REF) and os.path.exists(HARN)):
         sys.exit("build the reference first: make -C oracle ref")
+    if len(sys.argv) > 2 and sys.argv[1] == "--images":
+        return images_only(sys.argv[2:])
     bg = checker()
     bg_path = os.path.join(OUTD, "checker_bg.png")
     os.makedirs(OUTD, exist_ok=True)

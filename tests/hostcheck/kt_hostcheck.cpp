@@ -92,10 +92,27 @@ extern "C" int ktc_trace_pixels(const kt_frame* f, const kt_opts* o, const int32
 
 extern "C" int ktc_shade(const kt_geopixel* geo, int64_t n, double M, double r_isco, double exposure, double gamma,
                          double temp_scale, const uint8_t* bg, int bw, int bh, uint32_t* rgba) {
-    ShadeParams p{M, r_isco, exposure, gamma, temp_scale};
+    ShadeParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.M = M; p.r_isco = r_isco; p.exposure = exposure; p.gamma = gamma; p.temp_scale = temp_scale;
     for (int64_t i = 0; i < n; ++i)
-        rgba[i] = shade_pixel(geo[i].outcome, geo[i].r, geo[i].redshift, geo[i].magnif, geo[i].theta_esc,
-                              geo[i].phi_esc, p, bg, bw, bh);
+        rgba[i] = shade_pixel(geo[i].outcome, geo[i].r, geo[i].redshift, geo[i].magnif, geo[i].phi_disk,
+                              geo[i].theta_esc, geo[i].phi_esc, p, bg, bw, bh);
+    return 0;
+}
+
+// Interstellar palette on the host (same code as the shade kernel)
+extern "C" int ktc_shade_interstellar(const kt_geopixel* geo, int64_t n, double M, double r_isco, double r_in,
+                                      double r_out, int rings, int sectors, double sigma, double hue, double exposure,
+                                      double gamma, const uint8_t* bg, int bw, int bh, uint32_t* rgba) {
+    ShadeParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.M = M; p.r_isco = r_isco; p.exposure = exposure; p.gamma = gamma; p.temp_scale = 1.0;
+    p.palette = 1; p.rings = rings; p.sectors = sectors; p.sigma = sigma; p.hue_offset = hue;
+    p.r_in = r_in; p.r_out = r_out;
+    for (int64_t i = 0; i < n; ++i)
+        rgba[i] = shade_pixel(geo[i].outcome, geo[i].r, geo[i].redshift, geo[i].magnif, geo[i].phi_disk,
+                              geo[i].theta_esc, geo[i].phi_esc, p, bg, bw, bh);
     return 0;
 }
 

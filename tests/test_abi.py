@@ -33,7 +33,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     assert ctypes.sizeof(_abi.KtFrame) == 11 * 8 + 2 * 4
-    assert ctypes.sizeof(_abi.KtOpts) == 6 * 4 + 6 * 8 + 4 * 4 + 2 * 8
+    assert ctypes.sizeof(_abi.KtOpts) == 6 * 4 + 6 * 8 + 4 * 4 + 2 * 8 + 4 * 4 + 2 * 8
+    assert (_abi.KtOpts.palette.offset, _abi.KtOpts.disk_hue_offset.offset) == (104, 128)
     assert _abi.GEOPIXEL_DTYPE.itemsize == 28
 
 
@@ -44,6 +45,7 @@ def test_host_only_entry_points_without_gpu():
     lib.kt_hip_default_opts(ctypes.byref(o))
     assert (o.chart, o.integrator, o.precision, o.max_steps) == (0, 0, 0, 500000)
     assert o.rtol == 1e-7 and o.step_init == 1.0 and o.quirks == _abi.KT_QUIRKS_REFERENCE
+    assert (o.palette, o.disk_rings, o.disk_sectors, o.disk_sigma, o.disk_hue_offset) == (0, 7, 14, 0.5, 0.0)
     bg = kt.checker_background(1234, 64, 32, 32, 16)
     from tests.golden.make_golden import checker
     assert np.array_equal(bg, checker(1234, 64, 32, 32, 16))

@@ -131,6 +131,32 @@ def test_cli_geo_only_and_color_only(gpu_ctx, tmp_path, sky_png):
     np.testing.assert_array_equal(kt.read_image_rgb(png).reshape(-1, 3), want)
 
 
+def test_cli_interstellar_palette(gpu_ctx, tmp_path, sky_png):
+    W, H = 48, 32
+    cells = ["--disk-interstellar", "--disk-rings", "5", "--disk-sectors", "9", "--disk-sigma", "0.8",
+             "--disk-hue-offset", "0.3"]
+    out = _run(["--ks", "--custom-res", str(W), str(H), "--a", "0.5", "--theta", "80", "--fov", "45",
+                "--bg", sky_png] + cells, tmp_path)
+    png = re.search(r"Saved: (\S+\.png)", out).group(1)
+    gpu_ctx.set_background(_sky())
+    try:
+        o = _opts(chart=kt.KT_CHART_KS, palette=kt.KT_PALETTE_INTERSTELLAR, disk_rings=5, disk_sectors=9,
+                  disk_sigma=0.8, disk_hue_offset=0.3)
+        ref = gpu_ctx.render([kt.make_frame(W, H, a=0.5, theta=80, fov=45, disk_out=12)], o)
+    finally:
+        gpu_ctx.set_background(None)
+    np.testing.assert_array_equal(kt.read_image_rgb(png).reshape(-1, 3), kt.rgba_to_rgb(ref["rgba"]))
+    disk = ref["geo"]["outcome"] == 1
+    assert disk.sum() > 50
+    # a kgeo re-coloured with the same cell options gives the same pixels
+    kg = str(tmp_path / "i.kgeo")
+    _run(["--ks", "--custom-res", str(W), str(H), "--a", "0.5", "--theta", "80", "--fov", "45", "--geo-only",
+          "--geo-file", kg], tmp_path)
+    out = _run(["--color-only", kg, "--bg", sky_png] + cells, tmp_path)
+    png2 = re.search(r"Saved: (\S+_recolor_\d{8}-\d{6}\.png)", out).group(1)
+    np.testing.assert_array_equal(kt.read_image_rgb(png2), kt.read_image_rgb(png))
+
+
 def test_cli_full_render_naming_and_pixels(gpu_ctx, tmp_path, sky_png):
     W, H = 48, 32
     kg = str(tmp_path / "g.kgeo")

@@ -220,6 +220,44 @@ def test_device_colorize_vs_oracle(gpu_ctx):
     assert (got != want).any(axis=1).mean() < 1e-3
 
 
+@pytest.mark.parametrize("geo_name,img_name,cells", [
+    ("c1_bl_rk4_320x180", "c1_interstellar_rgb_320x180", {}),
+    ("ks_rk4_spin_pos_160x90", "ks_interstellar_rgb_160x90",
+     dict(disk_rings=5, disk_sectors=9, disk_sigma=0.8, disk_hue_offset=0.3))])
+def test_device_colorize_interstellar_vs_reference(gpu_ctx, geo_name, img_name, cells):
+    """Phase 2 on the device with the Interstellar palette over the reference's GeoPixels."""
+    from tests.golden.make_golden import checker
+    g = load_golden(geo_name)
+    m = g["meta"]
+    gpu_ctx.set_background(checker())
+    try:
+        rgba = gpu_ctx.colorize(g["pixels"], m[0], m[4], _opts(palette=kt.KT_PALETTE_INTERSTELLAR, **cells),
+                                r_disk_in=m[5], r_disk_out=m[6])
+    finally:
+        gpu_ctx.set_background(None)
+    want = load_golden(img_name)["rgb"].reshape(-1, 3).astype(int)
+    got = kt.rgba_to_rgb(rgba).astype(int)
+    assert np.abs(got - want).max() <= 1
+    assert (got != want).any(axis=1).mean() < 1e-3
+
+
+def test_c1_interstellar_render_vs_reference_png(gpu_ctx):
+    """Full render (trace + shade in one graph) with the Interstellar palette at configuration 1."""
+    from tests.golden.make_golden import checker
+    W, H = 320, 180
+    gpu_ctx.set_background(checker())
+    try:
+        out = gpu_ctx.render([kt.make_frame(W, H, a=0.5, theta=80, r_obs=40, fov=45, disk_out=12)],
+                             _opts(palette=kt.KT_PALETTE_INTERSTELLAR))
+    finally:
+        gpu_ctx.set_background(None)
+    want = load_golden("c1_interstellar_rgb_320x180")["rgb"].reshape(-1, 3).astype(int)
+    got = kt.rgba_to_rgb(out["rgba"]).astype(int)
+    band = photon_ring_band(load_golden("c1_bl_rk4_320x180")["pixels"], W, H)
+    diff = np.abs(got - want).max(axis=1)
+    assert np.mean(diff[~band] <= 2) >= 0.999
+
+
 def test_invalid_arguments_are_reported(gpu_ctx):
     with pytest.raises(kt.KtError):
         gpu_ctx.render([kt.make_frame(0, 10, a=0.5)], _opts())

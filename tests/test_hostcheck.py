@@ -91,6 +91,20 @@ def test_shading_matches_oracle():
     assert (got != want).any(axis=1).mean() < 1e-3
 
 
+@pytest.mark.parametrize("geo_name,img_name,cells", [
+    ("c1_bl_rk4_320x180", "c1_interstellar_rgb_320x180", {}),
+    ("ks_rk4_spin_pos_160x90", "ks_interstellar_rgb_160x90", dict(rings=5, sectors=9, sigma=0.8, hue=0.3))])
+def test_interstellar_shading_matches_reference(geo_name, img_name, cells):
+    """The shade kernel's Interstellar palette (its own sincos in the cell hash) built for the
+    host, against the reference binary's PNGs: bit-exact."""
+    from tests.golden.make_golden import checker
+    g = load_golden(geo_name)
+    m = g["meta"]
+    rgba = hc.shade_interstellar(g["pixels"], m[0], m[4], m[5], m[6], bg=checker(), **cells)
+    got = np.stack([rgba & 0xFF, (rgba >> 8) & 0xFF, (rgba >> 16) & 0xFF], -1)
+    assert np.array_equal(got, load_golden(img_name)["rgb"].reshape(-1, 3))
+
+
 def test_max_steps_one_positional_results():
     """--max-steps 1: every ray stops after one accepted step with the reference's
     positional TraceResult fill {ESCAPED, r, 1, theta, phi} (main.cpp:433)."""

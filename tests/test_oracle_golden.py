@@ -105,6 +105,25 @@ def test_oracle_colorize_matches_reference_png():
     assert man["background"]["cells_x"] == 32
 
 
+INTERSTELLAR = [("c1_bl_rk4_320x180", "c1_interstellar_rgb_320x180", {}),
+                ("ks_rk4_spin_pos_160x90", "ks_interstellar_rgb_160x90",
+                 dict(rings=5, sectors=9, sigma=0.8, hue_offset=0.3))]
+
+
+@pytest.mark.parametrize("geo_name,img_name,cells", INTERSTELLAR)
+def test_oracle_interstellar_matches_reference_png(geo_name, img_name, cells):
+    """Interstellar palette (main.cpp:1689-1797) over the reference's own GeoPixels: bit-exact
+    against the reference binary's PNG (default cells, and every cell option changed)."""
+    g = load_golden(geo_name)
+    M, r_isco, r_in, r_out = g["meta"][0], g["meta"][4], g["meta"][5], g["meta"][6]
+    want = load_golden(img_name)["rgb"].reshape(-1, 3)
+    rgb = kto.colorize_interstellar(g["pixels"], M, r_isco, r_in, r_out, bg=_sky(), **cells)
+    assert np.array_equal(rgb, want)
+    disk = g["pixels"]["outcome"] == 1
+    assert len(np.unique(rgb[disk], axis=0)) > 500
+    assert not np.array_equal(rgb[disk], kto.colorize(g["pixels"], M, r_isco, bg=_sky())[disk])
+
+
 def test_oracle_spp_resolve_matches_reference_png():
     """camera-spp=3: three offset passes resolved in linear space (main.cpp:2232-2322)."""
     bg = _sky()
