@@ -287,8 +287,8 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 template <int CHART, int METHOD, int PREC, bool PHI>
 struct GeodesicPolicy {
     typedef RayState State;
-    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, State& S) {
-        ray_load_init(A.init[F.pix_base + (int64_t)py*F.W + px], A.O, S);
+    __device__ static void init(const TraceArgs& A, const FrameDev&, int, int, int64_t pix, State& S) {
+        ray_load_init(A.init[pix], A.O, S);
     }
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
                                    const Knds<float>& Pf, State& S, RayOut& o, double& magnif) {
@@ -302,7 +302,7 @@ struct GeodesicPolicy {
 
 struct BundlePolicy {
     typedef BundleState State;
-    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, State& B) {
+    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, int64_t, State& B) {
         bundle_init(F, A.O, px, py, B);
     }
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>&, const Knds<float>&,
@@ -322,9 +322,9 @@ struct AdjointForwardPolicy {
         double* c = R.ckpt + 6*slot;
         c[0] = g.y[0]; c[1] = g.y[1]; c[2] = g.y[2]; c[3] = g.y[3]; c[4] = g.sc.s; c[5] = g.sc.c;
     }
-    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, State& S) {
+    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, int64_t pix, State& S) {
         ray_init<CHART, PREC_FP64>(F, A.O, px, py, S.g);
-        S.pix = F.pix_base + (int64_t)py*F.W + px;
+        S.pix = pix;
         const AdjRecord& R = A.rec;
         if (R.hoff) checkpoint(R, R.coff[S.pix], S.g);
     }
@@ -358,8 +358,12 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
     double magnif = 1.0;
     bool active = false;
     int frame = 0;
-    int px = 0, py = 0;
-    // single frame: the constants are wave-uniform (scalar registers)
+    int64_t pix = 0;        // output pixel of the lane's ray (frames back to back)
+    // single frame: a register copy of the frame, read once before any store (the
+    // compiler keeps the fields the loop uses, wave-uniform); the kernel's stores could
+    // alias the frame table, so reads through A.frames inside the loop are vector loads
+    FrameDev F0;
+    if (SINGLE) F0 = A.frames[0];
     Knds<double> P = A.frames[0].P;
     Knds<float> Pf = knds_cast<float>(P);
     const int n_frames = A.desc->n_frames;
@@ -397,15 +401,16 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
             }
             if (mine >= 0) {
                 frame = SINGLE ? 0 : find_frame(A.frames, n_frames, mine);
-                const FrameDev& F = A.frames[frame];
+                const FrameDev& F = SINGLE ? F0 : A.frames[frame];
                 const int64_t loc = mine - F.ray_base;
                 const int64_t tile = loc >> 6;
                 const int w = (int)(loc & 63);
-                px = (int)(tile % F.tiles_x) * 8 + (w & 7);
-                py = (int)(tile / F.tiles_x) * 8 + (w >> 3);
+                const int px = (int)(tile % F.tiles_x) * 8 + (w & 7);
+                const int py = (int)(tile / F.tiles_x) * 8 + (w >> 3);
                 if (px < F.W && py < F.H) {
                     if (!SINGLE) { P = F.P; Pf = knds_cast<float>(P); }
-                    Policy::init(A, F, px, py, S);
+                    pix = F.pix_base + (int64_t)py * F.W + px;
+                    Policy::init(A, F, px, py, pix, S);
                     active = true;
                 }
             }
@@ -415,11 +420,11 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
             continue;
         }
         if (active) {
-            const FrameDev& F = A.frames[SINGLE ? 0 : frame];
+            const FrameDev& F = SINGLE ? F0 : A.frames[frame];
             if (Policy::advance(A, F, P, Pf, S, o, magnif)) {
                 uint32_t wv[7];
                 pack_geopixel_m(o, magnif, wv);
-                uint32_t* dst = A.geo + 7 * (F.pix_base + (int64_t)py * F.W + px);
+                uint32_t* dst = A.geo + 7 * pix;
                 for (int i = 0; i < 7; ++i) dst[i] = wv[i];
                 const RayState& G = Policy::geo(S);
                 c_acc += (uint32_t)G.n_acc;

@@ -326,6 +326,8 @@ struct RayState {
     double h;
     double k1[5];          // RHS at y (FSAL / first stage); [4] = dphi/dl
     SinCos<double> sc;     // sin, cos of y[1]
+    double r_cut, r_esc;   // the frame's horizon cut and escape radius, tested every step:
+                           // per lane in registers, not re-read from the frame table
     int32_t n_acc, n_rej;
     int32_t rhs32, rhs64, hermites, rej_total;
 };
@@ -431,15 +433,16 @@ KT_HD bool step_events(const FrameDev& F, double r0, double th0, double ph0, dou
             }
         }
     }
-    if (((r0 > F.r_cut) && (r1 <= F.r_cut)) || (r1 <= F.r_cut)) {
+    const double r_cut = S.r_cut, r_esc = S.r_esc;
+    if (((r0 > r_cut) && (r1 <= r_cut)) || (r1 <= r_cut)) {
         const double dn = r0 - r1;
-        double ah = (fabs(dn) > 1e-12) ? ((r0 - F.r_cut) / dn) : 0.0;
+        double ah = (fabs(dn) > 1e-12) ? ((r0 - r_cut) / dn) : 0.0;
         ah = ah < 0.0 ? 0.0 : (ah > 1.0 ? 1.0 : ah);
         if (ah < best) { best = ah; ev = 2; }
     }
-    if (((r0 < F.r_esc) && (r1 >= F.r_esc)) || (r1 >= F.r_esc)) {
+    if (((r0 < r_esc) && (r1 >= r_esc)) || (r1 >= r_esc)) {
         const double dn = r1 - r0;
-        double ae = (fabs(dn) > 1e-12) ? ((F.r_esc - r0) / dn) : 1.0;
+        double ae = (fabs(dn) > 1e-12) ? ((r_esc - r0) / dn) : 1.0;
         ae = ae < 0.0 ? 0.0 : (ae > 1.0 ? 1.0 : ae);
         if (ae < best) { best = ae; ev = 3; }
     }
@@ -448,7 +451,7 @@ KT_HD bool step_events(const FrameDev& F, double r0, double th0, double ph0, dou
     if (ev == 1) { o.fate = 1; o.r = d_r; o.g = d_g; o.ph_disk = d_ph; }
     else if (ev == 2) { o.fate = 2; o.r = r0 + best*(r1 - r0); o.g = 0.0; }
     else {
-        o.fate = 0; o.r = F.r_esc; o.g = 1.0;
+        o.fate = 0; o.r = r_esc; o.g = 1.0;
         const double te = th0 + best*(th1 - th0);
         const double pe = out_phi<CHART>(ph0 + best*(S.ph - ph0));
         if (quirks & Q_ESC_POSITIONAL) { o.ph_disk = te; o.th_esc = pe; o.ph_esc = 0.0; }
@@ -676,6 +679,7 @@ KT_HD void ray_init(const FrameDev& F, const RunOpts& O, int px, int py, RayStat
     S.h = O.step_init < 1e-10 ? 1e-10 : O.step_init;
     kt_sincos(S.y[1], &S.sc.s, &S.sc.c);
     rhs<CHART>(F.P, S.y[0], S.sc, S.y[2], S.y[3], S.pt, S.pp, S.k1);
+    S.r_cut = F.r_cut; S.r_esc = F.r_esc;
     S.n_acc = 0; S.n_rej = 0;
     S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0;
 }
@@ -685,7 +689,7 @@ KT_HD void ray_init(const FrameDev& F, const RunOpts& O, int px, int py, RayStat
 // camera (tetrad, two square-root solves, the Kerr-Schild covector, sin/cos, first RHS)
 // for the whole wave every time one lane needs a new ray.
 struct RayInit {
-    double y[4], ph, pt, pp, k1[5], s, c, pad[2];
+    double y[4], ph, pt, pp, k1[5], s, c, r_cut, r_esc;
 };
 static_assert(sizeof(RayInit) == 128, "init record");
 
@@ -694,7 +698,7 @@ KT_HD void ray_store_init(const RayState& S, RayInit& R) {
     R.ph = S.ph; R.pt = S.pt; R.pp = S.pp;
     for (int i = 0; i < 5; ++i) R.k1[i] = S.k1[i];
     R.s = S.sc.s; R.c = S.sc.c;
-    R.pad[0] = R.pad[1] = 0.0;
+    R.r_cut = S.r_cut; R.r_esc = S.r_esc;
 }
 
 KT_HD void ray_load_init(const RayInit& R, const RunOpts& O, RayState& S) {
@@ -702,6 +706,7 @@ KT_HD void ray_load_init(const RayInit& R, const RunOpts& O, RayState& S) {
     S.ph = R.ph; S.pt = R.pt; S.pp = R.pp;
     for (int i = 0; i < 5; ++i) S.k1[i] = R.k1[i];
     S.sc.s = R.s; S.sc.c = R.c;
+    S.r_cut = R.r_cut; S.r_esc = R.r_esc;
     S.h = O.step_init < 1e-10 ? 1e-10 : O.step_init;
     S.n_acc = 0; S.n_rej = 0;
     S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0;
