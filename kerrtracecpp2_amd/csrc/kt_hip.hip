@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -403,7 +404,7 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
                 frame = SINGLE ? 0 : find_frame(A.frames, n_frames, mine);
                 const FrameDev& F = SINGLE ? F0 : A.frames[frame];
                 const int64_t loc = mine - F.ray_base;
-                const int64_t tile = loc >> 6;
+                const int64_t tile = F.tile_order ? (int64_t)F.tile_order[loc >> 6] : (loc >> 6);
                 const int w = (int)(loc & 63);
                 const int px = (int)(tile % F.tiles_x) * 8 + (w & 7);
                 const int py = (int)(tile / F.tiles_x) * 8 + (w >> 3);
@@ -690,6 +691,9 @@ struct kt_hip_ctx {
     // autotuner results per (chart, integrator, precision, bundles)
     kt_tune tune[2][2][2][2];
     bool tuned[2][2][2][2] = {};
+    // tile processing orders per frame shape (tiles_x, tiles_y), kept for the context's
+    // lifetime: frame tables and captured graphs point into them
+    std::map<uint64_t, uint32_t*> orders;
 };
 
 static int fail(kt_hip_ctx* c, int code, const char* fmt, ...) {
@@ -778,6 +782,7 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     (void)hipFree(c->d_words);
     (void)hipFree(c->d_rgb);
     (void)hipFree(c->d_tab);
+    for (auto& kv : c->orders) (void)hipFree(kv.second);
     for (int i = 0; i < 4; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -845,6 +850,38 @@ static int table_reserve(kt_hip_ctx* c, FrameTable& t, size_t n) {
 }
 
 // Fill the staging copy of the table (descriptor + frames) for a frame list.
+// The order in which the work queue hands out a frame's 8x8 tiles: by distance from the
+// image centre, ties in row-major order.  The camera looks at the hole, so the longest
+// rays (the photon ring, up to ~6x the mean step count) lie near the centre and the
+// frame's rim holds short sky and near-disk rays; starting the long rays first keeps the
+// persistent waves full until the queue drains instead of idling most lanes behind a
+// few long rays at the end.  Uploaded once per frame shape (never during a capture:
+// callers prepare frames before they enqueue).
+static int tile_order(kt_hip_ctx* c, int tx, int ty, const uint32_t** out) {
+    const uint64_t key = ((uint64_t)(uint32_t)tx << 32) | (uint32_t)ty;
+    auto it = c->orders.find(key);
+    if (it != c->orders.end()) { *out = it->second; return KT_OK; }
+    const size_t n = (size_t)tx * ty;
+    std::vector<uint32_t> ord(n);
+    std::vector<int64_t> d2(n);
+    for (int y = 0; y < ty; ++y)
+        for (int x = 0; x < tx; ++x) {
+            const int64_t dx = 2 * (int64_t)x - (tx - 1), dy = 2 * (int64_t)y - (ty - 1);
+            d2[(size_t)y * tx + x] = dx * dx + dy * dy;
+        }
+    for (size_t i = 0; i < n; ++i) ord[i] = (uint32_t)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t p, uint32_t q) { return d2[p] < d2[q]; });
+    uint32_t* d = nullptr;
+    if (hipMalloc(&d, n * sizeof(uint32_t)) != hipSuccess) return fail(c, KT_E_ALLOC, "tile order (%zu tiles)", n);
+    if (hipMemcpy(d, ord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return fail(c, KT_E_DEVICE, "tile order upload");
+    }
+    c->orders[key] = d;
+    *out = d;
+    return KT_OK;
+}
+
 static int prepare_frames(kt_hip_ctx* c, FrameTable& t, const kt_frame* frames, int n_frames, const kt_opts* o,
                           int64_t* total_rays, int64_t* total_pix, int* chart_out) {
     if (!frames || n_frames <= 0) return fail(c, KT_E_ARG, "no frames");
@@ -871,6 +908,8 @@ static int prepare_frames(kt_hip_ctx* c, FrameTable& t, const kt_frame* frames, 
         F.W = f.width; F.H = f.height;
         F.tiles_x = (f.width + 7) / 8;
         F.tiles_y = (f.height + 7) / 8;
+        const int orc = tile_order(c, F.tiles_x, F.tiles_y, &F.tile_order);
+        if (orc != KT_OK) return orc;
         F.ray_base = rays;
         F.pix_base = pix;
         F.chart = ch;

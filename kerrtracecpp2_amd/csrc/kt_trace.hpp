@@ -41,6 +41,7 @@ struct FrameDev {
     int32_t W, H, tiles_x, tiles_y;
     int64_t ray_base;                      // first global ray index (8x8 tile padded)
     int64_t pix_base;                      // first output pixel
+    const uint32_t* tile_order;            // queue rank -> 8x8 tile (row-major id); null: row-major
     // derived
     double r_h, r_isco, r_in, r_out, r_esc, r_cut;
     Knds<double> P;
@@ -174,25 +175,32 @@ KT_HD bool crosses(double f0, double f1) { return (f0 <= 0.0 && f1 >= 0.0) || (f
 KT_HD bool hermite_may_touch(double y0, double y1, double d0, double d1, double h, double tgt) {
     const double f0 = y0 - tgt, f1 = y1 - tgt;
     if (crosses(f0, f1)) return true;
-    // cubic in alpha: f(al) = c3 al^3 + c2 al^2 + c1 al + c0
     const double m0 = h*d0, m1 = h*d1;
+    // generous margin for the rounding of the monomial form
+    const double margin = 1e-12 + 1e-12*(fabs(f0) + fabs(f1) + fabs(m0) + fabs(m1));
+    // Bernstein form: the cubic lies in the hull of f0, f0 + m0/3, f1 - m1/3, f1; a hull
+    // clear of [-margin, margin] settles most steps near the plane without the extrema
+    const double b1 = f0 + m0*(1.0/3.0), b2 = f1 - m1*(1.0/3.0);
+    if (fmin(fmin(f0, f1), fmin(b1, b2)) > margin || fmax(fmax(f0, f1), fmax(b1, b2)) < -margin) return false;
+    // cubic in alpha: f(al) = c3 al^3 + c2 al^2 + c1 al + c0
     const double c3 = 2.0*f0 + m0 - 2.0*f1 + m1;
     const double c2 = -3.0*f0 - 2.0*m0 + 3.0*f1 - m1;
     const double c1 = m0;
     double lo = fmin(f0, f1), hi = fmax(f0, f1);
-    // roots of 3 c3 al^2 + 2 c2 al + c1
+    // roots of 3 c3 al^2 + 2 c2 al + c1 (an extremum's position only needs to be close:
+    // the cubic is flat there)
     const double qa = 3.0*c3, qb = 2.0*c2, qc = c1;
     double ex[2];
     int n = 0;
     if (fabs(qa) > 1e-300) {
         const double disc = qb*qb - 4.0*qa*qc;
         if (disc >= 0.0) {
-            const double sq = sqrt(disc);
-            ex[n++] = (-qb - sq) / (2.0*qa);
-            ex[n++] = (-qb + sq) / (2.0*qa);
+            const double sq = sqrt(disc), inv = kt_rcp(2.0*qa);
+            ex[n++] = (-qb - sq) * inv;
+            ex[n++] = (-qb + sq) * inv;
         }
     } else if (fabs(qb) > 1e-300) {
-        ex[n++] = -qc / qb;
+        ex[n++] = -qc * kt_rcp(qb);
     }
     for (int i = 0; i < n; ++i) {
         const double al = ex[i];
@@ -202,8 +210,6 @@ KT_HD bool hermite_may_touch(double y0, double y1, double d0, double d1, double 
             hi = fmax(hi, v);
         }
     }
-    // generous margin for the rounding of the monomial form
-    const double margin = 1e-12 + 1e-12*(fabs(f0) + fabs(f1) + fabs(m0) + fabs(m1));
     return lo <= margin && hi >= -margin;
 }
 
