@@ -12,6 +12,8 @@ import numpy as np
 from . import _abi
 from ._abi import (GEOPIXEL_DTYPE, KT_CHART_BL, KT_CHART_KS, KT_INTEGRATOR_DOPRI5, KT_INTEGRATOR_RK4,  # noqa: F401
                    KT_PALETTE_BLACKBODY, KT_PALETTE_INTERSTELLAR, KT_PRECISION_FP64, KT_PRECISION_MIXED,
+                   KT_QUIRK_BUNDLE_HORIZON_ESC, KT_QUIRK_BUNDLE_ROOT_SKIP, KT_QUIRK_DOPRI5_PHI_FROZEN,
+                   KT_QUIRK_ESCAPE_POSITIONAL, KT_QUIRK_KS_INIT_NO_SHIFT, KT_QUIRK_KS_LAMBDA_TO_BL,
                    KT_QUIRKS_REFERENCE, KtFrame, KtOpts, default_opts, make_frame)
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))

@@ -77,25 +77,32 @@ struct BundleState {
     double dW[4][2];       // M W at the current state (first stage of the next attempt)
 };
 
-// classic RK4 of the coupled system over h from (y0, W0) with first stage (k0, dW0)
+// classic RK4 of the coupled system over h from (y0, W0) with first stage (k0, dW0).
+// One stage per trip of a loop that is not unrolled: the weighted stage sum is
+// accumulated as the stages come (same order of additions as h/6 (k0 + 2 k2 + 2 k3 + k4)),
+// so one stage's derivatives are live at a time and the compiler cannot interleave the
+// dual-number right-hand sides of different stages (which exhausts the register file).
 KT_HD void bundle_rk4_incr(const Knds<double>& P, const double y0[4], const SinCos<double>& sc0,
                            const double W0[4][2], const double k0[5], const double dW0[4][2], double h, double pt,
                            double pp, double inc[5], double incW[4][2], int& nrhs) {
-    double k2[5], k3[5], k4[5], dW2[4][2], dW3[4][2], dW4[4][2], y[4], W[4][2];
-    SinCos<double> sc;
-    for (int i = 0; i < 4; ++i) { y[i] = y0[i] + 0.5*h*k0[i]; W[i][0] = W0[i][0] + 0.5*h*dW0[i][0]; W[i][1] = W0[i][1] + 0.5*h*dW0[i][1]; }
-    sc = sc_at(sc0, y0[1], 0.5*h*k0[1]);
-    bundle_rhs(P, y, sc, pt, pp, W, k2, dW2);
-    for (int i = 0; i < 4; ++i) { y[i] = y0[i] + 0.5*h*k2[i]; W[i][0] = W0[i][0] + 0.5*h*dW2[i][0]; W[i][1] = W0[i][1] + 0.5*h*dW2[i][1]; }
-    sc = sc_at(sc0, y0[1], 0.5*h*k2[1]);
-    bundle_rhs(P, y, sc, pt, pp, W, k3, dW3);
-    for (int i = 0; i < 4; ++i) { y[i] = y0[i] + h*k3[i]; W[i][0] = W0[i][0] + h*dW3[i][0]; W[i][1] = W0[i][1] + h*dW3[i][1]; }
-    sc = sc_at(sc0, y0[1], h*k3[1]);
-    bundle_rhs(P, y, sc, pt, pp, W, k4, dW4);
+    double k[5], dW[4][2], acc[5], accW[4][2];
+    for (int i = 0; i < 5; ++i) { k[i] = k0[i]; acc[i] = k0[i]; }
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 2; ++j) { dW[i][j] = dW0[i][j]; accW[i][j] = dW0[i][j]; }
+#pragma unroll 1
+    for (int st = 0; st < 3; ++st) {
+        const double c = (st == 2) ? h : 0.5*h;        // stage offset
+        const double wgt = (st == 2) ? 1.0 : 2.0;      // stage weight
+        double y[4], W[4][2];
+        for (int i = 0; i < 4; ++i) { y[i] = y0[i] + c*k[i]; W[i][0] = W0[i][0] + c*dW[i][0]; W[i][1] = W0[i][1] + c*dW[i][1]; }
+        const SinCos<double> sc = sc_at(sc0, y0[1], c*k[1]);
+        bundle_rhs(P, y, sc, pt, pp, W, k, dW);
+        for (int i = 0; i < 5; ++i) acc[i] = acc[i] + wgt*k[i];
+        for (int i = 0; i < 4; ++i) for (int j = 0; j < 2; ++j) accW[i][j] = accW[i][j] + wgt*dW[i][j];
+    }
     nrhs += 3;
-    for (int i = 0; i < 5; ++i) inc[i] = h/6.0*(k0[i] + 2.0*k2[i] + 2.0*k3[i] + k4[i]);
+    for (int i = 0; i < 5; ++i) inc[i] = h/6.0*acc[i];
     for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 2; ++j) incW[i][j] = h/6.0*(dW0[i][j] + 2.0*dW2[i][j] + 2.0*dW3[i][j] + dW4[i][j]);
+        for (int j = 0; j < 2; ++j) incW[i][j] = h/6.0*accW[i][j];
 }
 
 // Doppler factor of the bundle hit (ray_bundle.hpp:368-377): signed 1 - Omega b
@@ -149,22 +156,35 @@ KT_HD int bundle_step(const FrameDev& F, const RunOpts& O, BundleState& B, RayOu
     const double h = S.h;
     const double tol = O.rtol;
     int nrhs = 0;
-    double incA[5], incB1[5], incB2[5], iWA[4][2], iWB1[4][2], iWB2[4][2];
-    bundle_rk4_incr(P, S.y, S.sc, B.W, S.k1, B.dW, h, S.pt, S.pp, incA, iWA, nrhs);
-    bundle_rk4_incr(P, S.y, S.sc, B.W, S.k1, B.dW, 0.5*h, S.pt, S.pp, incB1, iWB1, nrhs);
-    double yB1[4], WB1[4][2], kB1[5], dWB1[4][2];
-    for (int i = 0; i < 4; ++i) {
-        yB1[i] = S.y[i] + incB1[i];
-        WB1[i][0] = B.W[i][0] + iWB1[i][0];
-        WB1[i][1] = B.W[i][1] + iWB1[i][1];
+    // The full step A only enters the error estimate, which is taken on the geodesic part
+    // (ray_bundle.hpp:226-237), and the geodesic does not depend on W: A is the plain
+    // binary64 RK4 increment, without the deviation matrix.
+    double incA[5];
+    rk4_incr<CHART_BL, PREC_FP64>(P, knds_cast<float>(P), true, S.y, S.sc, S.k1, h, S.pt, S.pp, incA, S);
+    // the two half steps B1, B2, one per trip (not unrolled); B2 starts from the
+    // derivatives at the B1 point
+    double yb[4], Wb[4][2], kb[5], dWb[4][2], inc[5], incW[4][2];
+    SinCos<double> scb = S.sc;
+    for (int i = 0; i < 4; ++i) { yb[i] = S.y[i]; Wb[i][0] = B.W[i][0]; Wb[i][1] = B.W[i][1]; }
+    for (int i = 0; i < 5; ++i) kb[i] = S.k1[i];
+    for (int i = 0; i < 4; ++i) { dWb[i][0] = B.dW[i][0]; dWb[i][1] = B.dW[i][1]; }
+    double yB1_th = 0.0, ph_inc = 0.0;
+#pragma unroll 1
+    for (int half = 0; half < 2; ++half) {
+        if (half == 1) {
+            for (int i = 0; i < 4; ++i) { yb[i] = yb[i] + inc[i]; Wb[i][0] = Wb[i][0] + incW[i][0]; Wb[i][1] = Wb[i][1] + incW[i][1]; }
+            scb = sc_at(S.sc, S.y[1], inc[1]);
+            bundle_rhs(P, yb, scb, S.pt, S.pp, Wb, kb, dWb);
+            nrhs++;
+            yB1_th = yb[1];
+        }
+        bundle_rk4_incr(P, yb, scb, Wb, kb, dWb, 0.5*h, S.pt, S.pp, inc, incW, nrhs);
+        ph_inc = (half == 0) ? inc[4] : ph_inc;
     }
-    const SinCos<double> scB1 = sc_at(S.sc, S.y[1], incB1[1]);
-    bundle_rhs(P, yB1, scB1, S.pt, S.pp, WB1, kB1, dWB1);
-    nrhs++;
-    bundle_rk4_incr(P, yB1, scB1, WB1, kB1, dWB1, 0.5*h, S.pt, S.pp, incB2, iWB2, nrhs);
+    // here yb, Wb = the B1 point and inc, incW = the B2 increment
     S.rhs64 += nrhs;
     double e2 = 0.0;
-    for (int i = 0; i < 4; ++i) { const double t = (S.y[i] + incA[i]) - (yB1[i] + incB2[i]); e2 += t*t; }
+    for (int i = 0; i < 4; ++i) { const double t = (S.y[i] + incA[i]) - (yb[i] + inc[i]); e2 += t*t; }
     const double err = sqrt(e2) / 15.0;
     if (!isfinite(err)) {
         const double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
@@ -187,12 +207,12 @@ KT_HD int bundle_step(const FrameDev& F, const RunOpts& O, BundleState& B, RayOu
     const double r0 = S.y[0], th0 = S.y[1], ph0 = S.ph, dr0 = S.k1[0], dth0 = S.k1[1];
     double W0[4][2];
     for (int i = 0; i < 4; ++i) { W0[i][0] = B.W[i][0]; W0[i][1] = B.W[i][1]; }
-    S.ph = S.ph + incB1[4] + incB2[4];
-    const double dth = (yB1[1] - S.y[1]) + incB2[1];
+    S.ph = S.ph + ph_inc + inc[4];
+    const double dth = (yB1_th - S.y[1]) + inc[1];
     for (int i = 0; i < 4; ++i) {
-        S.y[i] = yB1[i] + incB2[i];
-        B.W[i][0] = WB1[i][0] + iWB2[i][0];
-        B.W[i][1] = WB1[i][1] + iWB2[i][1];
+        S.y[i] = yb[i] + inc[i];
+        B.W[i][0] = Wb[i][0] + incW[i][0];
+        B.W[i][1] = Wb[i][1] + incW[i][1];
     }
     S.sc = sc_at(S.sc, th0, dth);
     double hn = h * ((err > 1e-14) ? 0.9*pow_neg(err / tol, 0.2f) : 4.0);

@@ -34,6 +34,15 @@ KT_HD float sqrtf(float x) { return std::sqrt(x); }
 
 #include <stdint.h>
 
+// Scheduling fence: the machine scheduler may not move instructions across it.  Placed
+// between phases of a long straight-line computation whose independent parts it would
+// otherwise interleave, multiplying the registers live at once (a host build ignores it).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KT_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define KT_SCHED_FENCE() ((void)0)
+#endif
+
 namespace kt {
 
 static constexpr double kPi = 3.14159265358979323846;

@@ -459,9 +459,12 @@ void kt_trace_kernel(TraceArgs A) {
     trace_loop<GeodesicPolicy<CHART, METHOD, PREC, PHI>, SINGLE>(A);
 }
 
-// Jacobi ray bundles: BL chart, RK4 step doubling, binary64 (ray_bundle.hpp)
-template <bool SINGLE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
+// Jacobi ray bundles: BL chart, RK4 step doubling, binary64 (ray_bundle.hpp).  A bundle
+// attempt keeps the geodesic, the 4x2 deviation matrix, its derivative and the old state
+// for a reject live across dual-number right-hand sides of ~110 registers each: about
+// 390 registers, so OCC = 1 (512 per lane) holds it without scratch, OCC = 2 spills.
+template <bool SINGLE, int OCC>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
 void kt_bundle_kernel(TraceArgs A) {
     trace_loop<BundlePolicy, SINGLE>(A);
 }
@@ -986,6 +989,12 @@ static RunOpts run_opts(const kt_opts* o) {
 // autotuner's result for this option class, else the built-in defaults
 struct Launch { int bpc, chunk, occ; };
 
+// bundle kernels: occupancy 1 (default) or 2
+static TraceKernelFn pick_bundle_kernel(int occ, bool single) {
+    if (occ == 2) return single ? kt_bundle_kernel<true, 2> : kt_bundle_kernel<false, 2>;
+    return single ? kt_bundle_kernel<true, 1> : kt_bundle_kernel<false, 1>;
+}
+
 static Launch launch_config(const kt_hip_ctx* c, const kt_opts* o, int chart) {
     Launch L{kDefaultBlocksPerCu, kDefaultChunk, 0};
     const int b = o->bundles ? 1 : 0, m = o->integrator ? 1 : 0, pr = o->precision ? 1 : 0, ch = chart ? 1 : 0;
@@ -1014,7 +1023,7 @@ struct RenderBufs {
 static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, int chart, int setup_grid,
                           int64_t max_rays, bool single, bool shade, const Launch& L, hipEvent_t* ev) {
     const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
-    TraceKernelFn fn = o->bundles ? (single ? kt_bundle_kernel<true> : kt_bundle_kernel<false>)
+    TraceKernelFn fn = o->bundles ? pick_bundle_kernel(L.occ, single)
                                   : pick_kernel(chart, o->integrator, o->precision, phi, L.occ, single);
     if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");
     // persistent grid: blocks beyond the resident limit start when others retire and

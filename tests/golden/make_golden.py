@@ -3,6 +3,7 @@
 Run in the build container only (needs /root/reference):
     make -C oracle ref && python tests/golden/make_golden.py
     python tests/golden/make_golden.py --images NAME...     (only those image fixtures)
+    python tests/golden/make_golden.py --c3-sample          (oracle sample of the 1920x1080 bundle frame)
 
 The reference CPU binary (oracle/_ref/kerr_tracer_ref, IEEE flags) produces
 .kgeo files and PNGs for small configurations, and oracle/_ref/ref_harness
@@ -191,7 +192,24 @@ def images_only(names):
         json.dump(manifest, f, indent=1)
 
 
+def c3_bundle_sample(n=256, seed=3):
+    """Configuration 3 at full size (1920x1080, a=0.99, theta=89, BL, RK4 doubling, Jacobi
+    bundles): n seeded pixels traced by the oracle (its bundle path is bit-exact to the
+    reference binary on the 64x36 bundle frame; a full reference run of this frame would
+    take about a day on 8 cores)."""
+    W, H = 1920, 1080
+    fr = kto.make_frame(W, H, a=0.99, theta=89, r_obs=40, disk_out=12, chart="bl", integrator="rk4", bundles=True)
+    rng = np.random.default_rng(seed)
+    px = rng.integers(0, W, n).astype(np.int32)
+    py = rng.integers(0, H, n).astype(np.int32)
+    g = kto.trace_pixels(fr, px, py)
+    np.savez_compressed(os.path.join(HERE, "c3_bundles_sample.npz"), px=px, py=py, pixels=g, W=W, H=H)
+    print("c3 bundle sample", n, np.bincount(g["outcome"], minlength=4))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--c3-sample":
+        return c3_bundle_sample()
     if not (os.path.exists(REF) and os.path.exists(HARN)):
         sys.exit("build the reference first: make -C oracle ref")
     if len(sys.argv) > 2 and sys.argv[1] == "--images":

@@ -285,3 +285,37 @@ def test_bundles_vs_reference(gpu_ctx):
     rel = np.abs(geo["magnif"][d] / g["magnif"][d] - 1.0)
     assert np.mean(rel < 0.10) >= 0.99
     assert out["info"][0]["eff_chart"] == 0
+
+
+def test_c3_full_size_bundles_vs_oracle_sample(gpu_ctx):
+    """Configuration 3 at full size (1920x1080, a=0.99, theta=89, BL, RK4 doubling, Jacobi
+    bundles): 256 seeded pixels against the oracle (tests/golden/c3_bundles_sample.npz).  The
+    band here is taken from the GPU image itself (fate changes among the 8 neighbours, and disk
+    pixels whose g differs from a neighbour's by more than 0.05).
+
+    The GPU runs with the bundle root-skip quirk off: with it, near-equatorial steps without a
+    disk root skip the horizon and escape tests and 58 of these 256 rays run to the 500 000-step
+    cap (outcome 0, r = 0, angles 0), which makes the full frame take minutes.  The quirk also
+    moves the recorded escape point of every escaping ray (to the first step after r_escape that
+    is not skipped), so escapes are compared on fate only; disk pixels in full."""
+    g = load_golden("c3_bundles_sample")
+    W, H = int(g["W"]), int(g["H"])
+    o = _opts(bundles=1)
+    o.quirks &= ~kt.KT_QUIRK_BUNDLE_ROOT_SKIP
+    out = gpu_ctx.render([kt.make_frame(W, H, a=0.99, theta=89, r_obs=40, disk_out=12)], o, want_rgba=False)
+    print("C3 bundles stats", out["stats"], flush=True)
+    full = out["geo"]
+    band = photon_ring_band(full, W, H).reshape(H, W)[g["py"], g["px"]]
+    got = full.reshape(H, W)[g["py"], g["px"]]
+    ref = g["pixels"]
+    keep = ~band
+    assert keep.sum() >= 200
+    assert (got["outcome"] == ref["outcome"])[keep].mean() >= 0.99
+    d = keep & (got["outcome"] == 1) & (ref["outcome"] == 1)
+    assert d.sum() >= 50
+    assert np.abs(got["r"][d] - ref["r"][d]).max() < 2e-3
+    assert np.abs(got["redshift"][d] - ref["redshift"][d]).max() < 5e-4
+    rel = np.abs(got["magnif"][d] / ref["magnif"][d] - 1.0)
+    assert np.mean(rel < 0.10) >= 0.95
+    capped = (ref["outcome"] == 0) & (ref["r"] == 0) & (ref["theta_esc"] == 0) & (ref["phi_esc"] == 0)
+    assert capped.sum() >= 40 and np.all(got["outcome"][capped] == 0)
