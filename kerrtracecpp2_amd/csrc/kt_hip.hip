@@ -73,9 +73,9 @@ __global__ void kt_fill_kernel(unsigned long long* p, int n) {
 }
 
 // ---------------------------------------------------------------------------
-//  setup: one workgroup (1024 threads) per frame
+//  setup: one workgroup (256 threads) per frame
 // ---------------------------------------------------------------------------
-constexpr int kSetupThreads = 1024;   // 16 waves: the scans are latency-bound dependency chains
+constexpr int kSetupThreads = 256;
 
 __device__ double bisect_ref(const FrameDev& F, double lo, double hi) {
     double flo = delta_r_ref(F, lo), fhi = delta_r_ref(F, hi);
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kSetupThreads) void kt_setup_kernel(FrameDev* frame
             __syncthreads();
             risco = s_rh;
         } else {
-            const int per = 8;
+            const int per = 32;
             const int64_t round_len = (int64_t)kSetupThreads * per;
             for (int64_t base = 0;; base += round_len) {
                 if (t == 0) s_best = ~0ull;
