@@ -32,23 +32,23 @@ RTOL, ATOL = 1e-9, 1e-12
 
 # Algorithmic floating-point work per unit (add/sub/mul 1, fma 2, div/sqrt/rcp 1):
 #   right-hand side, counted by tests/hostcheck/flopcount.cpp on the kernel source,
-#   incremental sin/cos rotation per stage (binary64 / binary32 form),
 #   DOPRI5 stage combination + embedded error + update per attempt,
+#   integrated stage sin/cos per attempt (u = cos theta', v = -sin theta' for stages 1-5 and
+#   their tableau combinations for stages 2-6),
+#   one sin/cos rotation per attempt (the FSAL stage; |d| <= 1/8 series),
 #   Hermite 8-bin/8-bisection disk search.
 F_RHS = {"bl": 98, "ks": 93}
-F_ROT = {64: 31, 32: 31}      # |d| <= 1/8 series (nearly every stage)
 F_DP5_COMB = 256
+F_SCI = 80
+F_ROT = 31
 F_HERMITE = 370
 PEAK_VALU_TFLOPS = 78.6   # MI355X vector FP64; FP32 without packed math issues at the same rate
 
 
 def flops_of(st, chart="ks"):
     attempts = st["steps_accepted"] + st["steps_rejected"]
-    rays = st["rays"]
-    stage64 = max(st["rhs_fp64"] - rays, 0)          # the first RHS of a ray uses a full sincos
-    stage32 = st["rhs_fp32"]
-    return (st["rhs_fp64"] * F_RHS[chart] + st["rhs_fp32"] * F_RHS[chart] + stage64 * F_ROT[64]
-            + stage32 * F_ROT[32] + attempts * F_DP5_COMB + st["events"] * F_HERMITE)
+    return ((st["rhs_fp64"] + st["rhs_fp32"]) * F_RHS[chart] + attempts * (F_DP5_COMB + F_SCI + F_ROT)
+            + st["events"] * F_HERMITE)
 
 
 def gpu_run(ctx, kt, prec, steps, warmup):

@@ -334,7 +334,7 @@ struct AdjointForwardPolicy {
         const double h = S.g.h;
         const int n0 = S.g.n_acc;
         magnif = 1.0;
-        const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false>(F, P, Pf, A.O, true, S.g, o);
+        const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false, false>(F, P, Pf, A.O, true, S.g, o);
         const AdjRecord& R = A.rec;
         if (R.hoff && S.g.n_acc > n0 && n0 < R.n_cap[S.pix]) {
             R.hs[R.hoff[S.pix] + n0] = h;

@@ -381,6 +381,25 @@ KT_HD void stage(const Knds<double>& P, const Knds<float>& Pf, bool use64, const
     }
 }
 
+// stage derivative at y0 + d with the sin/cos of theta given (integrated stage values)
+template <int CHART, int PREC, int NC>
+KT_HD void stage_given(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
+                       const double d[4], const SinCos<double>& sc, double pt, double pp, double k[NC], RayState& S) {
+    if (PREC == PREC_FP64 || use64) {
+        double o[5];
+        rhs<CHART>(P, y0[0] + d[0], sc, y0[2] + d[2], y0[3] + d[3], pt, pp, o);
+        for (int i = 0; i < NC; ++i) k[i] = o[i];
+        S.rhs64++;
+    } else {
+        const SinCos<float> scf = {(float)sc.s, (float)sc.c};
+        float kf[5];
+        rhs<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
+                   (float)pt, (float)pp, kf);
+        for (int i = 0; i < NC; ++i) k[i] = (double)kf[i];
+        S.rhs32++;
+    }
+}
+
 // step-size factor err^(-q) in binary32 (the controller needs ~3 digits)
 KT_HD double pow_neg(double err, float q) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -534,7 +553,15 @@ KT_HD int after_reject(const RunOpts& O, RayState& S) {
 //  DOPRI5 attempt (geodesic.hpp:312-414), ordered for a small live set (register
 //  economy); NC = 5 also advances phi with the same weights.
 // ---------------------------------------------------------------------------
-template <int CHART, int PREC, int NC>
+//
+// SCI (integrated stage sin/cos): the metric depends on theta only through sin and cos,
+// so stages 2-6 take (sin, cos) as two more integrated components,
+//   d(sin theta)/dl = cos theta * theta',   d(cos theta)/dl = -sin theta * theta',
+// combined with the same tableau as the other components (a consistent 5th-order method
+// for the augmented system), instead of a Taylor rotation per stage.  The FSAL stage at the
+// accepted point keeps the exact rotation, so every step starts from sin/cos of theta.
+// SCI = false (the differentiable path) rotates at every stage, as its replay does.
+template <int CHART, int PREC, int NC, bool SCI>
 KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
                    RayState& S, RayOut& o) {
     typedef DP5 C;
@@ -546,16 +573,49 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     // k1 are live while a stage is evaluated.
     double k2[NC], k3[NC], k4[NC];
     double d[4], a5[NC], ae[4];
+    // SCI: stage sin/cos, and their derivatives u = cos*theta', v = -sin*theta' per stage
+    const double s1 = S.sc.s, c1 = S.sc.c;
+    const double u1 = c1*k1[1], v1 = -s1*k1[1];
+    double u2 = 0.0, v2 = 0.0, u3 = 0.0, v3 = 0.0, u4 = 0.0, v4 = 0.0;
+    SinCos<double> sc;
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
-    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2, S);
+    if (SCI) {
+        sc.s = s1 + h*(C::a21*u1); sc.c = c1 + h*(C::a21*v1);
+        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k2, S);
+        u2 = sc.c*k2[1]; v2 = -sc.s*k2[1];
+    } else {
+        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2, S);
+    }
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
-    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3, S);
+    if (SCI) {
+        sc.s = s1 + h*(C::a31*u1 + C::a32*u2); sc.c = c1 + h*(C::a31*v1 + C::a32*v2);
+        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k3, S);
+        u3 = sc.c*k3[1]; v3 = -sc.s*k3[1];
+    } else {
+        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3, S);
+    }
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
-    stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4, S);
+    if (SCI) {
+        sc.s = s1 + h*(C::a41*u1 + C::a42*u2 + C::a43*u3); sc.c = c1 + h*(C::a41*v1 + C::a42*v2 + C::a43*v3);
+        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k4, S);
+        u4 = sc.c*k4[1]; v4 = -sc.s*k4[1];
+    } else {
+        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4, S);
+    }
     {
         double k5[NC];
         for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
-        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5, S);
+        double u5 = 0.0, v5 = 0.0;
+        if (SCI) {
+            sc.s = s1 + h*(C::a51*u1 + C::a52*u2 + C::a53*u3 + C::a54*u4);
+            sc.c = c1 + h*(C::a51*v1 + C::a52*v2 + C::a53*v3 + C::a54*v4);
+            stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k5, S);
+            u5 = sc.c*k5[1]; v5 = -sc.s*k5[1];
+            sc.s = s1 + h*(C::a61*u1 + C::a62*u2 + C::a63*u3 + C::a64*u4 + C::a65*u5);
+            sc.c = c1 + h*(C::a61*v1 + C::a62*v2 + C::a63*v3 + C::a64*v4 + C::a65*v5);
+        } else {
+            stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5, S);
+        }
         for (int i = 0; i < 4; ++i)
             d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
         for (int i = 0; i < NC; ++i) a5[i] = C::b1*k1[i] + C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i];
@@ -563,7 +623,8 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     }
     {
         double k6[NC];
-        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6, S);
+        if (SCI) stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k6, S);
+        else stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6, S);
         for (int i = 0; i < NC; ++i) a5[i] += C::b6*k6[i];
         for (int i = 0; i < 4; ++i) ae[i] += C::e6*k6[i];
     }
@@ -753,11 +814,12 @@ KT_HD void pack_geopixel_m(const RayOut& o, double magnif, uint32_t w[7]) {
 
 // One attempt + bookkeeping for an active ray; returns true when the ray is done.
 // PHI: DOPRI5 advances phi (false reproduces the reference's frozen phi).
-template <int CHART, int METHOD, int PREC, bool PHI>
+// SCI: DOPRI5 integrates the stage sin/cos (dp5_step); false rotates at every stage.
+template <int CHART, int METHOD, int PREC, bool PHI, bool SCI = true>
 KT_HD bool ray_advance(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O,
                        bool use64, RayState& S, RayOut& o) {
     int rc;
-    if (METHOD == METHOD_DP5) rc = dp5_step<CHART, PREC, PHI ? 5 : 4>(F, P, Pf, O, use64, S, o);
+    if (METHOD == METHOD_DP5) rc = dp5_step<CHART, PREC, PHI ? 5 : 4, SCI>(F, P, Pf, O, use64, S, o);
     else rc = rk4_step<CHART, PREC>(F, P, Pf, O, use64, S, o);
     if (rc != STEP_DONE) return false;
     if (S.n_rej > 64 || !isfinite(S.h)) stopped_result<CHART>(S, O.quirks, o);   // reject cap

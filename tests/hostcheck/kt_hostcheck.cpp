@@ -185,7 +185,7 @@ static void adjoint_run(const FrameDev& F, const RunOpts& O, const int32_t* px, 
         for (;;) {
             const double h = S.h;
             const int n0 = S.n_acc;
-            const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false>(F, F.P, Pf, O, true, S, o);
+            const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false, false>(F, F.P, Pf, O, true, S, o);
             if (S.n_acc > n0) {
                 hs.push_back(h);
                 if (!done && S.n_acc % kCkptEvery == 0) push_ck();
@@ -330,7 +330,7 @@ extern "C" int ktc_ray_steps(const kt_frame* f, const kt_opts* o, int px, int py
         for (;;) {
             const double h = S.h;
             const int n0 = S.n_acc;
-            const bool done = ray_advance<CH, METHOD_DP5, PREC_FP64, false>(F, F.P, Pf, O, true, S, out);
+            const bool done = ray_advance<CH, METHOD_DP5, PREC_FP64, false, false>(F, F.P, Pf, O, true, S, out);
             if (S.n_acc > n0 && n < cap) hs[n++] = h;
             if (done) break;
         }

@@ -48,8 +48,12 @@ def _check(ref, got, W, H, dg_tol=1e-4, dr_tol=2e-3, fate_min=0.999):
     if d.any():
         dg = np.abs(ref["redshift"][d] - got["redshift"][d]).max()
         dr = np.abs(ref["r"][d] - got["r"][d]).max()
-        assert dg <= dg_tol, f"max |dg| {dg}"
-        assert dr <= dr_tol, f"max |dr| {dr}"
+        worst = np.flatnonzero(d)[np.argmax(np.abs(ref["redshift"][d] - got["redshift"][d]))]
+        where = (f"pixel ({worst % W}, {worst // W}): g {ref['redshift'][worst]} vs {got['redshift'][worst]}, "
+                 f"r {ref['r'][worst]} vs {got['r'][worst]}; 3x3 g of the first image "
+                 f"{ref['redshift'].reshape(H, W)[max(worst // W - 1, 0):worst // W + 2, max(worst % W - 1, 0):worst % W + 2].tolist()}")
+        assert dg <= dg_tol, f"max |dg| {dg} at {where}"
+        assert dr <= dr_tol, f"max |dr| {dr} at {where}"
     return band
 
 
