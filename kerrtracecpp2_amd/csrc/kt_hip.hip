@@ -2,8 +2,8 @@
 //  kt_hip.hip -- HIP kernels (gfx950) and the C ABI of include/kt_hip.h.
 //
 //  Kernels:
-//    kt_setup_kernel    one workgroup per frame: horizon (parallel Delta_r scan +
-//                       bisection), ISCO (exact parallel scan), camera constants
+//    kt_setup_*_kernel  per frame: horizon (Delta_r scan over 16 workgroups, bisection),
+//                       constants, ISCO (exact scan over 32 workgroups)
 //    kt_trace_kernel    persistent, work-queue driven: each wave64 pulls 64-ray
 //                       chunks (8x8 pixel tiles) from a device counter and refills
 //                       lanes whose ray finished, so uneven step counts do not idle
@@ -81,7 +81,11 @@ __device__ double bisect_ref(const FrameDev& F, double lo, double hi) {
     double flo = delta_r_ref(F, lo), fhi = delta_r_ref(F, hi);
     if (!isfinite(flo) || !isfinite(fhi)) return NAN;
     for (int i = 0; i < 160; ++i) {
-        const double mid = 0.5*(lo + hi), fm = delta_r_ref(F, mid);
+        const double mid = 0.5*(lo + hi);
+        // lo and hi adjacent: every further step leaves (lo, hi) as they are, so stopping
+        // here returns what all 160 steps return (~55 steps from a scan bracket)
+        if (mid == lo || mid == hi) break;
+        const double fm = delta_r_ref(F, mid);
         if (!isfinite(fm)) break;
         if ((flo > 0.0 && fm < 0.0) || (flo < 0.0 && fm > 0.0)) { hi = mid; fhi = fm; }
         else { lo = mid; flo = fm; }
@@ -90,153 +94,218 @@ __device__ double bisect_ref(const FrameDev& F, double lo, double hi) {
     return 0.5*(lo + hi);
 }
 
-__global__ __launch_bounds__(kSetupThreads) void kt_setup_kernel(FrameDev* frames, const LaunchDesc* desc) {
-    if ((int)blockIdx.x >= desc->n_frames) return;       // grid sized for a capacity (graphs)
-    FrameDev& F = frames[blockIdx.x];
-    const int t = threadIdx.x;
-    __shared__ double s_rh;
-    __shared__ int s_nc;
+// Per-frame scratch of the setup kernels (device memory beside the frame table).
+struct SetupScratch {
+    int32_t n_cand;                // horizon sign-change candidates found (capped at 64)
+    int32_t ns;                    // ISCO scan segments; -1: sequential fallback ran, -2: closed form
+    int64_t k_end;                 // scan indices [0, k_end): r_k < rmax, plus the first r_k >= rmax
+    unsigned long long best;       // first ISCO scan index (atomicMin)
+    int64_t pad;
+    double cand_r[64];
+    int32_t cand_k[64];
+    ScanSeg seg[kMaxSeg];
+};
+
+constexpr int kHorizonBlocks = 16;     // workgroups per frame for the Delta_r scan
+constexpr int kIscoBlocks = 32;        // workgroups per frame for the L_circ scan
+constexpr int kHorizonN = 20000;       // scan points (knds_metric.hpp:259)
+
+// zero the candidate counters and the ISCO minimum (no memset nodes in graphs)
+__global__ void kt_setup_reset_kernel(SetupScratch* scr, const LaunchDesc* desc) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < desc->n_frames) { scr[f].n_cand = 0; scr[f].ns = 0; scr[f].k_end = 0; scr[f].best = ~0ull; }
+}
+
+// horizon, part 1 (knds_metric.hpp:237-315, Lambda != 0): the 20 000-point log scan of
+// Delta_r over kHorizonBlocks workgroups per frame; each sign change is bisected (160 steps)
+// by the thread that found it, each exact zero recorded, keyed by scan position.
+__global__ __launch_bounds__(kSetupThreads) void kt_setup_horizon_kernel(const FrameDev* frames,
+                                                                         const LaunchDesc* desc, SetupScratch* scr) {
+    const int f = blockIdx.x / kHorizonBlocks, part = blockIdx.x % kHorizonBlocks;
+    if (f >= desc->n_frames) return;                      // grid sized for a capacity (graphs)
+    const FrameDev& F = frames[f];
+    if (fabs(F.Lambda) < 1e-15) return;                   // closed form (setup_mid)
+    SetupScratch& S = scr[f];
+    const double scale = fabs(F.M) > 1.0 ? fabs(F.M) : 1.0;
+    const double rmin = (1e-6*scale > 1e-9) ? 1e-6*scale : 1e-9;
+    double rmax;
+    if (F.Lambda > 0.0) { const double q = 1.25*sqrt(3.0/F.Lambda); rmax = 20.0*scale > q ? 20.0*scale : q; }
+    else rmax = 200.0*scale + 20.0*fabs(F.a) + 20.0*fabs(F.Q);
+    if (!(rmax > rmin)) rmax = rmin * 10.0;
+    const int N = kHorizonN;
+    const double span = log(rmax / rmin);
+    const int threads = kHorizonBlocks * kSetupThreads;
+    const int t = part * kSetupThreads + threadIdx.x;
+    const int per = (N + threads - 1) / threads;
+    const int i0 = 1 + t*per;
+    const int i1 = min(N, i0 + per - 1);
+    if (i0 > N) return;
+    double r0 = (i0 - 1 == 0) ? rmin : rmin * exp(span * (double(i0 - 1) / double(N)));
+    double f0 = delta_r_ref(F, r0);
+    for (int i = i0; i <= i1; ++i) {
+        const double r1 = rmin * exp(span * (double(i) / double(N)));
+        const double f1 = delta_r_ref(F, r1);
+        if (isfinite(f0) && isfinite(f1)) {
+            if (f0 == 0.0) { const int j = atomicAdd(&S.n_cand, 1); if (j < 64) { S.cand_r[j] = r0; S.cand_k[j] = 3*i; } }
+            if (f1 == 0.0) { const int j = atomicAdd(&S.n_cand, 1); if (j < 64) { S.cand_r[j] = r1; S.cand_k[j] = 3*i + 1; } }
+            if ((f0 > 0.0 && f1 < 0.0) || (f0 < 0.0 && f1 > 0.0)) {
+                const double rr = bisect_ref(F, r0, r1);
+                const int j = atomicAdd(&S.n_cand, 1);
+                if (j < 64) { S.cand_r[j] = rr; S.cand_k[j] = 3*i + 2; }
+            }
+        }
+        r0 = r1; f0 = f1;
+    }
+}
+
+// horizon, part 2, and the ISCO scan plan (one workgroup per frame): metric and camera
+// constants; the horizon from the candidates in scan order (roots deduplicated at 1e-8
+// relative; the second largest for Lambda > 0, knds_metric.hpp:296-315); then the ISCO:
+// Bardeen's closed form for Q = Lambda = 0, else the exact segments of "r += 1e-4 M"
+// from 1.05 r_+ (knds_metric.hpp:330-349) and the index range the parallel scan covers.
+__global__ __launch_bounds__(64) void kt_setup_mid_kernel(FrameDev* frames, const LaunchDesc* desc,
+                                                          SetupScratch* scr) {
+    const int f = blockIdx.x;
+    if (f >= desc->n_frames) return;
+    SetupScratch& S = scr[f];
+    // the candidates into LDS (one per lane): the serial work below then has no global
+    // memory round trips on its dependency chain
     __shared__ double s_cr[64];
     __shared__ int s_ck[64];
     __shared__ ScanSeg s_seg[kMaxSeg];
-    __shared__ int s_ns;
-    __shared__ unsigned long long s_best;
-    if (t == 0) { s_nc = 0; frame_knds(F); frame_camera_consts(F); }
+    __shared__ double roots[16];             // LDS, not private: no scratch for this kernel
+    const int nc = S.n_cand < 64 ? S.n_cand : 64;
+    if ((int)threadIdx.x < nc) { s_cr[threadIdx.x] = S.cand_r[threadIdx.x]; s_ck[threadIdx.x] = S.cand_k[threadIdx.x]; }
     __syncthreads();
-
-    // ---- horizon (knds_metric.hpp:237-315) ----
+    if (threadIdx.x != 0) return;
+    FrameDev F = frames[f];                  // a private copy: written back once at the end
+    frame_knds(F);
+    frame_camera_consts(F);
+    double rh;
     if (fabs(F.Lambda) < 1e-15) {
-        if (t == 0) {
-            const double disc = F.M*F.M - F.a*F.a - F.Q*F.Q;
-            s_rh = F.M + sqrt(disc > 0.0 ? disc : 0.0);
-        }
+        const double disc = F.M*F.M - F.a*F.a - F.Q*F.Q;
+        rh = F.M + sqrt(disc > 0.0 ? disc : 0.0);
     } else {
-        const double scale = fabs(F.M) > 1.0 ? fabs(F.M) : 1.0;
-        const double rmin = (1e-6*scale > 1e-9) ? 1e-6*scale : 1e-9;
-        double rmax;
-        if (F.Lambda > 0.0) { const double q = 1.25*sqrt(3.0/F.Lambda); rmax = 20.0*scale > q ? 20.0*scale : q; }
-        else rmax = 200.0*scale + 20.0*fabs(F.a) + 20.0*fabs(F.Q);
-        if (!(rmax > rmin)) rmax = rmin * 10.0;
-        const int N = 20000;
-        const double span = log(rmax / rmin);
-        const int per = (N + kSetupThreads - 1) / kSetupThreads;
-        const int i0 = 1 + t*per;
-        const int i1 = min(N, i0 + per - 1);
-        if (i0 <= N) {
-            double r0 = (i0 - 1 == 0) ? rmin : rmin * exp(span * (double(i0 - 1) / double(N)));
-            double f0 = delta_r_ref(F, r0);
-            for (int i = i0; i <= i1; ++i) {
-                const double r1 = rmin * exp(span * (double(i) / double(N)));
-                const double f1 = delta_r_ref(F, r1);
-                if (isfinite(f0) && isfinite(f1)) {
-                    if (f0 == 0.0) { const int j = atomicAdd(&s_nc, 1); if (j < 64) { s_cr[j] = r0; s_ck[j] = 3*i; } }
-                    if (f1 == 0.0) { const int j = atomicAdd(&s_nc, 1); if (j < 64) { s_cr[j] = r1; s_ck[j] = 3*i + 1; } }
-                    if ((f0 > 0.0 && f1 < 0.0) || (f0 < 0.0 && f1 > 0.0)) {
-                        const double rr = bisect_ref(F, r0, r1);
-                        const int j = atomicAdd(&s_nc, 1);
-                        if (j < 64) { s_cr[j] = rr; s_ck[j] = 3*i + 2; }
-                    }
-                }
-                r0 = r1; f0 = f1;
+        for (int i = 1; i < nc; ++i)     // scan order
+            for (int j = i; j > 0 && s_ck[j-1] > s_ck[j]; --j) {
+                const int tk = s_ck[j]; s_ck[j] = s_ck[j-1]; s_ck[j-1] = tk;
+                const double tr = s_cr[j]; s_cr[j] = s_cr[j-1]; s_cr[j-1] = tr;
             }
+        int nr = 0;
+        for (int c = 0; c < nc; ++c) {
+            const double rr = s_cr[c];
+            if (!isfinite(rr) || rr <= 0.0) continue;
+            bool dup = false;
+            for (int i = 0; i < nr; ++i) {
+                double rel = 1.0;
+                if (fabs(roots[i]) > rel) rel = fabs(roots[i]);
+                if (fabs(rr) > rel) rel = fabs(rr);
+                if (fabs(roots[i] - rr) <= 1e-8 * rel) { dup = true; break; }
+            }
+            if (!dup && nr < 16) roots[nr++] = rr;
         }
-        __syncthreads();
-        if (t == 0) {
-            const int nc = s_nc < 64 ? s_nc : 64;
-            for (int i = 1; i < nc; ++i)     // scan order
-                for (int j = i; j > 0 && s_ck[j-1] > s_ck[j]; --j) {
-                    const int tk = s_ck[j]; s_ck[j] = s_ck[j-1]; s_ck[j-1] = tk;
-                    const double tr = s_cr[j]; s_cr[j] = s_cr[j-1]; s_cr[j-1] = tr;
+        if (nr == 0) {
+            const double disc = F.M*F.M - F.a*F.a - F.Q*F.Q;
+            rh = F.M + sqrt(disc > 0.0 ? disc : 0.0);
+        } else {
+            for (int i = 1; i < nr; ++i)
+                for (int j = i; j > 0 && roots[j-1] > roots[j]; --j) {
+                    const double tr = roots[j]; roots[j] = roots[j-1]; roots[j-1] = tr;
                 }
-            double roots[16];
-            int nr = 0;
-            for (int c = 0; c < nc; ++c) {
-                const double rr = s_cr[c];
-                if (!isfinite(rr) || rr <= 0.0) continue;
-                bool dup = false;
-                for (int i = 0; i < nr; ++i) {
-                    double rel = 1.0;
-                    if (fabs(roots[i]) > rel) rel = fabs(roots[i]);
-                    if (fabs(rr) > rel) rel = fabs(rr);
-                    if (fabs(roots[i] - rr) <= 1e-8 * rel) { dup = true; break; }
-                }
-                if (!dup && nr < 16) roots[nr++] = rr;
-            }
-            double rh;
-            if (nr == 0) {
-                const double disc = F.M*F.M - F.a*F.a - F.Q*F.Q;
-                rh = F.M + sqrt(disc > 0.0 ? disc : 0.0);
-            } else {
-                for (int i = 1; i < nr; ++i)
-                    for (int j = i; j > 0 && roots[j-1] > roots[j]; --j) {
-                        const double tr = roots[j]; roots[j] = roots[j-1]; roots[j-1] = tr;
-                    }
-                rh = (F.Lambda > 0.0 && nr >= 2) ? roots[nr - 2] : roots[nr - 1];
-            }
-            s_rh = rh;
+            rh = (F.Lambda > 0.0 && nr >= 2) ? roots[nr - 2] : roots[nr - 1];
         }
     }
-    __syncthreads();
-    const double rh = s_rh;
-
-    // ---- ISCO (knds_metric.hpp:318-349) ----
+    F.r_h = rh;
+    F.r_out = F.disk_out;
+    F.r_esc = F.r_obs * 1.05;
+    F.r_cut = rh * 1.03;
     double risco = 0.0;
     if (fabs(F.Lambda) < 1e-15 && fabs(F.Q) < 1e-15) {
+        S.ns = -2;
         risco = F.isco_closed;
     } else {
-        const double rmax = 30.0*F.M;
-        if (t == 0) s_ns = isco_segments(rh * 1.05, 1e-4*F.M, rmax, s_seg);
-        __syncthreads();
-        const int ns = s_ns;
-        if (ns < 0) {
-            if (t == 0) s_best = 0;   // fallback: sequential loop in thread 0
-            if (t == 0) {
-                double r = rh * 1.05;
-                const double dr = 1e-4*F.M;
-                double Lp = isco_L(F, r);
-                bool done = false;
-                for (; r < rmax; r += dr) {
-                    const double L = isco_L(F, r + dr);
-                    if (L < Lp && Lp > 0.0) { done = true; break; }
-                    Lp = L;
+        const double rmax = 30.0*F.M, dr = 1e-4*F.M;
+        const int ns = isco_segments(rh * 1.05, dr, rmax, s_seg);
+        S.ns = ns;
+        if (ns >= 0) {
+            // first index whose abscissa reaches rmax (the scan stops there at the latest)
+            int64_t k_end = 0;
+            bool found = false;
+            for (int i = 0; i < ns && !found; ++i) {
+                const ScanSeg& g = s_seg[i];
+                const double u = ldexp(1.0, g.ex);
+                const double rl = (double)(g.m0 + (g.n - 1)*g.d) * u;
+                if (rl >= rmax) {
+                    const double jm = ceil((rmax / u - (double)g.m0) / (double)g.d);
+                    int64_t j = jm < 0.0 ? 0 : (int64_t)jm;
+                    while (j > 0 && (double)(g.m0 + (j - 1)*g.d) * u >= rmax) --j;
+                    while ((double)(g.m0 + j*g.d) * u < rmax) ++j;
+                    k_end = g.k0 + j + 1;
+                    found = true;
                 }
-                (void)done;
-                s_rh = r;   // reuse as the result slot
             }
-            __syncthreads();
-            risco = s_rh;
-        } else {
-            const int per = 32;
-            const int64_t round_len = (int64_t)kSetupThreads * per;
-            for (int64_t base = 0;; base += round_len) {
-                if (t == 0) s_best = ~0ull;
-                __syncthreads();
-                const int64_t k0 = base + (int64_t)t * per;
-                double rk = isco_abscissa(s_seg, ns, k0);
-                double Lk = isco_L(F, rk);
-                for (int j = 0; j < per; ++j) {
-                    const int64_t k = k0 + j;
-                    if (!(rk < rmax)) { atomicMin(&s_best, (unsigned long long)k); break; }
-                    const double rn = isco_abscissa(s_seg, ns, k + 1);
-                    const double Ln = isco_L(F, rn);
-                    if (Ln < Lk && Lk > 0.0) { atomicMin(&s_best, (unsigned long long)k); break; }
-                    rk = rn; Lk = Ln;
-                }
-                __syncthreads();
-                const unsigned long long b = s_best;
-                __syncthreads();
-                if (b != ~0ull) { risco = isco_abscissa(s_seg, ns, (int64_t)b); break; }
-            }
+            // segments end below rmax: the index past them reads as 1e300, where the scan stops
+            if (!found) k_end = s_seg[ns - 1].k0 + s_seg[ns - 1].n + 1;
+            S.k_end = k_end;
+            for (int i = 0; i < ns; ++i) S.seg[i] = s_seg[i];
+            frames[f] = F;
+            return;                                     // the scan kernels finish r_isco
         }
+        // rounding model failed (ties): the reference's sequential loop
+        double r = rh * 1.05;
+        double Lp = isco_L(F, r);
+        for (; r < rmax; r += dr) {
+            const double L = isco_L(F, r + dr);
+            if (L < Lp && Lp > 0.0) break;
+            Lp = L;
+        }
+        risco = r;
     }
-    if (t == 0) {
-        F.r_h = rh;
-        F.r_isco = risco;
-        F.r_in = risco;
-        F.r_out = F.disk_out;
-        F.r_esc = F.r_obs * 1.05;
-        F.r_cut = rh * 1.03;
+    F.r_isco = risco;
+    F.r_in = risco;
+    frames[f] = F;
+}
+
+// ISCO, parallel scan (kIscoBlocks workgroups per frame): the first index k with
+// L(r_{k+1}) < L(r_k) and L(r_k) > 0, or with r_k >= rmax (knds_metric.hpp:335-348).
+// Launched over index windows [k_lo, k_hi) in order; a window after the one holding the
+// answer returns at once.
+__global__ __launch_bounds__(kSetupThreads) void kt_setup_isco_kernel(const FrameDev* frames, const LaunchDesc* desc,
+                                                                      SetupScratch* scr, int64_t k_lo, int64_t k_hi) {
+    const int f = blockIdx.x / kIscoBlocks, part = blockIdx.x % kIscoBlocks;
+    if (f >= desc->n_frames) return;
+    SetupScratch& S = scr[f];
+    const int ns = S.ns;
+    if (ns < 0 || S.best != ~0ull || k_lo >= S.k_end) return;
+    __shared__ ScanSeg s_seg[kMaxSeg];
+    if ((int)threadIdx.x < ns) s_seg[threadIdx.x] = S.seg[threadIdx.x];
+    __syncthreads();
+    const FrameDev& F = frames[f];
+    const double rmax = 30.0*F.M;
+    const int64_t threads = (int64_t)kIscoBlocks * kSetupThreads;
+    const int64_t k_end = S.k_end < k_hi ? S.k_end : k_hi;
+    const int64_t per = (k_end - k_lo + threads - 1) / threads;
+    const int64_t k0 = k_lo + ((int64_t)part * kSetupThreads + threadIdx.x) * per;
+    if (k0 >= k_end) return;
+    double rk = isco_abscissa(s_seg, ns, k0);
+    double Lk = isco_L(F, rk);
+    for (int64_t j = 0; j < per && k0 + j < k_end; ++j) {
+        const int64_t k = k0 + j;
+        if (!(rk < rmax)) { atomicMin(&S.best, (unsigned long long)k); return; }
+        const double rn = isco_abscissa(s_seg, ns, k + 1);
+        const double Ln = isco_L(F, rn);
+        if (Ln < Lk && Lk > 0.0) { atomicMin(&S.best, (unsigned long long)k); return; }
+        rk = rn; Lk = Ln;
     }
+}
+
+// ISCO from the scan's first index (frames whose scan ran)
+__global__ void kt_setup_final_kernel(FrameDev* frames, const LaunchDesc* desc, const SetupScratch* scr) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= desc->n_frames || scr[f].ns < 0) return;
+    const double risco = isco_abscissa(scr[f].seg, scr[f].ns, (int64_t)scr[f].best);
+    frames[f].r_isco = risco;
+    frames[f].r_in = risco;
 }
 
 // ---------------------------------------------------------------------------
@@ -666,6 +735,7 @@ TraceKernelFn pick_kernel(int chart, int method, int prec, bool phi, int occ, bo
 struct FrameTable {
     unsigned char* dev = nullptr;
     unsigned char* host = nullptr;       // hipHostMalloc
+    SetupScratch* scratch = nullptr;     // device only, one per frame slot
     size_t cap = 0;                      // frames
     LaunchDesc* desc() const { return reinterpret_cast<LaunchDesc*>(dev); }
     FrameDev* frames() const { return reinterpret_cast<FrameDev*>(dev + sizeof(LaunchDesc)); }
@@ -777,6 +847,7 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->table.dev);
+    (void)hipFree(c->table.scratch);
     if (c->table.host) (void)hipHostFree(c->table.host);
     (void)hipFree(c->d_geo);
     (void)hipFree(c->d_rgba);
@@ -843,9 +914,12 @@ static int frame_chart(const kt_opts* o, const kt_frame& f) {
 static int table_reserve(kt_hip_ctx* c, FrameTable& t, size_t n) {
     if (n <= t.cap) return KT_OK;
     (void)hipFree(t.dev);
+    (void)hipFree(t.scratch);
     if (t.host) (void)hipHostFree(t.host);
-    t.dev = nullptr; t.host = nullptr; t.cap = 0;
+    t.dev = nullptr; t.host = nullptr; t.scratch = nullptr; t.cap = 0;
     if (hipMalloc(&t.dev, t.bytes(n)) != hipSuccess) return fail(c, KT_E_ALLOC, "frame table allocation failed");
+    if (hipMalloc(&t.scratch, n * sizeof(SetupScratch)) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "setup scratch allocation failed");
     if (hipHostMalloc(&t.host, t.bytes(n), hipHostMallocDefault) != hipSuccess)
         return fail(c, KT_E_ALLOC, "pinned frame staging allocation failed");
     t.cap = n;
@@ -1011,11 +1085,29 @@ static Launch launch_config(const kt_hip_ctx* c, const kt_opts* o, int chart) {
 struct RenderBufs {
     LaunchDesc* desc;
     FrameDev* frames;
+    SetupScratch* scratch;
     uint32_t* geo;
     uint32_t* rgba;
     unsigned long long* words;
     RayInit* init;           // per-pixel initial states (capacity >= pixels)
 };
+
+// Per-frame setup on the context stream for `slots` frame slots (frames beyond the
+// descriptor's count exit at once, so the grid can be sized for a capacity):
+// reset, Delta_r scan, horizon + constants + ISCO plan, L_circ scan, ISCO.
+static void enqueue_setup(kt_hip_ctx* c, FrameDev* frames, const LaunchDesc* desc, SetupScratch* scr, int slots) {
+    const unsigned small = (unsigned)((slots + 63) / 64);
+    hipLaunchKernelGGL(kt_setup_reset_kernel, dim3(small), dim3(64), 0, c->stream, scr, desc);
+    hipLaunchKernelGGL(kt_setup_horizon_kernel, dim3((unsigned)(slots * kHorizonBlocks)), dim3(kSetupThreads), 0,
+                       c->stream, frames, desc, scr);
+    hipLaunchKernelGGL(kt_setup_mid_kernel, dim3((unsigned)slots), dim3(64), 0, c->stream, frames, desc, scr);
+    // windows of 2^15 and 2^17 indices, then the rest (the scan ends at 30 M, < 2^19 indices)
+    const int64_t win[4] = {0, 1 << 15, 1 << 17, (int64_t)1 << 40};
+    for (int w = 0; w < 3; ++w)
+        hipLaunchKernelGGL(kt_setup_isco_kernel, dim3((unsigned)(slots * kIscoBlocks)), dim3(kSetupThreads), 0,
+                           c->stream, frames, desc, scr, win[w], win[w + 1]);
+    hipLaunchKernelGGL(kt_setup_final_kernel, dim3(small), dim3(64), 0, c->stream, frames, desc, scr);
+}
 
 // Enqueue fill + setup + persistent trace (+ shade) on the context stream.  Grid
 // sizes depend only on capacities, so the same sequence can be captured in a graph.
@@ -1045,7 +1137,7 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
 
     hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, B.words, (int)W_COUNT);
     if (ev) KT_HIP(c, hipEventRecord(ev[0], c->stream));
-    hipLaunchKernelGGL(kt_setup_kernel, dim3(setup_grid), dim3(kSetupThreads), 0, c->stream, B.frames, B.desc);
+    enqueue_setup(c, B.frames, B.desc, B.scratch, setup_grid);
     if (!o->bundles) {
         if (chart == KT_CHART_KS)
             hipLaunchKernelGGL(kt_init_kernel<CHART_KS>, dim3((unsigned)(c->n_cu * 8)), dim3(256), 0, c->stream,
@@ -1101,7 +1193,7 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
     const Launch L = force ? *force : launch_config(c, o, chart);
-    RenderBufs B{c->table.desc(), c->table.frames(), c->d_geo, c->d_rgba, c->d_words, c->d_init};
+    RenderBufs B{c->table.desc(), c->table.frames(), c->table.scratch, c->d_geo, c->d_rgba, c->d_words, c->d_init};
     rc = enqueue_render(c, B, o, chart, n_frames, rays, n_frames == 1, shade, L, c->ev);
     if (rc) return rc;
     KT_HIP(c, hipStreamSynchronize(c->stream));
@@ -1159,8 +1251,7 @@ int kt_hip_frame_info(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const
     int chart = 0;
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
-    hipLaunchKernelGGL(kt_setup_kernel, dim3(n_frames), dim3(kSetupThreads), 0, c->stream, c->d_frames,
-                       c->table.desc());
+    enqueue_setup(c, c->d_frames, c->table.desc(), c->table.scratch, n_frames);
     KT_HIP(c, hipGetLastError());
     KT_HIP(c, hipStreamSynchronize(c->stream));
     std::vector<FrameDev> fr(n_frames);
@@ -1473,7 +1564,7 @@ static void plan_drop_graph(kt_hip_plan* p) {
 
 static int plan_capture(kt_hip_plan* p) {
     kt_hip_ctx* c = p->c;
-    RenderBufs B{p->table.desc(), p->table.frames(), p->d_geo, p->d_rgba, p->d_words, p->d_init};
+    RenderBufs B{p->table.desc(), p->table.frames(), p->table.scratch, p->d_geo, p->d_rgba, p->d_words, p->d_init};
     KT_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = KT_OK;
     if (hipMemcpyAsync(p->table.dev, p->table.host, p->table.bytes(p->max_frames), hipMemcpyHostToDevice,
@@ -1654,6 +1745,7 @@ void kt_hip_plan_destroy(kt_hip_plan* p) {
     (void)hipStreamSynchronize(c->stream);
     plan_drop_graph(p);
     (void)hipFree(p->table.dev);
+    (void)hipFree(p->table.scratch);
     if (p->table.host) (void)hipHostFree(p->table.host);
     (void)hipFree(p->d_geo);
     (void)hipFree(p->d_rgba);
@@ -1726,7 +1818,7 @@ int adjoint_prepare(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, kt_opt
     KT_HIP(c, hipSetDevice(c->device));
     rc = upload_frames(c, f, 1, o, rays, pix, chart);
     if (rc) return rc;
-    hipLaunchKernelGGL(kt_setup_kernel, dim3(1), dim3(kSetupThreads), 0, c->stream, c->table.frames(), c->table.desc());
+    enqueue_setup(c, c->table.frames(), c->table.desc(), c->table.scratch, 1);
     KT_HIP(c, hipGetLastError());
     return KT_OK;
 }

@@ -185,7 +185,9 @@ inline double horizon_seq(const FrameDev& F) {
         double flo = delta_r_ref(F, lo), fhi = delta_r_ref(F, hi);
         if (!std::isfinite(flo) || !std::isfinite(fhi)) return std::numeric_limits<double>::quiet_NaN();
         for (int i = 0; i < 160; ++i) {
-            const double mid = 0.5*(lo + hi), fm = delta_r_ref(F, mid);
+            const double mid = 0.5*(lo + hi);
+            if (mid == lo || mid == hi) break;      // fixed point: the remaining steps change nothing
+            const double fm = delta_r_ref(F, mid);
             if (!std::isfinite(fm)) break;
             if ((flo > 0.0 && fm < 0.0) || (flo < 0.0 && fm > 0.0)) { hi = mid; fhi = fm; }
             else { lo = mid; flo = fm; }
