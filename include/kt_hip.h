@@ -88,8 +88,9 @@ typedef struct kt_opts {
     int32_t tuning_blocks_per_cu;         /* persistent grid = blocks_per_cu x CUs (256-thread
                                              blocks); 0: autotuned / built-in default         */
     int32_t tuning_occupancy;             /* waves per SIMD the trace kernel's register budget
-                                             is sized for: 0 built-in default (2; bundles 1),
-                                             2 or 3 (geodesics), 1 or 2 (bundles)               */
+                                             is sized for: 0 autotuned / built-in default
+                                             (binary64 3, mixed 2, bundles 1), 2 or 3
+                                             (geodesics), 1 or 2 (bundles)                      */
     int32_t tuning_chunk;                 /* rays per work-queue grab; 0: 64                   */
     int32_t tuning_pad;
     double mixed_noise;     /* MIXED: binary32 stage-noise coefficient of the island test;

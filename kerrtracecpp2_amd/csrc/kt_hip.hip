@@ -336,7 +336,11 @@ struct TraceArgs {
 
 constexpr int kBlock = 256;
 constexpr int kDefaultBlocksPerCu = 3;
-constexpr int kDefaultOcc = 2;     // waves per SIMD the register budget is sized for
+// waves per SIMD the trace kernel's register budget is sized for: binary64
+// fits 3 (168 VGPRs, a 36 B spill on the cold disk-redshift path) and runs
+// 7-9% faster there; the mixed kernel's binary32 copies of the state need
+// more registers and spill 132 B at 3, so it stays at 2
+__host__ __device__ constexpr int default_occ(int prec) { return prec ? 2 : 3; }
 constexpr int kDefaultChunk = 64;
 
 __device__ __forceinline__ int find_frame(const FrameDev* fr, int n, int64_t g) {
@@ -717,13 +721,14 @@ TraceKernelFn pick_kernel_occ(int chart, int method, int prec, bool phi) {
                              : pick_kernel_chart<CHART_BL, OCC, SINGLE>(method, prec, phi);
 }
 
-// occ: 0 built-in default, 2 or 3 waves per SIMD
+// occ: 0 built-in default for the precision, 2 or 3 waves per SIMD
 TraceKernelFn pick_kernel(int chart, int method, int prec, bool phi, int occ, bool single) {
+    if (occ <= 0) occ = default_occ(prec);
     if (occ == 3)
         return single ? pick_kernel_occ<3, true>(chart, method, prec, phi)
                       : pick_kernel_occ<3, false>(chart, method, prec, phi);
-    return single ? pick_kernel_occ<kDefaultOcc, true>(chart, method, prec, phi)
-                  : pick_kernel_occ<kDefaultOcc, false>(chart, method, prec, phi);
+    return single ? pick_kernel_occ<2, true>(chart, method, prec, phi)
+                  : pick_kernel_occ<2, false>(chart, method, prec, phi);
 }
 
 }  // namespace
@@ -1611,7 +1616,7 @@ static int plan_run(kt_hip_plan* p, const kt_frame* frames, int n_frames, kt_sta
         p->L = launch_config(c, &p->o, chart);
         if (c->tuned[idx[0]][idx[1]][idx[2]][idx[3]] && !explicit_tuning) p->tune = c->tune[idx[0]][idx[1]][idx[2]][idx[3]];
         else { std::memset(&p->tune, 0, sizeof(p->tune)); }
-        p->tune.blocks_per_cu = p->L.bpc; p->tune.chunk = p->L.chunk; p->tune.occupancy = p->L.occ > 0 ? p->L.occ : kDefaultOcc;
+        p->tune.blocks_per_cu = p->L.bpc; p->tune.chunk = p->L.chunk; p->tune.occupancy = p->L.occ > 0 ? p->L.occ : (p->o.bundles ? 1 : default_occ(p->o.precision));
         // the autotuner rendered through the context's own table; refill the plan's staging
         rc = prepare_frames(c, p->table, frames, n_frames, &p->o, &rays, &pix, &chart);
         if (rc) return rc;
@@ -1659,20 +1664,37 @@ int kt_hip_autotune(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const k
     std::memset(&best, 0, sizeof(best));
     best.best_ms = 1e300;
     best.default_ms = -1.0;
+    // register budget: an explicit tuning_occupancy is kept, else the
+    // precision's default is swept with and the other budget tried once at
+    // the best (blocks per CU, chunk)
+    const int occ0 = o->tuning_occupancy > 0 ? o->tuning_occupancy
+                                             : (o->bundles ? 1 : default_occ(o->precision));
+    auto time_launch = [&](const Launch& L, double* ms) -> int {
+        *ms = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            const int r = render_on_device(c, pf.data(), n_frames, o, false, &st, &pix, &L);
+            if (r) return r;
+            *ms = std::min(*ms, st.trace_ms);
+        }
+        ++best.candidates;
+        return KT_OK;
+    };
+    best.occupancy = occ0;
     for (int b : bpcs)
         for (int ch : chunks) {
-            Launch L{b, ch, 0};
-            double ms = 1e300;
-            for (int rep = 0; rep < 2; ++rep) {
-                rc = render_on_device(c, pf.data(), n_frames, o, false, &st, &pix, &L);
-                if (rc) return rc;
-                ms = std::min(ms, st.trace_ms);
-            }
-            ++best.candidates;
+            double ms;
+            rc = time_launch(Launch{b, ch, occ0}, &ms);
+            if (rc) return rc;
             if (b == kDefaultBlocksPerCu && ch == kDefaultChunk) best.default_ms = ms;
             if (ms < best.best_ms) { best.best_ms = ms; best.blocks_per_cu = b; best.chunk = ch; }
         }
-    best.occupancy = kDefaultOcc;
+    if (o->tuning_occupancy <= 0) {
+        const int occ1 = o->bundles ? 2 : (occ0 == 3 ? 2 : 3);
+        double ms;
+        rc = time_launch(Launch{best.blocks_per_cu, best.chunk, occ1}, &ms);
+        if (rc) return rc;
+        if (ms < best.best_ms) { best.best_ms = ms; best.occupancy = occ1; }
+    }
     best.probe_rays = probe;
     int idx[4];
     tune_slot(o, frame_chart(o, pf[0]), idx);

@@ -104,13 +104,15 @@ def test_plan_capacity_and_chart_errors(gpu_ctx):
 def test_autotune_and_configuration_independence(gpu_ctx):
     o = _opts()
     t = gpu_ctx.autotune(LIST_A, o)
-    assert t["candidates"] == 12
-    assert t["blocks_per_cu"] in (2, 3, 4, 6) and t["chunk"] in (32, 64, 128)
+    assert t["candidates"] == 13        # 4 grids x 3 chunks at the default budget, then the other budget
+    assert t["blocks_per_cu"] in (2, 3, 4, 6) and t["chunk"] in (32, 64, 128) and t["occupancy"] in (2, 3)
     assert 0 < t["best_ms"] <= t["default_ms"]
     assert t["probe_rays"] == sum(f.width * f.height for f in LIST_A)     # small lists are their own probe
     base = gpu_ctx.render(LIST_A, o)                                      # uses the tuned configuration
     for bpc, chunk in ((2, 32), (6, 128), (4, 64)):
         _same(gpu_ctx.render(LIST_A, _opts(tuning_blocks_per_cu=bpc, tuning_chunk=chunk)), base)
+    for occ in (2, 3):
+        _same(gpu_ctx.render(LIST_A, _opts(tuning_occupancy=occ)), base)
 
 
 def test_c4_batched_graph_vs_oracle(gpu_ctx):
@@ -127,7 +129,7 @@ def test_c4_batched_graph_vs_oracle(gpu_ctx):
     finally:
         plan.close()
     assert out["stats"]["rays"] == npix
-    assert tune["candidates"] == 12
+    assert tune["candidates"] == 13
     # per-pixel parity on 6 frames, 300 seeded pixels each
     rng = np.random.default_rng(7)
     offs = np.cumsum([0] + [f.width * f.height for f in frames])
