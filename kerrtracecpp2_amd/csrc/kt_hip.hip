@@ -73,7 +73,8 @@ __global__ void kt_fill_kernel(unsigned long long* p, int n) {
 }
 
 // ---------------------------------------------------------------------------
-//  setup: one workgroup (256 threads) per frame
+//  setup: five short kernels per render (reset, horizon, mid, ISCO windows, final),
+//  several workgroups of 256 threads per frame for the scans
 // ---------------------------------------------------------------------------
 constexpr int kSetupThreads = 256;
 
@@ -82,15 +83,16 @@ __device__ double bisect_ref(const FrameDev& F, double lo, double hi) {
     if (!isfinite(flo) || !isfinite(fhi)) return NAN;
     for (int i = 0; i < 160; ++i) {
         const double mid = 0.5*(lo + hi);
-        // lo and hi adjacent: every further step leaves (lo, hi) as they are, so stopping
-        // here returns what all 160 steps return (~55 steps from a scan bracket)
-        if (mid == lo || mid == hi) break;
+        // lo and hi adjacent with Delta_r of strictly opposite signs there: a further step
+        // evaluates Delta_r at lo or hi again and leaves (lo, hi, flo, fhi) as they are, so
+        // stopping returns what all 160 steps return (~55 steps from a scan bracket).  Once
+        // flo is an exact zero the steps still move lo, and the loop runs on.
+        if ((mid == lo || mid == hi) && ((flo > 0.0 && fhi < 0.0) || (flo < 0.0 && fhi > 0.0))) break;
         const double fm = delta_r_ref(F, mid);
         if (!isfinite(fm)) break;
         if ((flo > 0.0 && fm < 0.0) || (flo < 0.0 && fm > 0.0)) { hi = mid; fhi = fm; }
         else { lo = mid; flo = fm; }
     }
-    (void)fhi;
     return 0.5*(lo + hi);
 }
 

@@ -186,7 +186,9 @@ inline double horizon_seq(const FrameDev& F) {
         if (!std::isfinite(flo) || !std::isfinite(fhi)) return std::numeric_limits<double>::quiet_NaN();
         for (int i = 0; i < 160; ++i) {
             const double mid = 0.5*(lo + hi);
-            if (mid == lo || mid == hi) break;      // fixed point: the remaining steps change nothing
+            // fixed point while flo, fhi have strictly opposite signs: the remaining steps
+            // change nothing (with flo an exact zero they still move lo)
+            if ((mid == lo || mid == hi) && ((flo > 0.0 && fhi < 0.0) || (flo < 0.0 && fhi > 0.0))) break;
             const double fm = delta_r_ref(F, mid);
             if (!std::isfinite(fm)) break;
             if ((flo > 0.0 && fm < 0.0) || (flo < 0.0 && fm > 0.0)) { hi = mid; fhi = fm; }
