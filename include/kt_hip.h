@@ -89,10 +89,12 @@ typedef struct kt_opts {
                                              blocks); 0: autotuned / built-in default         */
     int32_t tuning_occupancy;             /* waves per SIMD the trace kernel's register budget
                                              is sized for: 0 autotuned / built-in default
-                                             (binary64 3, mixed 2, bundles 1), 2 or 3
-                                             (geodesics), 1 or 2 (bundles)                      */
+                                             (geodesics 3, bundles 1), 2 or 3 (geodesics),
+                                             1 or 2 (bundles)                                   */
     int32_t tuning_chunk;                 /* rays per work-queue grab; 0: 64                   */
-    int32_t tuning_pad;
+    int32_t tuning_priority_steps;        /* wave issue priority rises one level per this many
+                                             accepted steps of the wave's longest-running ray
+                                             (levels 0-3); 0: 48; negative: off               */
     double mixed_noise;     /* MIXED: binary32 stage-noise coefficient of the island test;
                                0 selects the built-in default                               */
     double mixed_r_island;  /* MIXED: binary64 steps below r < mixed_r_island * r_horizon;

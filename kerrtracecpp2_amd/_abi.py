@@ -36,7 +36,7 @@ class KtOpts(ctypes.Structure):
                 ("step_init", ctypes.c_double), ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
                 ("exposure", ctypes.c_double), ("gamma", ctypes.c_double), ("temp_scale", ctypes.c_double),
                 ("tuning_blocks_per_cu", ctypes.c_int32), ("tuning_occupancy", ctypes.c_int32),
-                ("tuning_chunk", ctypes.c_int32), ("tuning_pad", ctypes.c_int32),
+                ("tuning_chunk", ctypes.c_int32), ("tuning_priority_steps", ctypes.c_int32),
                 ("mixed_noise", ctypes.c_double), ("mixed_r_island", ctypes.c_double),
                 ("palette", ctypes.c_int32), ("disk_rings", ctypes.c_int32), ("disk_sectors", ctypes.c_int32),
                 ("pad2", ctypes.c_int32), ("disk_sigma", ctypes.c_double), ("disk_hue_offset", ctypes.c_double)]

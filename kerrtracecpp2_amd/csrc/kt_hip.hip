@@ -334,16 +334,17 @@ struct TraceArgs {
     unsigned long long* words;
     RunOpts O;
     int32_t chunk;           // rays per queue grab
+    int32_t prio_steps;      // accepted steps per wave-priority level; 0: no priorities
 };
 
 constexpr int kBlock = 256;
 constexpr int kDefaultBlocksPerCu = 3;
-// waves per SIMD the trace kernel's register budget is sized for: binary64
-// fits 3 (168 VGPRs, a 36 B spill on the cold disk-redshift path) and runs
-// 7-9% faster there; the mixed kernel's binary32 copies of the state need
-// more registers and spill 132 B at 3, so it stays at 2
-__host__ __device__ constexpr int default_occ(int prec) { return prec ? 2 : 3; }
+// waves per SIMD the trace kernel's register budget is sized for: 3 (168 VGPRs).
+// Binary64 spills 36 B per lane there (the cold disk-redshift path), mixed 132 B;
+// with wave priorities (below) both run faster at 3 than at 2 (C2: -9 %, -5 %).
+__host__ __device__ constexpr int default_occ(int /*prec*/) { return 3; }
 constexpr int kDefaultChunk = 64;
+constexpr int kPrioSteps = 48;     // default accepted steps per wave-priority level
 
 __device__ __forceinline__ int find_frame(const FrameDev* fr, int n, int64_t g) {
     int lo = 0, hi = n - 1;
@@ -494,6 +495,16 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
         if (!__any(active)) {
             if (exhausted && pool_next >= pool_end) break;
             continue;
+        }
+        // Issue priority by the longest-running ray of the wave: the photon-ring rays (up to
+        // ~6x the mean step count) set the end of the launch, so the arbiter serves their waves
+        // first and the other waves fill the remaining issue slots.
+        if (A.prio_steps > 0) {
+            const int n = active ? Policy::geo(S).n_acc : 0;
+            if (__any(n >= 3 * A.prio_steps)) __builtin_amdgcn_s_setprio(3);
+            else if (__any(n >= 2 * A.prio_steps)) __builtin_amdgcn_s_setprio(2);
+            else if (__any(n >= A.prio_steps)) __builtin_amdgcn_s_setprio(1);
+            else __builtin_amdgcn_s_setprio(0);
         }
         if (active) {
             const FrameDev& F = SINGLE ? F0 : A.frames[frame];
@@ -1054,6 +1065,10 @@ static int upload_frames(kt_hip_ctx* c, const kt_frame* frames, int n_frames, co
     return KT_OK;
 }
 
+static int prio_steps(const kt_opts* o) {
+    return o->tuning_priority_steps == 0 ? kPrioSteps : (o->tuning_priority_steps < 0 ? 0 : o->tuning_priority_steps);
+}
+
 static RunOpts run_opts(const kt_opts* o) {
     RunOpts R;
     R.max_steps = o->max_steps < 1 ? 1 : o->max_steps;
@@ -1141,6 +1156,7 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
     A.words = B.words;
     A.O = run_opts(o);
     A.chunk = L.chunk > 0 ? L.chunk : kDefaultChunk;
+    A.prio_steps = prio_steps(o);
 
     hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, B.words, (int)W_COUNT);
     if (ev) KT_HIP(c, hipEventRecord(ev[0], c->stream));
@@ -1816,6 +1832,7 @@ int adjoint_forward(kt_hip_ctx* c, const kt_opts* o, int chart, int64_t rays, co
     A.words = c->d_words;
     A.O = run_opts(o);
     A.chunk = L.chunk > 0 ? L.chunk : kDefaultChunk;
+    A.prio_steps = prio_steps(o);
     AdjFwdFn fn = chart == KT_CHART_KS ? kt_adjoint_forward_kernel<CHART_KS> : kt_adjoint_forward_kernel<CHART_BL>;
     hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
     KT_HIP(c, hipEventRecord(c->ev[0], c->stream));

@@ -113,6 +113,8 @@ def test_autotune_and_configuration_independence(gpu_ctx):
         _same(gpu_ctx.render(LIST_A, _opts(tuning_blocks_per_cu=bpc, tuning_chunk=chunk)), base)
     for occ in (2, 3):
         _same(gpu_ctx.render(LIST_A, _opts(tuning_occupancy=occ)), base)
+    for ps in (-1, 16, 200):            # wave issue priorities change the schedule, not the rays
+        _same(gpu_ctx.render(LIST_A, _opts(tuning_priority_steps=ps)), base)
 
 
 def test_c4_batched_graph_vs_oracle(gpu_ctx):
