@@ -232,7 +232,9 @@ KT_HD int bundle_step(const FrameDev& F, const RunOpts& O, BundleState& B, RayOu
     const double q0 = th0 - halfpi, q1 = th1 - halfpi;
     const bool near_eq = crosses(q0, q1) || (fmin(fabs(q0), fabs(q1)) < 0.35);
     bool skip = false;
-    if (near_eq) {
+    // without the root-skip quirk a search that cannot land on the disk changes nothing
+    const bool q_skip = (O.quirks & Q_BUNDLE_ROOT_SKIP) != 0;
+    if (near_eq && (q_skip || hermite_may_reach(r0, r1, dr0, S.k1[0], h, F.r_in, F.r_out))) {
         double al = 0.0;
         bool found = false;
         if (hermite_may_touch(th0, th1, dth0, S.k1[1], h, halfpi)) {
@@ -240,7 +242,7 @@ KT_HD int bundle_step(const FrameDev& F, const RunOpts& O, BundleState& B, RayOu
             found = hermite_first_root(th0, th1, dth0, S.k1[1], h, halfpi, al);
         }
         if (!found) {
-            skip = (O.quirks & Q_BUNDLE_ROOT_SKIP) != 0;       // `continue` of ray_bundle.hpp:360-364
+            skip = q_skip;                                      // `continue` of ray_bundle.hpp:360-364
         } else {
             const double rh = hermite(r0, r1, dr0, S.k1[0], h, al);
             if (rh >= F.r_in && rh <= F.r_out) {

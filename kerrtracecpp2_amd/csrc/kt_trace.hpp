@@ -213,6 +213,18 @@ KT_HD bool hermite_may_touch(double y0, double y1, double d0, double d1, double 
     return lo <= margin && hi >= -margin;
 }
 
+// Can the Hermite interpolant of r over the step reach [r_in, r_out]?  The cubic lies in
+// the hull of its Bernstein control points r0, r0 + m0/3, r1 - m1/3, r1 (m = h dr/dl), so a
+// hull clear of the disk's radii (with a margin for rounding) rules out a disk hit at any
+// root the search could return; the search is then skipped with the same outcome.
+KT_HD bool hermite_may_reach(double r0, double r1, double dr0, double dr1, double h, double r_in, double r_out) {
+    const double m0 = h*dr0, m1 = h*dr1;
+    const double b1 = r0 + m0*(1.0/3.0), b2 = r1 - m1*(1.0/3.0);
+    const double lo = fmin(fmin(r0, r1), fmin(b1, b2)), hi = fmax(fmax(r0, r1), fmax(b1, b2));
+    const double margin = 1e-9*(fabs(r0) + fabs(r1) + fabs(m0) + fabs(m1));
+    return hi >= r_in - margin && lo <= r_out + margin;
+}
+
 // first root of the Hermite cubic on [0,1]: 8 bins, 8 bisections, |f| <= 1e-12 short cut
 KT_HD bool hermite_first_root(double y0, double y1, double d0, double d1, double h, double tgt, double& out) {
     const double feps = 1e-12;
@@ -443,7 +455,8 @@ KT_HD bool step_events(const FrameDev& F, double r0, double th0, double ph0, dou
     double d_r = 0.0, d_g = 1.0, d_ph = 0.0;
     const double q0 = th0 - halfpi, q1 = th1 - halfpi;
     const bool near_eq = crosses(q0, q1) || (fmin(fabs(q0), fabs(q1)) < 0.35);
-    if (near_eq && hermite_may_touch(th0, th1, dth0, dth1, hu, halfpi)) {
+    if (near_eq && hermite_may_reach(r0, r1, dr0, dr1, hu, F.r_in, F.r_out) &&
+        hermite_may_touch(th0, th1, dth0, dth1, hu, halfpi)) {
         hermites++;
         double al = 0.0;
         if (hermite_first_root(th0, th1, dth0, dth1, hu, halfpi, al)) {
