@@ -361,16 +361,35 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 // Per-ray state machines driven by the persistent loop.
+// kUnconditional: every lane runs the attempt, an idle lane on a parked state (park())
+// whose result is dropped.  The attempt then is not a divergent region, and the compiler
+// stops copying the whole ray state between registers on every loop trip (~1.5 % mixed);
+// the cost of the idle lanes is the same either way, since a wave issues for all 64.
+// Policies with side effects in advance() keep the divergent form.
 template <int CHART, int METHOD, int PREC, bool PHI>
 struct GeodesicPolicy {
     typedef RayState State;
+    static constexpr bool kUnconditional = true;
     __device__ static void init(const TraceArgs& A, const FrameDev&, int, int, int64_t pix, State& S) {
         ray_load_init(A.init[pix], A.O, S);
     }
+    // A benign state for an idle lane: outside the escape radius, 1 rad from the axis and
+    // the equator, at rest in r and theta.  Each attempt from it ends in the escape test
+    // (no disk search, no wide rotation, no non-finite values), so the lane never pulls
+    // the wave onto a rare path; n_acc stays below the priority steps it votes with.
+    __device__ static void park(State& S) {
+        S.y[0] = S.r_esc > 0.0 ? S.r_esc : 1e3;
+        S.y[1] = 1.0; S.y[2] = 0.0; S.y[3] = 0.0;
+        S.sc.s = 0.8414709848078965; S.sc.c = 0.5403023058681398;   // sin 1, cos 1
+        for (int i = 0; i < 5; ++i) S.k1[i] = 0.0;
+        if (!(S.pt < 0.0)) { S.pt = -1.0; S.pp = 0.0; }
+        S.h = 1.0;
+        S.n_acc = 0; S.n_rej = 0;
+    }
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
-                                   const Knds<float>& Pf, State& S, RayOut& o, double& magnif) {
+                                   const Knds<float>& Pf, State& S, RayOut& o, double& magnif, bool live) {
         bool use64 = true;
-        if (PREC == PREC_MIXED) use64 = WaveVote::any(wants_fp64(F, A.O, S));
+        if (PREC == PREC_MIXED) use64 = WaveVote::any(live && wants_fp64(F, A.O, S));
         magnif = 1.0;
         return ray_advance<CHART, METHOD, PREC, PHI>(F, P, Pf, A.O, use64, S, o);
     }
@@ -379,11 +398,13 @@ struct GeodesicPolicy {
 
 struct BundlePolicy {
     typedef BundleState State;
+    static constexpr bool kUnconditional = false;
+    __device__ static void park(State&) {}
     __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, int64_t, State& B) {
         bundle_init(F, A.O, px, py, B);
     }
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>&, const Knds<float>&,
-                                   State& B, RayOut& o, double& magnif) {
+                                   State& B, RayOut& o, double& magnif, bool) {
         return bundle_step(F, A.O, B, o, magnif) == STEP_DONE;
     }
     __device__ static const RayState& geo(const State& B) { return B.g; }
@@ -395,6 +416,8 @@ struct BundlePolicy {
 template <int CHART>
 struct AdjointForwardPolicy {
     struct State { RayState g; int64_t pix; };
+    static constexpr bool kUnconditional = false;
+    __device__ static void park(State&) {}
     __device__ static void checkpoint(const AdjRecord& R, int64_t slot, const RayState& g) {
         double* c = R.ckpt + 6*slot;
         c[0] = g.y[0]; c[1] = g.y[1]; c[2] = g.y[2]; c[3] = g.y[3]; c[4] = g.sc.s; c[5] = g.sc.c;
@@ -406,7 +429,7 @@ struct AdjointForwardPolicy {
         if (R.hoff) checkpoint(R, R.coff[S.pix], S.g);
     }
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
-                                   const Knds<float>& Pf, State& S, RayOut& o, double& magnif) {
+                                   const Knds<float>& Pf, State& S, RayOut& o, double& magnif, bool) {
         const double h = S.g.h;
         const int n0 = S.g.n_acc;
         magnif = 1.0;
@@ -431,6 +454,10 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     typename Policy::State S;
+    if constexpr (Policy::kUnconditional) {
+        S = typename Policy::State{};
+        Policy::park(S);
+    }
     RayOut o;
     double magnif = 1.0;
     bool active = false;
@@ -506,9 +533,10 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
             else if (__any(n >= A.prio_steps)) __builtin_amdgcn_s_setprio(1);
             else __builtin_amdgcn_s_setprio(0);
         }
-        if (active) {
+        if (Policy::kUnconditional || active) {
             const FrameDev& F = SINGLE ? F0 : A.frames[frame];
-            if (Policy::advance(A, F, P, Pf, S, o, magnif)) {
+            const bool done = Policy::advance(A, F, P, Pf, S, o, magnif, active);
+            if (active && done) {
                 uint32_t wv[7];
                 pack_geopixel_m(o, magnif, wv);
                 uint32_t* dst = A.geo + 7 * pix;
@@ -521,6 +549,7 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
                 c_herm += (uint32_t)G.hermites;
                 c_rays += 1u;
                 active = false;
+                if constexpr (Policy::kUnconditional) Policy::park(S);
             }
         }
     }
