@@ -160,7 +160,8 @@ KT_HD int bundle_step(const FrameDev& F, const RunOpts& O, BundleState& B, RayOu
     // (ray_bundle.hpp:226-237), and the geodesic does not depend on W: A is the plain
     // binary64 RK4 increment, without the deviation matrix.
     double incA[5];
-    rk4_incr<CHART_BL, PREC_FP64>(P, knds_cast<float>(P), true, S.y, S.sc, S.k1, h, S.pt, S.pp, incA, S);
+    rk4_incr<CHART_BL, PREC_FP64>(P, knds_cast<float>(P), true, S.y, S.sc, S.k1, h, S.pt, S.pp, incA);
+    nrhs += 3;
     // the two half steps B1, B2, one per trip (not unrolled); B2 starts from the
     // derivatives at the B1 point
     double yb[4], Wb[4][2], kb[5], dWb[4][2], inc[5], incW[4][2];

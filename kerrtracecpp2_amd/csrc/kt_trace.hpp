@@ -382,33 +382,37 @@ KT_HD void stage_f32(const Knds<float>& Pf, const double y0[4], const double d[4
 
 template <int CHART, int PREC, int NC>
 KT_HD void stage(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4], const double d[4],
-                 const SinCos<double>& sc0, double pt, double pp, double k[NC], RayState& S) {
+                 const SinCos<double>& sc0, double pt, double pp, double k[NC]) {
     if (PREC == PREC_FP64 || use64) {
         SinCos<double> sc;
         stage_f64<CHART, NC>(P, y0, d, sc0, pt, pp, k, sc);
-        S.rhs64++;
     } else {
         stage_f32<CHART, NC>(Pf, y0, d, sc0, pt, pp, k);
-        S.rhs32++;
     }
+}
+
+// right-hand-side evaluations of an attempt, counted once per attempt (all stages of an
+// attempt run in one precision) so that no counter is live through the stages
+template <int PREC>
+KT_HD void count_rhs(RayState& S, bool use64, int n) {
+    if (PREC == PREC_FP64 || use64) S.rhs64 += n;
+    else S.rhs32 += n;
 }
 
 // stage derivative at y0 + d with the sin/cos of theta given (integrated stage values)
 template <int CHART, int PREC, int NC>
 KT_HD void stage_given(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
-                       const double d[4], const SinCos<double>& sc, double pt, double pp, double k[NC], RayState& S) {
+                       const double d[4], const SinCos<double>& sc, double pt, double pp, double k[NC]) {
     if (PREC == PREC_FP64 || use64) {
         double o[5];
         rhs<CHART>(P, y0[0] + d[0], sc, y0[2] + d[2], y0[3] + d[3], pt, pp, o);
         for (int i = 0; i < NC; ++i) k[i] = o[i];
-        S.rhs64++;
     } else {
         const SinCos<float> scf = {(float)sc.s, (float)sc.c};
         float kf[5];
         rhs<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
                    (float)pt, (float)pp, kf);
         for (int i = 0; i < NC; ++i) k[i] = (double)kf[i];
-        S.rhs32++;
     }
 }
 
@@ -594,26 +598,26 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
     if (SCI) {
         sc.s = s1 + h*(C::a21*u1); sc.c = c1 + h*(C::a21*v1);
-        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k2, S);
+        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k2);
         u2 = sc.c*k2[1]; v2 = -sc.s*k2[1];
     } else {
-        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2, S);
+        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2);
     }
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
     if (SCI) {
         sc.s = s1 + h*(C::a31*u1 + C::a32*u2); sc.c = c1 + h*(C::a31*v1 + C::a32*v2);
-        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k3, S);
+        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k3);
         u3 = sc.c*k3[1]; v3 = -sc.s*k3[1];
     } else {
-        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3, S);
+        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3);
     }
     for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
     if (SCI) {
         sc.s = s1 + h*(C::a41*u1 + C::a42*u2 + C::a43*u3); sc.c = c1 + h*(C::a41*v1 + C::a42*v2 + C::a43*v3);
-        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k4, S);
+        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k4);
         u4 = sc.c*k4[1]; v4 = -sc.s*k4[1];
     } else {
-        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4, S);
+        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4);
     }
     {
         double k5[NC];
@@ -622,12 +626,12 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
         if (SCI) {
             sc.s = s1 + h*(C::a51*u1 + C::a52*u2 + C::a53*u3 + C::a54*u4);
             sc.c = c1 + h*(C::a51*v1 + C::a52*v2 + C::a53*v3 + C::a54*v4);
-            stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k5, S);
+            stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k5);
             u5 = sc.c*k5[1]; v5 = -sc.s*k5[1];
             sc.s = s1 + h*(C::a61*u1 + C::a62*u2 + C::a63*u3 + C::a64*u4 + C::a65*u5);
             sc.c = c1 + h*(C::a61*v1 + C::a62*v2 + C::a63*v3 + C::a64*v4 + C::a65*v5);
         } else {
-            stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5, S);
+            stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5);
         }
         for (int i = 0; i < 4; ++i)
             d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
@@ -636,8 +640,8 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     }
     {
         double k6[NC];
-        if (SCI) stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k6, S);
-        else stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6, S);
+        if (SCI) stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k6);
+        else stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6);
         for (int i = 0; i < NC; ++i) a5[i] += C::b6*k6[i];
         for (int i = 0; i < 4; ++i) ae[i] += C::e6*k6[i];
     }
@@ -647,12 +651,11 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     SinCos<double> sc7;
     if (PREC == PREC_FP64 || use64) {
         stage_f64<CHART, 5>(P, y, d, S.sc, S.pt, S.pp, k7, sc7);
-        S.rhs64++;
     } else {
         stage_f32<CHART, 5>(Pf, y, d, S.sc, S.pt, S.pp, k7);
-        S.rhs32++;
         sc7 = sc_at(S.sc, y[1], d[1]);
     }
+    count_rhs<PREC>(S, use64, 6);      // stages 2-6 and the FSAL stage
     // scaled RMS of the embedded error; the accept test needs ~3 digits, so the
     // quotients are formed in binary32 (reciprocal, no IEEE division sequence)
     float e2 = 0.0f;
@@ -689,14 +692,14 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
 template <int CHART, int PREC>
 KT_HD void rk4_incr(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
                     const SinCos<double>& sc0, const double k0[5], double h, double pt, double pp,
-                    double inc[5], RayState& S) {
+                    double inc[5]) {
     double k2[5], k3[5], k4[5], d[4];
     for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k0[i];
-    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k2, S);
+    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k2);
     for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k2[i];
-    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k3, S);
+    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k3);
     for (int i = 0; i < 4; ++i) d[i] = h*k3[i];
-    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k4, S);
+    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k4);
     for (int i = 0; i < 5; ++i) inc[i] = h/6.0*(k0[i] + 2.0*k2[i] + 2.0*k3[i] + k4[i]);
 }
 
@@ -707,17 +710,18 @@ KT_HD int rk4_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     const double h = S.h;
     const double tol = O.rtol;
     double incA[5], incB1[5], incB2[5];
-    rk4_incr<CHART, PREC>(P, Pf, use64, S.y, S.sc, S.k1, h, S.pt, S.pp, incA, S);
-    rk4_incr<CHART, PREC>(P, Pf, use64, S.y, S.sc, S.k1, 0.5*h, S.pt, S.pp, incB1, S);
+    rk4_incr<CHART, PREC>(P, Pf, use64, S.y, S.sc, S.k1, h, S.pt, S.pp, incA);
+    rk4_incr<CHART, PREC>(P, Pf, use64, S.y, S.sc, S.k1, 0.5*h, S.pt, S.pp, incB1);
     double yB1[4];
     for (int i = 0; i < 4; ++i) yB1[i] = S.y[i] + incB1[i];
     const SinCos<double> scB1 = sc_at(S.sc, S.y[1], incB1[1]);
     double kB1[5];
     {
         const double z[4] = {0.0, 0.0, 0.0, 0.0};
-        stage<CHART, PREC, 5>(P, Pf, use64, yB1, z, scB1, S.pt, S.pp, kB1, S);
+        stage<CHART, PREC, 5>(P, Pf, use64, yB1, z, scB1, S.pt, S.pp, kB1);
     }
-    rk4_incr<CHART, PREC>(P, Pf, use64, yB1, scB1, kB1, 0.5*h, S.pt, S.pp, incB2, S);
+    rk4_incr<CHART, PREC>(P, Pf, use64, yB1, scB1, kB1, 0.5*h, S.pt, S.pp, incB2);
+    count_rhs<PREC>(S, use64, 10);     // three RK4 increments and f at the B1 point
     double e2 = 0.0;
     for (int i = 0; i < 4; ++i) { const double t = (S.y[i] + incA[i]) - (yB1[i] + incB2[i]); e2 += t*t; }
     const double err = sqrt(e2) / 15.0;
@@ -737,7 +741,8 @@ KT_HD int rk4_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
         S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
         // f(y_new): event derivatives and first stage of the next attempt
         const double z[4] = {0.0, 0.0, 0.0, 0.0};
-        stage<CHART, PREC, 5>(P, Pf, use64, S.y, z, S.sc, S.pt, S.pp, S.k1, S);
+        stage<CHART, PREC, 5>(P, Pf, use64, S.y, z, S.sc, S.pt, S.pp, S.k1);
+        count_rhs<PREC>(S, use64, 1);
         return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o);
     }
     const double half = h*0.5;
