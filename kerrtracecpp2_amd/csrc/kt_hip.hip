@@ -377,8 +377,8 @@ struct GeodesicPolicy {
     // the equator, at rest in r and theta.  Each attempt from it ends in the escape test
     // (no disk search, no wide rotation, no non-finite values), so the lane never pulls
     // the wave onto a rare path; n_acc stays below the priority steps it votes with.
-    __device__ static void park(State& S) {
-        S.y[0] = S.r_esc > 0.0 ? S.r_esc : 1e3;
+    __device__ static void park(State& S, double r_esc) {
+        S.y[0] = r_esc > 0.0 ? r_esc : 1e3;
         S.y[1] = 1.0; S.y[2] = 0.0; S.y[3] = 0.0;
         S.sc.s = 0.8414709848078965; S.sc.c = 0.5403023058681398;   // sin 1, cos 1
         for (int i = 0; i < 5; ++i) S.k1[i] = 0.0;
@@ -386,8 +386,12 @@ struct GeodesicPolicy {
         S.h = 1.0;
         S.n_acc = 0; S.n_rej = 0;
     }
+    template <bool SINGLE>
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
                                    const Knds<float>& Pf, State& S, RayOut& o, double& magnif, bool live) {
+        // one frame: the event radii are the frame's (wave-uniform) values, so the ray's own
+        // copies need no registers across the loop
+        if (SINGLE) { S.r_cut = F.r_cut; S.r_esc = F.r_esc; }
         bool use64 = true;
         if (PREC == PREC_MIXED) use64 = WaveVote::any(live && wants_fp64(F, A.O, S));
         magnif = 1.0;
@@ -399,10 +403,11 @@ struct GeodesicPolicy {
 struct BundlePolicy {
     typedef BundleState State;
     static constexpr bool kUnconditional = false;
-    __device__ static void park(State&) {}
+    __device__ static void park(State&, double) {}
     __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, int64_t, State& B) {
         bundle_init(F, A.O, px, py, B);
     }
+    template <bool SINGLE>
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>&, const Knds<float>&,
                                    State& B, RayOut& o, double& magnif, bool) {
         return bundle_step(F, A.O, B, o, magnif) == STEP_DONE;
@@ -417,7 +422,7 @@ template <int CHART>
 struct AdjointForwardPolicy {
     struct State { RayState g; int64_t pix; };
     static constexpr bool kUnconditional = false;
-    __device__ static void park(State&) {}
+    __device__ static void park(State&, double) {}
     __device__ static void checkpoint(const AdjRecord& R, int64_t slot, const RayState& g) {
         double* c = R.ckpt + 6*slot;
         c[0] = g.y[0]; c[1] = g.y[1]; c[2] = g.y[2]; c[3] = g.y[3]; c[4] = g.sc.s; c[5] = g.sc.c;
@@ -428,6 +433,7 @@ struct AdjointForwardPolicy {
         const AdjRecord& R = A.rec;
         if (R.hoff) checkpoint(R, R.coff[S.pix], S.g);
     }
+    template <bool SINGLE>
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
                                    const Knds<float>& Pf, State& S, RayOut& o, double& magnif, bool) {
         const double h = S.g.h;
@@ -454,10 +460,6 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     typename Policy::State S;
-    if constexpr (Policy::kUnconditional) {
-        S = typename Policy::State{};
-        Policy::park(S);
-    }
     RayOut o;
     double magnif = 1.0;
     bool active = false;
@@ -468,13 +470,20 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
     // alias the frame table, so reads through A.frames inside the loop are vector loads
     FrameDev F0;
     if (SINGLE) F0 = A.frames[0];
+    if constexpr (Policy::kUnconditional) {
+        S = typename Policy::State{};
+        Policy::park(S, SINGLE ? F0.r_esc : 0.0);
+    }
     Knds<double> P = A.frames[0].P;
     Knds<float> Pf = knds_cast<float>(P);
     const int n_frames = A.desc->n_frames;
     const int64_t total_rays = A.desc->total_rays;
     int64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
-    uint32_t c_acc = 0, c_rej = 0, c_r32 = 0, c_r64 = 0, c_herm = 0, c_rays = 0;
+    // per-lane statistics in LDS: touched once per finished ray, not worth six VGPRs
+    __shared__ uint32_t cnt[6][kBlock];
+    const int t = threadIdx.x;
+    for (int i = 0; i < 6; ++i) cnt[i][t] = 0u;
 
     for (;;) {
         // ---- refill idle lanes from the wave's pool / the global queue ----
@@ -535,26 +544,26 @@ __device__ __forceinline__ void trace_loop(const TraceArgs& A) {
         }
         if (Policy::kUnconditional || active) {
             const FrameDev& F = SINGLE ? F0 : A.frames[frame];
-            const bool done = Policy::advance(A, F, P, Pf, S, o, magnif, active);
+            const bool done = Policy::template advance<SINGLE>(A, F, P, Pf, S, o, magnif, active);
             if (active && done) {
                 uint32_t wv[7];
                 pack_geopixel_m(o, magnif, wv);
                 uint32_t* dst = A.geo + 7 * pix;
                 for (int i = 0; i < 7; ++i) dst[i] = wv[i];
                 const RayState& G = Policy::geo(S);
-                c_acc += (uint32_t)G.n_acc;
-                c_rej += (uint32_t)G.rej_total;
-                c_r32 += (uint32_t)G.rhs32;
-                c_r64 += (uint32_t)G.rhs64;
-                c_herm += (uint32_t)G.hermites;
-                c_rays += 1u;
+                cnt[0][t] += (uint32_t)G.n_acc;
+                cnt[1][t] += (uint32_t)G.rej_total;
+                cnt[2][t] += (uint32_t)G.rhs32;
+                cnt[3][t] += (uint32_t)G.rhs64;
+                cnt[4][t] += (uint32_t)G.hermites;
+                cnt[5][t] += 1u;
                 active = false;
-                if constexpr (Policy::kUnconditional) Policy::park(S);
+                if constexpr (Policy::kUnconditional) Policy::park(S, F.r_esc);
             }
         }
     }
-    const unsigned long long t_acc = wave_sum(c_acc), t_rej = wave_sum(c_rej), t_r32 = wave_sum(c_r32);
-    const unsigned long long t_r64 = wave_sum(c_r64), t_herm = wave_sum(c_herm), t_rays = wave_sum(c_rays);
+    const unsigned long long t_acc = wave_sum(cnt[0][t]), t_rej = wave_sum(cnt[1][t]), t_r32 = wave_sum(cnt[2][t]);
+    const unsigned long long t_r64 = wave_sum(cnt[3][t]), t_herm = wave_sum(cnt[4][t]), t_rays = wave_sum(cnt[5][t]);
     if (lane == 0) {
         atomicAdd(&A.words[W_ACC], t_acc);
         atomicAdd(&A.words[W_REJ], t_rej);

@@ -368,6 +368,18 @@ KT_HD void stage_f64(const Knds<double>& P, const double y0[4], const double d[4
     for (int i = 0; i < NC; ++i) k[i] = o[i];
 }
 
+// The binary32 copies of p_t and p_phi, made opaque at each stage: otherwise the compiler
+// forms the per-ray products (2 p_t, 2 a p_t, a p_phi) once per attempt and keeps them live
+// through all stages, which at 3 waves per SIMD (168 VGPRs) spills them and reloads them in
+// every stage; recomputing them costs three multiplies per stage.
+KT_HD void opaque_f32(float& x, float& y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(x), "+v"(y));
+#else
+    (void)x; (void)y;
+#endif
+}
+
 template <int CHART, int NC>
 KT_HD void stage_f32(const Knds<float>& Pf, const double y0[4], const double d[4], const SinCos<double>& sc0,
                      double pt, double pp, double k[NC]) {
@@ -375,8 +387,10 @@ KT_HD void stage_f32(const Knds<float>& Pf, const double y0[4], const double d[4
     if (fabs(d[1]) <= kRotateMax) scf = rotate(SinCos<float>{(float)sc0.s, (float)sc0.c}, (float)d[1]);
     else kt_sincos((float)(y0[1] + d[1]), &scf.s, &scf.c);
     float kf[5];
+    float ptf = (float)pt, ppf = (float)pp;
+    opaque_f32(ptf, ppf);
     rhs<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
-               (float)pt, (float)pp, kf);
+               ptf, ppf, kf);
     for (int i = 0; i < NC; ++i) k[i] = (double)kf[i];
 }
 
@@ -410,8 +424,10 @@ KT_HD void stage_given(const Knds<double>& P, const Knds<float>& Pf, bool use64,
     } else {
         const SinCos<float> scf = {(float)sc.s, (float)sc.c};
         float kf[5];
+        float ptf = (float)pt, ppf = (float)pp;
+        opaque_f32(ptf, ppf);
         rhs<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
-                   (float)pt, (float)pp, kf);
+                   ptf, ppf, kf);
         for (int i = 0; i < NC; ++i) k[i] = (double)kf[i];
     }
 }
