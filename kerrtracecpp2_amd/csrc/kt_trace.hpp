@@ -540,10 +540,10 @@ KT_HD void renormalise(const Knds<double>& P, RayState& S) {
     const double K = ra*S.pt + P.a*S.pp;
     const double L = P.a*s2*S.pt + S.pp;
     const double pr = S.y[2], pth = S.y[3];
-    const double tL = P.xi2*L*L/(dT*s2);
+    const double tL = P.xi2*L*L*kt_rcp(dT*s2);
     double N, gr, scale;
     if (CHART == CHART_BL) {
-        const double tK = P.xi2*K*K/dR;
+        const double tK = P.xi2*K*K*kt_rcp(dR);
         N = dR*pr*pr + dT*pth*pth - tK + tL;
         gr = 2.0*dR*pr;
         scale = fabs(dR*pr*pr) + dT*pth*pth + fabs(tK) + tL;
@@ -556,7 +556,7 @@ KT_HD void renormalise(const Knds<double>& P, RayState& S) {
     const double gt = 2.0*dT*pth;
     const double g2 = gr*gr + gt*gt;
     if (!(fabs(N) > 1e-12*scale) || !(g2 > 0.0)) return;
-    const double f = N / g2;
+    const double f = N * kt_rcp(g2);
     if (!isfinite(f)) return;
     S.y[2] = pr - f*gr;
     S.y[3] = pth - f*gt;
@@ -567,10 +567,12 @@ enum { STEP_REJECTED = 0, STEP_ACCEPTED = 1, STEP_DONE = 2 };
 // bookkeeping after an accepted step: constraint projection, events, max_steps
 template <int CHART, int PREC>
 KT_HD int after_accept(const FrameDev& F, const Knds<double>& P, const RunOpts& O, RayState& S, double r0,
-                       double th0, double ph0, double dr0, double dth0, double hu, RayOut& o) {
+                       double th0, double ph0, double dr0, double dth0, double hu, RayOut& o, bool step64) {
     S.n_rej = 0;
     S.n_acc++;
-    if (PREC == PREC_MIXED) renormalise<CHART>(P, S);
+    // binary32 steps only: a step the wave took in binary64 drifts off H = 0 no more than
+    // the binary64 path, which projects nothing
+    if (PREC == PREC_MIXED && !step64) renormalise<CHART>(P, S);
     if (step_events<CHART>(F, r0, th0, ph0, dr0, dth0, S, hu, O.quirks, o, S.hermites)) return STEP_DONE;
     if (S.n_acc >= O.max_steps) { stopped_result<CHART>(S, O.quirks, o); return STEP_DONE; }
     return STEP_ACCEPTED;
@@ -695,7 +697,7 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
         double hn = h * ((err > 1e-12) ? 0.9*pow_neg(err, 0.2f) : 4.0);
         if (!isfinite(hn)) hn = h;
         S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
-        return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o);
+        return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o, use64);
     }
     double hn = h * 0.9 * pow_neg(err, 0.25f);
     if (!isfinite(hn)) hn = h*0.5;
@@ -759,7 +761,7 @@ KT_HD int rk4_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
         const double z[4] = {0.0, 0.0, 0.0, 0.0};
         stage<CHART, PREC, 5>(P, Pf, use64, S.y, z, S.sc, S.pt, S.pp, S.k1);
         count_rhs<PREC>(S, use64, 1);
-        return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o);
+        return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o, use64);
     }
     const double half = h*0.5;
     double hn = h * 0.9 * pow_neg(err / tol, 0.25f);
