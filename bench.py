@@ -10,6 +10,10 @@ rays/s; the FP64 path is timed the same way and reported beside it.
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--only mixed|fp64]
     python bench.py --copy-calibration     # known-size device copy, to scale the PMC byte counters
 
+cpu_baseline is the reference's own CPU renderer (oracle/_ref/kerr_tracer_ref, built from the
+reference sources by oracle/Makefile) on the whole frame; cpu_port is the oracle
+restatement on strided rows of the frame itself (also the source of accuracy_vs_oracle).
+
 roofline.traffic (HBM bytes per trace launch) comes from the committed rocprofv3 --pmc summary
 profiles/kt_trace_kernel_c2_1280x720_traffic.txt (FETCH_SIZE + WRITE_SIZE of the trace kernel,
 each scaled by the same counter's reading on the calibration copy); null when that file is absent.
@@ -17,6 +21,8 @@ each scaled by the same counter's reading on the calibration copy); null when th
 import argparse
 import json
 import os
+import re
+import subprocess
 import sys
 import time
 
@@ -167,6 +173,37 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": COPY_BYTES, "launches": 3}))
 
 
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "kerr_tracer_ref")
+REF_SAMPLE = (W, H)
+
+
+def cpu_reference(threads):
+    """The reference's own CPU renderer (main.cpp built from its sources by oracle/Makefile, IEEE
+    flags, OpenMP over pixels) on the whole C2 frame (~14 s on 16 host cores); its "Trace:" time.
+    Run as a child process before this process touches the GPU.  None when the binary is not
+    there."""
+    if not os.access(REF_BIN, os.X_OK):
+        return None
+    w, h = REF_SAMPLE
+    f = FRAME
+    cmd = [REF_BIN, "--custom-res", str(w), str(h), "--a", repr(f["a"]), "--charge", repr(f["Q"]),
+           "--lambda", repr(f["Lambda"]), "--theta", repr(f["theta"]), "--r-obs", repr(f["r_obs"]),
+           "--fov", repr(f["fov"]), "--disk-out", repr(f["disk_out"]), "--dopri5", "--ks", "--tol", repr(RTOL),
+           "--geo-only"]
+    try:
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=os.path.dirname(REF_BIN),
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    m = re.search(r"Trace: ([0-9.]+)s", p.stdout)
+    if p.returncode != 0 or not m:
+        return None
+    el = float(m.group(1))
+    return {"value": w * h / el, "unit": "rays/s", "cores": threads, "kind": "reference",
+            "sample": f"{w * h} rays: the reference CPU renderer (main.cpp, DOPRI5 tol 1e-9, its KS-to-BL "
+                      f"fallback for Lambda != 0) on the whole {w}x{h} frame, {el:.1f} s trace time"}
+
+
 def cpu_baseline(budget_s):
     """The oracle (FP64 CPU restatement, OpenMP over pixels) on this host's cores, on whole rows
     of the same frame taken in a strided order until the time budget is used."""
@@ -212,6 +249,8 @@ def main():
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
               "(rays and frames shard across GPUs without exchange; see the sharding notes)", file=sys.stderr)
         sys.exit(2)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ref_cpu = cpu_reference(threads) if (not a.no_cpu and a.only == "both") else None
     import kerrtracecpp2_amd as kt
     ctx = kt.Context(0)
     if a.only != "both":   # profiling runs: one kernel variant, no JSON contract
@@ -265,7 +304,8 @@ def main():
         result["bundles_c3"] = c3_run(ctx, kt, 2)
     if not a.no_cpu:
         cpu, ref, px, py = cpu_baseline(a.cpu_budget)
-        result["cpu_baseline"] = cpu
+        result["cpu_baseline"] = ref_cpu if ref_cpu is not None else cpu
+        result["cpu_port"] = cpu
         result["accuracy_vs_oracle"] = {"mixed": accuracy(geo_m, ref, px, py, geo_d),
                                         "fp64": accuracy(geo_d, ref, px, py, geo_d)}
     ctx.close()
