@@ -1,6 +1,6 @@
 """Benchmark: rays/s of the KNdS geodesic hot path on one MI355X.
 
-Workload (BASELINE configuration 2): Kerr-Newman-de Sitter a=0.9, Q=0.3, Lambda=1e-5,
+Workload: Kerr-Newman-de Sitter a=0.9, Q=0.3, Lambda=1e-5,
 r_obs=40 M, theta_obs=85 deg, fov 30 deg (CLI default), disk r_isco..12 M, 1280x720,
 ingoing Kerr-Schild chart, DOPRI5 rtol=1e-9 atol=1e-12.  A "step" is one full render of
 the frame (device setup + persistent trace + shading) with its parameters already on the
@@ -179,9 +179,9 @@ REF_SAMPLE = (W, H)
 
 def cpu_reference(threads):
     """The reference's own CPU renderer (main.cpp built from its sources by oracle/Makefile, IEEE
-    flags, OpenMP over pixels) on the whole C2 frame (~14 s on 16 host cores); its "Trace:" time.
-    Run as a child process before this process touches the GPU.  None when the binary is not
-    there."""
+    flags, OpenMP over pixels) on the whole benchmark frame (~14 s on 16 host cores); its "Trace:"
+    time.  Run as a child process before this process touches the GPU.  None when the binary is
+    not there."""
     if not os.access(REF_BIN, os.X_OK):
         return None
     w, h = REF_SAMPLE

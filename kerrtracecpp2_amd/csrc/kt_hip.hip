@@ -340,8 +340,10 @@ struct TraceArgs {
 constexpr int kBlock = 256;
 constexpr int kDefaultBlocksPerCu = 3;
 // waves per SIMD the trace kernel's register budget is sized for: 3 (168 VGPRs).
-// Binary64 spills 36 B per lane there (the cold disk-redshift path), mixed 132 B;
-// with wave priorities (below) both run faster at 3 than at 2 (C2: -9 %, -5 %).
+// Binary64 spills 12 B per lane there, mixed 76 B (stored at entry, reloaded on the cold
+// disk-redshift path);
+// with wave priorities (below) both run faster at 3 than at 2 (1280x720 KS DOPRI5 frame:
+// -9 % binary64, -5 % mixed).
 __host__ __device__ constexpr int default_occ(int /*prec*/) { return 3; }
 constexpr int kDefaultChunk = 64;
 constexpr int kPrioSteps = 48;     // default accepted steps per wave-priority level
