@@ -132,8 +132,9 @@ def c3_run(ctx, kt, reps):
 
 
 def accuracy(geo, ref, px, py, band_geo):
-    """Per-pixel agreement with the oracle outside the photon-ring band (fate boundaries + the
-    steep inner rim, |g - g_neighbour| > 0.05), the band taken from the full FP64 GPU image."""
+    """Per-pixel agreement with the oracle outside the photon-ring band (fate boundaries, the
+    steep inner rim |g - g_neighbour| > 0.05, and the images of the disk's edges), the band taken
+    from the full FP64 GPU image."""
     from tests._util import photon_ring_band
     band = photon_ring_band(band_geo, W, H).reshape(H, W)[py, px]
     sub = geo.reshape(H, W)[py, px]
