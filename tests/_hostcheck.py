@@ -41,6 +41,7 @@ def lib():
         L.ktc_rhs.argtypes = [ctypes.c_int, D, P, P, D, D, P]
         L.ktc_replay_intensity.argtypes = [P, P, P, P, D, D, P, ctypes.c_int, P, P]
         L.ktc_ray_steps.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P, P]
+        L.ktc_event_filters.argtypes = [P, ctypes.c_int64, P]
         _lib = L
     return _lib
 
@@ -159,4 +160,13 @@ def shade_interstellar(geo, M, r_isco, r_in, r_out, rings=7, sectors=14, sigma=0
     lib().ktc_shade_interstellar(_p(geo), geo.size, M, r_isco, r_in, r_out, rings, sectors, sigma, hue, exposure,
                                  gamma, _p(bgp), 0 if bg is None else bg.shape[1], 0 if bg is None else bg.shape[0],
                                  _p(out))
+    return out
+
+
+def event_filters(cases):
+    """cases: (n, 11) th0, th1, dth0, dth1, r0, r1, dr0, dr1, h, r_in, r_out -> (n, 4) int32:
+    may_touch, crossing found, may_reach, crossing on the disk."""
+    c = np.ascontiguousarray(cases, np.float64)
+    out = np.zeros((c.shape[0], 4), np.int32)
+    lib().ktc_event_filters(_p(c), c.shape[0], _p(out))
     return out

@@ -341,3 +341,23 @@ extern "C" int ktc_ray_steps(const kt_frame* f, const kt_opts* o, int px, int py
     else go(std::integral_constant<int, CHART_BL>());
     return 0;
 }
+
+// Event pre-tests against the search they guard.  Per case: th0, th1, dth0, dth1, r0, r1, dr0, dr1,
+// h, r_in, r_out.  out (4 per case): hermite_may_touch on theta; hermite_first_root found a
+// crossing of pi/2; hermite_may_reach on r; the crossing's Hermite r lies in [r_in, r_out].
+extern "C" int ktc_event_filters(const double* in, int64_t n, int32_t* out) {
+    const double halfpi = kPi / 2.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double* c = in + 11*i;
+        const double th0 = c[0], th1 = c[1], dth0 = c[2], dth1 = c[3];
+        const double r0 = c[4], r1 = c[5], dr0 = c[6], dr1 = c[7], h = c[8], r_in = c[9], r_out = c[10];
+        double al = 0.0;
+        const bool found = hermite_first_root(th0, th1, dth0, dth1, h, halfpi, al);
+        const double rh = found ? hermite(r0, r1, dr0, dr1, h, al) : 0.0;
+        out[4*i + 0] = hermite_may_touch(th0, th1, dth0, dth1, h, halfpi) ? 1 : 0;
+        out[4*i + 1] = found ? 1 : 0;
+        out[4*i + 2] = hermite_may_reach(r0, r1, dr0, dr1, h, r_in, r_out) ? 1 : 0;
+        out[4*i + 3] = (found && rh >= r_in && rh <= r_out) ? 1 : 0;
+    }
+    return 0;
+}

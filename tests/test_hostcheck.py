@@ -174,3 +174,48 @@ def test_kernel_sincos_accuracy():
     assert np.array_equal(s[:2], [0.0, -0.0]) and np.signbit(s[1]) and np.array_equal(c[:2], [1.0, 1.0])
     assert s[2] == 1.0 and c[2] == 6.123233995736766e-17        # glibc's cos(M_PI/2)
     assert np.all(np.isnan(s[3:])) and np.all(np.isnan(c[3:]))
+
+
+def test_event_pretests_never_skip_a_crossing():
+    """The two pre-tests that let a step skip the 8-bin / 8-bisection disk search
+    (geodesic.hpp:49-140) may only skip steps on which the search finds nothing that counts:
+    hermite_may_touch is true whenever the search finds a crossing of the equator, and
+    hermite_may_reach is true whenever that crossing's Hermite radius lies on the disk.  Random
+    steps near the equator, plus steps built to graze the plane and the disk's edges."""
+    rng = np.random.default_rng(11)
+    n = 200000
+    hp = np.pi / 2
+    c = np.empty((n, 11))
+    h = rng.uniform(0.01, 5.0, n)
+    c[:, 0] = hp + rng.uniform(-0.4, 0.4, n)
+    c[:, 1] = hp + rng.uniform(-0.4, 0.4, n)
+    c[:, 2] = rng.uniform(-0.5, 0.5, n) / h
+    c[:, 3] = rng.uniform(-0.5, 0.5, n) / h
+    # a quarter graze the plane: endpoints on one side, slopes that bend the cubic back to it
+    g = rng.random(n) < 0.25
+    side = np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    eps = 10.0 ** rng.uniform(-14, -3, n)
+    c[g, 0] = hp + side[g] * eps[g]
+    c[g, 1] = hp + side[g] * rng.uniform(0.0, 0.01, g.sum())
+    c[g, 2] = -side[g] * rng.uniform(0.0, 0.05, g.sum()) / h[g]
+    c[g, 3] = side[g] * rng.uniform(0.0, 0.05, g.sum()) / h[g]
+    c[:, 4] = rng.uniform(1.5, 20.0, n)
+    c[:, 5] = c[:, 4] + rng.uniform(-3.0, 3.0, n)
+    c[:, 6] = rng.uniform(-5.0, 5.0, n) / h
+    c[:, 7] = rng.uniform(-5.0, 5.0, n) / h
+    # a quarter of the radii end near an edge of the disk [3.87, 12]
+    e = rng.random(n) < 0.25
+    edge = np.where(rng.random(n) < 0.5, 3.87, 12.0)
+    c[e, 4] = edge[e] + rng.uniform(-0.05, 0.05, e.sum())
+    c[e, 5] = edge[e] + rng.uniform(-0.05, 0.05, e.sum())
+    c[:, 8] = h
+    c[:, 9] = 3.87
+    c[:, 10] = 12.0
+    out = hc.event_filters(c)
+    touch, found, reach, on_disk = (out[:, k].astype(bool) for k in range(4))
+    assert found.sum() > 1000 and on_disk.sum() > 1000           # the cases exercise both tests
+    assert not (found & ~touch).any(), np.where(found & ~touch)[0][:5]
+    assert not (on_disk & ~reach).any(), np.where(on_disk & ~reach)[0][:5]
+    # and they do skip: most steps without a crossing, and of those with one most off the disk
+    assert (~touch & ~found).sum() > 0.5 * (~found).sum()
+    assert (~reach & found & ~on_disk).sum() > 0.3 * (found & ~on_disk).sum()
