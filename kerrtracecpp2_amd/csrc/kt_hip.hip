@@ -4,10 +4,15 @@
 //  Kernels:
 //    kt_setup_*_kernel  per frame: horizon (Delta_r scan over 16 workgroups, bisection),
 //                       constants, ISCO (exact scan over 32 workgroups)
+//    kt_init_kernel     per pixel: camera ray, chart covector, first RHS -> 128-byte record
 //    kt_trace_kernel    persistent, work-queue driven: each wave64 pulls 64-ray
-//                       chunks (8x8 pixel tiles) from a device counter and refills
-//                       lanes whose ray finished, so uneven step counts do not idle
-//                       the SIMD; per ray: camera -> adaptive steps -> events -> GeoPixel
+//                       chunks (8x8 pixel tiles, centre-out) from a device counter and
+//                       refills lanes whose ray finished, so uneven step counts do not
+//                       idle the SIMD; idle lanes run parked, waves holding the longest-
+//                       running rays get issue priority; per ray: init record ->
+//                       adaptive steps -> events -> GeoPixel
+//    kt_bundle_kernel   the same loop for Jacobi ray bundles
+//    kt_adjoint_*       forward (counting, recording) and reverse passes of the adjoint
 //    kt_shade_kernel    GeoPixel -> ABGR (colorize_buffer)
 //    kt_fill_kernel     zeroes the queue/statistics words (no memset nodes)
 //
