@@ -219,3 +219,29 @@ def test_event_pretests_never_skip_a_crossing():
     # and they do skip: most steps without a crossing, and of those with one most off the disk
     assert (~touch & ~found).sum() > 0.5 * (~found).sum()
     assert (~reach & found & ~on_disk).sum() > 0.3 * (found & ~on_disk).sum()
+
+
+def test_device_rhs_matches_reference_finite_differences():
+    """The kernels' analytic Boyer-Lindquist right-hand side (kt_device.hpp rhs<CHART_BL>) against
+    the reference's (geodesic.hpp:143-166: g^rr p_r, g^thth p_th, and -dH/dr, -dH/dth by central
+    differences with eps_r = 1e-5 max(|r|, 0.01), eps_th = 1e-6), restated in the oracle.  The
+    difference is the reference's truncation and rounding, ~1e-9 relative; a dropped term or a
+    sign error is O(1)."""
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for _ in range(400):
+        a = rng.uniform(-0.95, 0.95)
+        Q = rng.uniform(0.0, 0.3) * np.sqrt(max(0.0, 1.0 - a * a))
+        Lam = rng.choice([0.0, rng.uniform(0.0, 1e-4)])
+        mp = [1.0, a, Q, Lam]
+        rh = kto.r_horizon(mp)
+        r = rng.uniform(1.2 * rh, 40.0)
+        th = rng.uniform(0.15, np.pi - 0.15)
+        pr, pth = rng.uniform(-2.0, 2.0), rng.uniform(-6.0, 6.0)
+        pt, pp = -1.0, rng.uniform(-6.0, 6.0)
+        ref = np.asarray(kto.rhs_bl(mp, [r, th, pr, pth, pt, pp]))[:4]
+        got = hc.rhs(0, 1.0, [a, Q, Lam], [r, th, pr, pth], pt, pp)
+        err = np.abs(got - ref) / (np.abs(ref) + 1e-2)
+        worst = max(worst, float(err.max()))
+        assert np.all(np.abs(got - ref) <= 2e-7 * np.abs(ref) + 2e-9), (mp, r, th, pr, pth, pp, got, ref)
+    print("worst relative difference", worst)
