@@ -42,6 +42,7 @@ def lib():
         L.ktc_replay_intensity.argtypes = [P, P, P, P, D, D, P, ctypes.c_int, P, P]
         L.ktc_ray_steps.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P, P]
         L.ktc_event_filters.argtypes = [P, ctypes.c_int64, P]
+        L.ktc_renormalise.argtypes = [ctypes.c_int, ctypes.c_double, P, P, ctypes.c_int64, P]
         _lib = L
     return _lib
 
@@ -169,4 +170,13 @@ def event_filters(cases):
     c = np.ascontiguousarray(cases, np.float64)
     out = np.zeros((c.shape[0], 4), np.int32)
     lib().ktc_event_filters(_p(c), c.shape[0], _p(out))
+    return out
+
+
+def renormalise(chart, M, par, states):
+    """states: (n, 6) r, theta, p_r, p_theta, p_t, p_phi -> (n, 2) p_r, p_theta after the mixed
+    path's constraint projection."""
+    st = np.ascontiguousarray(states, np.float64)
+    out = np.zeros((st.shape[0], 2))
+    lib().ktc_renormalise(chart, M, _p(np.asarray(par, np.float64)), _p(st), st.shape[0], _p(out))
     return out

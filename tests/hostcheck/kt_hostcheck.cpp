@@ -361,3 +361,27 @@ extern "C" int ktc_event_filters(const double* in, int64_t n, int32_t* out) {
     }
     return 0;
 }
+
+// The mixed path's constraint projection (kt_trace.hpp renormalise) on a BL or KS state.
+// in: 6 per case (r, theta, p_r, p_theta, p_t, p_phi); out: 2 per case (p_r, p_theta after).
+extern "C" int ktc_renormalise(int chart, double M, const double* par3, const double* in, int64_t n, double* out) {
+    kt_frame f{};
+    f.M = M; f.a = par3[0]; f.Q = par3[1]; f.Lambda = par3[2];
+    f.r_obs = 40.0; f.theta_deg = 80.0; f.fov_deg = 30.0; f.disk_out = 12.0; f.width = 8; f.height = 8;
+    kt_opts o{};
+    o.chart = chart; o.integrator = KT_INTEGRATOR_DOPRI5; o.rtol = 1e-9; o.atol = 1e-9; o.max_steps = 1000;
+    o.step_init = 1.0; o.quirks = KT_QUIRKS_REFERENCE;
+    FrameDev F;
+    fill_frame(f, o, F);
+    for (int64_t i = 0; i < n; ++i) {
+        const double* c = in + 6*i;
+        RayState S;
+        S.y[0] = c[0]; S.y[1] = c[1]; S.y[2] = c[2]; S.y[3] = c[3];
+        S.pt = c[4]; S.pp = c[5];
+        kt_sincos(S.y[1], &S.sc.s, &S.sc.c);
+        if (chart == KT_CHART_KS) renormalise<CHART_KS>(F.P, S); else renormalise<CHART_BL>(F.P, S);
+        out[2*i] = S.y[2];
+        out[2*i + 1] = S.y[3];
+    }
+    return 0;
+}

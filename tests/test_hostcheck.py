@@ -245,3 +245,40 @@ def test_device_rhs_matches_reference_finite_differences():
         worst = max(worst, float(err.max()))
         assert np.all(np.abs(got - ref) <= 2e-7 * np.abs(ref) + 2e-9), (mp, r, th, pr, pth, pp, got, ref)
     print("worst relative difference", worst)
+
+
+def test_constraint_projection_restores_null_condition():
+    """The mixed path's binary64 projection onto H = 0 (kt_trace.hpp renormalise, BL), judged by
+    the reference's own Hamiltonian and right-hand side (knds_metric.hpp:122-132,
+    geodesic.hpp:143-166, restated in the oracle): from camera states of random spacetimes pushed
+    off the shell by 1e-8..1e-5 in (p_r, p_theta), one projection cuts |H| by at least 1e4; the
+    correction is parallel to grad_p H = (g^rr p_r, g^thth p_theta); a state on the shell is
+    left as it is."""
+    rng = np.random.default_rng(9)
+    n_checked = 0
+    for _ in range(60):
+        a = rng.uniform(-0.9, 0.9)
+        Q = rng.uniform(0.0, 0.3) * np.sqrt(1 - a * a)
+        Lam = float(rng.choice([0.0, rng.uniform(0.0, 1e-5)]))
+        mp = [1.0, a, Q, Lam]
+        cam = [rng.uniform(20, 60), rng.uniform(30, 85), 0.0, 30.0, 64, 36]
+        s = kto.camera_pixel(mp, cam, float(rng.integers(0, 64)), float(rng.integers(0, 36)))
+        r, th, pr, pth, pt, pp = s[0], s[1], s[3], s[4], s[5], s[6]
+        H0 = kto.hamiltonian_bl(mp, [r, th, pr, pth, pt, pp])
+        # on the shell: unchanged
+        same = hc.renormalise(0, 1.0, [a, Q, Lam], [[r, th, pr, pth, pt, pp]])[0]
+        assert abs(H0) < 1e-10 and same[0] == pr and same[1] == pth
+        eps = 10.0 ** rng.uniform(-8, -5)
+        pr1, pth1 = pr * (1 + eps * rng.uniform(-1, 1)), pth * (1 + eps * rng.uniform(-1, 1)) + eps
+        H1 = kto.hamiltonian_bl(mp, [r, th, pr1, pth1, pt, pp])
+        if abs(H1) < 1e-11:
+            continue
+        out = hc.renormalise(0, 1.0, [a, Q, Lam], [[r, th, pr1, pth1, pt, pp]])[0]
+        H2 = kto.hamiltonian_bl(mp, [r, th, out[0], out[1], pt, pp])
+        assert abs(H2) <= 1e-4 * abs(H1) + 1e-15, (mp, cam, H1, H2)
+        g = np.asarray(kto.rhs_bl(mp, [r, th, pr1, pth1, pt, pp]))[:2]      # g^rr p_r, g^thth p_theta
+        dp = np.array([out[0] - pr1, out[1] - pth1])
+        cosang = abs(dp @ g) / (np.linalg.norm(dp) * np.linalg.norm(g))
+        assert cosang > 1 - 1e-6, (mp, cam, cosang)
+        n_checked += 1
+    assert n_checked >= 50
