@@ -82,3 +82,20 @@ def test_random_config_rk4_vs_oracle(gpu_ctx, i):
         if d.any():
             assert np.abs(ref["redshift"][d] - g["redshift"][d]).max() <= 5e-4, what
             assert np.abs(ref["r"][d] - g["r"][d]).max() <= 2e-3, what
+
+
+def test_random_configs_in_one_plan_equal_single_renders(gpu_ctx):
+    """All 16 spacetimes in one graph-captured plan (the multi-frame kernel, per-lane frame
+    constants, one setup pass for 16 horizons and ISCOs) against one-frame renders: binary64
+    GeoPixels equal bit for bit, twice in a row."""
+    frames = [kt.make_frame(W, H, **cfg) for cfg in CONFIGS]
+    o = _opts(kt.KT_CHART_KS, kt.KT_PRECISION_FP64)
+    singles = [gpu_ctx.render([f], o, want_rgba=False)["geo"] for f in frames]
+    plan = gpu_ctx.plan(o, len(frames), W * H * len(frames))
+    try:
+        for _ in range(2):
+            geo = plan.render(frames, want_rgba=False)["geo"]
+            for k, g in enumerate(singles):
+                assert geo[k * W * H:(k + 1) * W * H].tobytes() == g.tobytes(), f"frame {k}"
+    finally:
+        plan.close()
