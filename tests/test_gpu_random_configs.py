@@ -1,8 +1,8 @@
 """Seeded random spacetimes and cameras against the oracle, beyond the fixed configurations:
 spin in (-0.95, 0.95), charge up to 0.4 sqrt(1 - a^2), Lambda 0 or up to 1e-5, inclination 20-89
 deg, distance 20-80 M, disk edge 8-20 M; 96x54 pixels, DOPRI5 at tol 1e-9 (atol = rtol, the
-reference's semantics), Boyer-Lindquist and Kerr-Schild charts, binary64 and mixed; and RK4 step
-doubling at tol 1e-7 in BL.
+reference's semantics), Boyer-Lindquist and Kerr-Schild charts, binary64 and mixed; RK4 step
+doubling at tol 1e-7 in BL, and in KS where Lambda = 0; and all 16 in one batched plan.
 
 The oracle runs what the reference runs: BL DOPRI5; for KS at Lambda = 0 its own KS path (the
 initial covector without the shift terms, main.cpp:1373-1430, and RK4, main.cpp:1539-1545), which
@@ -99,3 +99,27 @@ def test_random_configs_in_one_plan_equal_single_renders(gpu_ctx):
                 assert geo[k * W * H:(k + 1) * W * H].tobytes() == g.tobytes(), f"frame {k}"
     finally:
         plan.close()
+
+
+KS_CASES = [i for i, c in enumerate(CONFIGS) if c["Lambda"] == 0.0]
+
+
+@pytest.mark.parametrize("i", KS_CASES)
+def test_random_config_ks_rk4_vs_oracle(gpu_ctx, i):
+    """RK4 step doubling at tol 1e-7 in the Kerr-Schild chart, for the spacetimes with Lambda = 0
+    (where the reference runs its KS path, main.cpp:1539-1545), against the oracle's KS path:
+    the tol-1e-7 bars."""
+    cfg = CONFIGS[i]
+    ref = kto.trace_frame(kto.make_frame(W, H, chart="ks", integrator="rk4", tol=1e-7, **cfg))
+    keep = ~photon_ring_band(ref, W, H)
+    for prec in (kt.KT_PRECISION_FP64, kt.KT_PRECISION_MIXED):
+        o = kt.default_opts()
+        o.chart, o.integrator, o.precision = kt.KT_CHART_KS, kt.KT_INTEGRATOR_RK4, prec
+        o.rtol = o.atol = 1e-7
+        g = gpu_ctx.render([kt.make_frame(W, H, **cfg)], o, want_rgba=False)["geo"]
+        what = f"config {i} {cfg} precision {prec}"
+        assert (ref["outcome"] == g["outcome"])[keep].mean() >= 0.999, what
+        d = keep & (ref["outcome"] == 1) & (g["outcome"] == 1)
+        if d.any():
+            assert np.abs(ref["redshift"][d] - g["redshift"][d]).max() <= 5e-4, what
+            assert np.abs(ref["r"][d] - g["r"][d]).max() <= 2e-3, what
