@@ -38,8 +38,11 @@ extern "C" {
 #define KT_CHART_KS 1          /* ingoing Kerr-Schild; any Lambda (reference: Lambda == 0 only)  */
 #define KT_INTEGRATOR_RK4 0    /* adaptive RK4 step doubling  (geodesic.hpp:204-257)             */
 #define KT_INTEGRATOR_DOPRI5 1 /* Dormand-Prince 5(4) with FSAL (geodesic.hpp:312-414)           */
-#define KT_PRECISION_FP64 0    /* every operation in binary64                                     */
-#define KT_PRECISION_MIXED 1   /* binary32 stage derivatives, binary64 state/error control/islands */
+#define KT_PRECISION_FP64 0    /* the integration in binary64; only the step-size factor err^(-q)
+                                  takes its log2/exp2 in binary32 (the accept test is binary64)  */
+#define KT_PRECISION_MIXED 1   /* binary32 stage derivatives; binary64 state, accept test, constraint
+                                  projection, and the steps near the horizon, the pole and turning
+                                  points.  Jacobi bundles: binary64 only (KT_E_UNSUPPORTED)       */
 #define KT_PALETTE_BLACKBODY 0     /* disk_colour (main.cpp:1799-1814)                              */
 #define KT_PALETTE_INTERSTELLAR 1  /* disk_colour_interstellar (main.cpp:1689-1797)                 */
 
@@ -83,7 +86,8 @@ typedef struct kt_opts {
     uint32_t quirks;        /* KT_QUIRK_* mask                                              */
     double step_init;       /* initial affine step (main.cpp:195)                            */
     double rtol, atol;      /* RK4: err < rtol (absolute, geodesic.hpp:234);
-                               DOPRI5: scale = atol + rtol*|y0| (reference: atol = rtol = tol) */
+                               DOPRI5: scale = atol + rtol*|y0| (reference: atol = rtol = tol);
+                               atol below 1.1754943508222875e-38 (FLT_MIN) is raised to it   */
     double exposure, gamma, temp_scale;   /* ColorParams (main.cpp:72-81)                   */
     int32_t tuning_blocks_per_cu;         /* persistent grid = blocks_per_cu x CUs (256-thread
                                              blocks); 0: autotuned / built-in default         */
@@ -103,7 +107,9 @@ typedef struct kt_opts {
     int32_t disk_rings;     /* Interstellar palette: rings x sectors cells (main.cpp:76-77)   */
     int32_t disk_sectors;
     int32_t pad2;
-    double disk_sigma;      /* Interstellar Gaussian blend width in cells (main.cpp:78)       */
+    double disk_sigma;      /* Interstellar Gaussian blend width in cells (main.cpp:78).  With
+                               s = ceil(3 sigma) + 1, (rings + 2s + 1)(sectors + 2s + 1) must
+                               stay within 65536 cells per pixel, else KT_E_ARG            */
     double disk_hue_offset; /* Interstellar hue offset (main.cpp:79)                          */
 } kt_opts;
 
@@ -148,6 +154,10 @@ typedef struct kt_hip_plan kt_hip_plan;
 /* Library / context life cycle. */
 int  kt_hip_abi_version(void);
 void kt_hip_default_opts(kt_opts* opts);  /* reference defaults: BL, RK4, tol 1e-7, step 1, 500000 */
+/* Validates an option set without a device (the checks every render makes first; main.cpp's
+   argument handling, :2548-2684, has none): KT_OK or the KT_E_* status a render would return,
+   with its message copied to msg (nullable, NUL-terminated within msg_len). */
+int  kt_check_opts(const kt_opts* opts, char* msg, size_t msg_len);
 int  kt_hip_create(int device, kt_hip_ctx** out);
 void kt_hip_destroy(kt_hip_ctx* ctx);
 const char* kt_hip_last_error(const kt_hip_ctx* ctx);

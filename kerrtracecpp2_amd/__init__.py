@@ -23,7 +23,9 @@ _lib = None
 
 
 class KtError(RuntimeError):
-    pass
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
 
 
 def load_library():
@@ -65,7 +67,7 @@ class Context:
     def _check(self, rc, what):
         if rc != 0:
             msg = self.lib.kt_hip_last_error(self.h)
-            raise KtError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+            raise KtError(f"{what} failed ({rc}): {msg.decode() if msg else ''}", rc)
 
     def device_info(self):
         name = ctypes.create_string_buffer(256)
@@ -230,6 +232,15 @@ class Plan:
 
 def _info_dict(i):
     return {k: getattr(i, k) for k, _ in i._fields_}
+
+
+def check_opts(opts):
+    """Validate an option set without a device: raises KtError with the status and message a
+    render would return for it."""
+    msg = ctypes.create_string_buffer(512)
+    rc = load_library().kt_check_opts(ctypes.byref(opts), msg, len(msg))
+    if rc != 0:
+        raise KtError(f"kt_check_opts: {msg.value.decode()} (status {rc})", rc)
 
 
 def checker_background(seed=1234, width=512, height=256, cells_x=32, cells_y=16):

@@ -91,6 +91,7 @@ _P, _I, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
 PROTOTYPES = {
     "kt_hip_abi_version": (ctypes.c_int, []),
     "kt_hip_default_opts": (None, [_P]),
+    "kt_check_opts": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_size_t]),
     "kt_hip_create": (ctypes.c_int, [_I, _P]),
     "kt_hip_destroy": (None, [_P]),
     "kt_hip_last_error": (ctypes.c_char_p, [_P]),

@@ -211,6 +211,7 @@ int main(int argc, char** argv) {
             "and Hermite disk intersection)");
     }
     if (precision != "fp64" && precision != "mixed") die("--precision must be fp64 or mixed");
+    if (use_bundles && precision == "mixed" && !gpu_fp64) die("--bundles runs in binary64 only (--precision fp64)");
     max_steps = std::max(1, max_steps);
     step_init = std::max(1e-10, step_init);
     tol = std::max(1e-10, tol);

@@ -380,16 +380,17 @@ struct GeodesicPolicy {
     __device__ static void init(const TraceArgs& A, const FrameDev&, int, int, int64_t pix, State& S) {
         ray_load_init(A.init[pix], A.O, S);
     }
-    // A benign state for an idle lane: outside the escape radius, 1 rad from the axis and
-    // the equator, at rest in r and theta.  Each attempt from it ends in the escape test
-    // (no disk search, no wide rotation, no non-finite values), so the lane never pulls
-    // the wave onto a rare path; n_acc stays below the priority steps it votes with.
+    // A benign state for an idle lane: at the escape radius, 1 rad from the axis and the
+    // equator, with every momentum zero.  The Hamiltonian is quadratic in the momenta, so
+    // the right-hand side vanishes there: the state is a fixed point of every attempt (error
+    // 0, accepted, ends in the escape test), never drifts into the horizon or the disk
+    // search, and never pulls the wave onto a rare path.  Its result is dropped.
     __device__ static void park(State& S, double r_esc) {
         S.y[0] = r_esc > 0.0 ? r_esc : 1e3;
         S.y[1] = 1.0; S.y[2] = 0.0; S.y[3] = 0.0;
         S.sc.s = 0.8414709848078965; S.sc.c = 0.5403023058681398;   // sin 1, cos 1
         for (int i = 0; i < 5; ++i) S.k1[i] = 0.0;
-        if (!(S.pt < 0.0)) { S.pt = -1.0; S.pp = 0.0; }
+        S.pt = 0.0; S.pp = 0.0;
         S.h = 1.0;
         S.n_acc = 0; S.n_rej = 0;
     }
@@ -875,6 +876,18 @@ void kt_hip_default_opts(kt_opts* o) {
     o->disk_hue_offset = 0.0;
 }
 
+static int check_opts(kt_hip_ctx* c, const kt_opts* o);
+
+int kt_check_opts(const kt_opts* o, char* msg, size_t msg_len) {
+    kt_hip_ctx tmp;            // message holder only: no device call is made
+    const int rc = check_opts(&tmp, o);
+    if (msg && msg_len) {
+        std::strncpy(msg, rc ? tmp.err.c_str() : "", msg_len - 1);
+        msg[msg_len - 1] = 0;
+    }
+    return rc;
+}
+
 int kt_hip_create(int device, kt_hip_ctx** out) {
     if (!out) return KT_E_ARG;
     *out = nullptr;
@@ -953,6 +966,28 @@ int kt_hip_set_background(kt_hip_ctx* c, const kt_background* bg) {
     return KT_OK;
 }
 
+// The Interstellar palette loops over (rings + 2s + 1)(sectors + 2s + 1) cells per pixel,
+// s = ceil(3 sigma) + 1 (main.cpp:1717-1797): bounded here so that no argument turns the
+// shade kernel into an effectively endless launch.
+static constexpr int64_t kMaxPaletteCells = 1 << 16;
+
+static int check_palette(kt_hip_ctx* c, const kt_opts* o) {
+    if (o->palette != KT_PALETTE_BLACKBODY && o->palette != KT_PALETTE_INTERSTELLAR)
+        return fail(c, KT_E_ARG, "bad palette %d", o->palette);
+    if (o->palette != KT_PALETTE_INTERSTELLAR) return KT_OK;
+    if (!std::isfinite(o->disk_sigma) || o->disk_sigma > 1e4)
+        return fail(c, KT_E_ARG, "disk_sigma %g out of range", o->disk_sigma);
+    if (o->disk_rings > 65536 || o->disk_sectors > 65536)
+        return fail(c, KT_E_ARG, "disk_rings/disk_sectors above 65536");
+    const int64_t nr = o->disk_rings > 0 ? o->disk_rings : 1, ns = o->disk_sectors > 0 ? o->disk_sectors : 1;
+    const double sg = o->disk_sigma > 0.0 ? o->disk_sigma : 0.5;
+    const int64_t srch = (int64_t)std::ceil(3.0 * sg) + 1;
+    if ((nr + 2 * srch + 1) * (ns + 2 * srch + 1) > kMaxPaletteCells)
+        return fail(c, KT_E_ARG, "Interstellar palette of %lld x %lld cells with sigma %g exceeds %lld cells per pixel",
+                    (long long)nr, (long long)ns, sg, (long long)kMaxPaletteCells);
+    return KT_OK;
+}
+
 static int check_opts(kt_hip_ctx* c, const kt_opts* o) {
     if (!o) return fail(c, KT_E_ARG, "opts is NULL");
     if (o->chart != KT_CHART_BL && o->chart != KT_CHART_KS) return fail(c, KT_E_ARG, "bad chart %d", o->chart);
@@ -961,7 +996,11 @@ static int check_opts(kt_hip_ctx* c, const kt_opts* o) {
     if (o->precision != KT_PRECISION_FP64 && o->precision != KT_PRECISION_MIXED)
         return fail(c, KT_E_ARG, "bad precision %d", o->precision);
     if (!(o->rtol > 0.0) || !(o->atol > 0.0)) return fail(c, KT_E_ARG, "tolerances must be positive");
-    return KT_OK;
+    // Jacobi bundles are binary64 only: a binary32 deviation matrix cannot hold RK4 doubling's
+    // tolerance at these step sizes and would cost more instructions, not fewer, on this chip
+    if (o->bundles && o->precision != KT_PRECISION_FP64)
+        return fail(c, KT_E_UNSUPPORTED, "ray bundles run in binary64 only: use KT_PRECISION_FP64");
+    return check_palette(c, o);
 }
 
 // Effective chart of a frame: the reference runs KS only for Lambda == 0
@@ -1120,7 +1159,7 @@ static RunOpts run_opts(const kt_opts* o) {
     R.quirks = o->quirks;
     R.step_init = o->step_init < 1e-10 ? 1e-10 : o->step_init;
     R.rtol = o->rtol < 1e-10 && o->integrator == KT_INTEGRATOR_RK4 ? 1e-10 : o->rtol;
-    R.atol = o->atol;
+    R.atol = o->atol < 1.1754943508222875e-38 ? 1.1754943508222875e-38 : o->atol;   // FLT_MIN
     R.isl_noise = o->mixed_noise > 0.0 ? o->mixed_noise : kMixedNoiseDefault;
     R.isl_r = o->mixed_r_island > 0.0 ? o->mixed_r_island : kMixedRIslandDefault;
     return R;
@@ -1339,6 +1378,8 @@ int kt_hip_frame_info(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const
 int kt_hip_colorize(kt_hip_ctx* c, const kt_geopixel* geo, int64_t n, double M, double r_isco, double r_in,
                     double r_out, const kt_opts* o, uint32_t* rgba_out) {
     if (!c || !geo || !rgba_out || n <= 0 || !o) return KT_E_ARG;
+    const int prc = check_palette(c, o);
+    if (prc) return prc;
     KT_HIP(c, hipSetDevice(c->device));
     if ((size_t)n > c->cap_pix) {
         (void)(void)hipFree(c->d_geo); (void)hipFree(c->d_rgba);

@@ -220,8 +220,9 @@ inline bool decode_ppm(const std::vector<uint8_t>& b, std::vector<uint8_t>& rgb,
         return true;
     };
     long w = 0, h = 0, mx = 0;
+    // width and height are each below 2^30 after token(); capped at 65536 before the product
     if (b.size() < 2 || b[0] != 'P' || b[1] != '6' || !token(w) || !token(h) || !token(mx) || mx != 255 || w <= 0 ||
-        h <= 0 || w * h > (1L << 28)) {
+        h <= 0 || w > 65536 || h > 65536 || w * h > (1L << 28)) {
         if (err) *err = "unsupported PPM (need binary P6, maxval 255)";
         return false;
     }

@@ -432,13 +432,19 @@ KT_HD void stage_given(const Knds<double>& P, const Knds<float>& Pf, bool use64,
     }
 }
 
-// step-size factor err^(-q) in binary32 (the controller needs ~3 digits)
-KT_HD double pow_neg(double err, float q) {
+// Step-size factor x^(-q) for a finite x > 0 of any binary64 magnitude: the binary exponent
+// is split off exactly (x = m 2^e, m in [0.5, 1)), log2 of the mantissa and the final exp2
+// are binary32 (~1e-7 relative).  The factor only scales the next trial step; the accept
+// test itself is binary64.
+KT_HD double pow_neg(double x, float q) {
+    int e;
+    const double m = frexp(x, &e);
 #if defined(__HIP_DEVICE_COMPILE__)
-    const float e = (float)err;
-    return (double)__builtin_amdgcn_exp2f(-q * __builtin_amdgcn_logf(e));
+    const float l = (float)e + __builtin_amdgcn_logf((float)m);
+    return (double)__builtin_amdgcn_exp2f(-q * l);
 #else
-    return (double)std::exp2(-q * std::log2((float)err));
+    const float l = (float)e + std::log2((float)m);
+    return (double)std::exp2(-q * l);
 #endif
 }
 
@@ -674,32 +680,34 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
         sc7 = sc_at(S.sc, y[1], d[1]);
     }
     count_rhs<PREC>(S, use64, 6);      // stages 2-6 and the FSAL stage
-    // scaled RMS of the embedded error; the accept test needs ~3 digits, so the
-    // quotients are formed in binary32 (reciprocal, no IEEE division sequence)
-    float e2 = 0.0f;
+    // Scaled RMS of the embedded error, err^2 = mean_i (dy_i / (atol + rtol |y_i|))^2, in
+    // binary64 in both precisions: each quotient through the refined reciprocal, and the
+    // accept test err <= 1 taken on err^2 (no square root; sqrt is monotone, so the test is
+    // the same).
+    double e2 = 0.0;
     for (int i = 0; i < 4; ++i) {
-        const float num = (float)(h*(ae[i] + C::e7*k7[i]));
-        const float q = num * kt_rcp((float)(O.atol + O.rtol*fabs(y[i])));
-        e2 = fmaf(q, q, e2);
+        const double q = (h*(ae[i] + C::e7*k7[i])) * kt_rcp(O.atol + O.rtol*fabs(y[i]));
+        e2 = fma(q, q, e2);
     }
-    const double err = (double)sqrtf(e2 * 0.25f);
-    if (!isfinite(err)) {
+    const double err2 = e2 * 0.25;
+    if (!isfinite(err2)) {
         const double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
         S.h = hn < 1e-10 ? 1e-10 : hn;
         return after_reject(O, S);
     }
-    if (err <= 1.0 || h < 1e-10) {
+    if (err2 <= 1.0 || h < 1e-10) {
         const double r0 = y[0], th0 = y[1], ph0 = S.ph, dr0 = k1[0], dth0 = k1[1];
         if (NC == 5) S.ph += h*a5[NC - 1];
         for (int i = 0; i < 4; ++i) S.y[i] += d[i];
         for (int i = 0; i < 5; ++i) S.k1[i] = k7[i];
         S.sc = sc7;
-        double hn = h * ((err > 1e-12) ? 0.9*pow_neg(err, 0.2f) : 4.0);
+        // err > 1e-12 <=> err^2 > 1e-24; err^(-1/5) = (err^2)^(-1/10)
+        double hn = h * ((err2 > 1e-24) ? 0.9*pow_neg(err2, 0.1f) : 4.0);
         if (!isfinite(hn)) hn = h;
         S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
         return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o, use64);
     }
-    double hn = h * 0.9 * pow_neg(err, 0.25f);
+    double hn = h * 0.9 * pow_neg(err2, 0.125f);
     if (!isfinite(hn)) hn = h*0.5;
     S.h = hn < 1e-10 ? 1e-10 : hn;
     return after_reject(O, S);

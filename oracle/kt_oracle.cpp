@@ -268,6 +268,17 @@ struct Spacetime {
 // ---------------------------------------------------------------------------
 //  Geodesic integrators (geodesic.hpp)
 // ---------------------------------------------------------------------------
+// Reference behaviours the oracle reproduces by default (same bit values as the KT_QUIRK_*
+// macros of include/kt_hip.h); a cleared bit runs the corrected behaviour instead.
+enum : uint32_t {
+    QK_DP5_PHI_FROZEN = 1u << 0,      // DOPRI5 never advances phi            geodesic.hpp:300-310
+    QK_ESC_POSITIONAL = 1u << 1,      // positional TraceResult fill          main.cpp:360,428-433
+    QK_KS_INIT_NO_SHIFT = 1u << 2,    // KS covector without the shift terms  main.cpp:1373-1430
+    QK_BUNDLE_HORIZON_ESC = 1u << 3,  // bundle horizon -> empty result       ray_bundle.hpp:426-428
+    QK_BUNDLE_ROOT_SKIP = 1u << 4,    // failed bundle root skips the tests   ray_bundle.hpp:357-365
+    QK_REFERENCE = 0x1Fu,
+};
+
 struct Ray { double r, th, ph, pr, pth, pt, pp; };        // geodesic.hpp:33-37
 
 static inline double phi_rate(const Spacetime& g, double r, double th, double pt, double pp) {  // :40-45
@@ -405,7 +416,10 @@ static inline Ray stage(const Ray& s0, double h, const K4* ks, const double* w, 
     }
     return s;
 }
-static bool dopri5(const Spacetime& g, Ray& s, double& h, K4& fsal, double tol) {
+// adv_phi (quirk QK_DP5_PHI_FROZEN cleared): phi advanced with the 5th-order weights from
+// dphi/dl at the six stage points, as RK4 does (geodesic.hpp:184-191); the reference's
+// DOPRI5 leaves phi at its initial value (advance(), geodesic.hpp:300-310).
+static bool dopri5(const Spacetime& g, Ray& s, double& h, K4& fsal, double tol, bool adv_phi = false) {
     K4 k1;
     if (!std::isnan(fsal.v[0])) k1 = fsal; else rhs4(g, s, k1);
     static const double c2[] = {1.0/5.0};
@@ -442,7 +456,17 @@ static bool dopri5(const Spacetime& g, Ray& s, double& h, K4& fsal, double tol) 
     }
     const bool ok = (err <= 1.0 || h < 1e-10);
     if (ok) {
-        s = y5;
+        if (adv_phi) {
+            const Ray* st[6] = {&s, &s2, &s3, &s4, &s5, &s6};
+            double dph = 0.0;
+            for (int i = 0; i < 6; ++i)
+                if (b5[i] != 0.0) dph += b5[i] * phi_rate(g, st[i]->r, st[i]->th, s.pt, s.pp);
+            const double ph1 = s.ph + h * dph;
+            s = y5;
+            s.ph = ph1;
+        } else {
+            s = y5;
+        }
         fsal = k[6];
         const double fac = (err > 1e-12) ? 0.9 * std::pow(1.0/err, 0.2) : 4.0;
         double hn = h * fac;
@@ -460,8 +484,9 @@ static bool dopri5(const Spacetime& g, Ray& s, double& h, K4& fsal, double tol) 
 }
 
 enum Method { RK4D = 0, DP5 = 1 };
-static inline bool take_step(const Spacetime& g, Ray& s, double& h, int method, K4& fsal, double tol) {  // :423-431
-    if (method == DP5) return dopri5(g, s, h, fsal, tol);
+static inline bool take_step(const Spacetime& g, Ray& s, double& h, int method, K4& fsal, double tol,
+                             uint32_t quirks = QK_REFERENCE) {  // :423-431
+    if (method == DP5) return dopri5(g, s, h, fsal, tol, (quirks & QK_DP5_PHI_FROZEN) == 0);
     return rk4_doubling(g, s, h, tol);
 }
 
@@ -547,7 +572,7 @@ struct Result {                     // main.cpp:187-191 (TraceResult) + bundle m
 };
 struct Counters { int64_t accepted = 0, rejected = 0; };
 
-struct Controls { int max_steps; double step_init; double tol; };
+struct Controls { int max_steps; double step_init; double tol; uint32_t quirks = QK_REFERENCE; };
 
 static double disk_g(double r, double pt, double pp, const Spacetime& g) {   // main.cpp:329-341
     double Om = g.omega_k(r);
@@ -559,6 +584,16 @@ static double disk_g(double r, double pt, double pp, const Spacetime& g) {   // 
     if (dv < 1e-8) return 20.0;
     const double red = std::sqrt(d2) / dv;
     return std::isfinite(red) ? red : 1.0;
+}
+
+// Escape-type TraceResult {ESCAPED, r, 1, theta, phi}.  The reference fills it positionally
+// (main.cpp:360,428-433): phi_disk <- theta, theta_esc <- phi, phi_esc <- 0; with the quirk
+// cleared the angles go to theta_esc / phi_esc.
+static Result esc_result(double r, double th, double ph, uint32_t quirks) {
+    Result R{ESC, r, 1.0};
+    if (quirks & QK_ESC_POSITIONAL) { R.ph_disk = th; R.th_esc = ph; R.ph_esc = 0.0; }
+    else { R.ph_disk = 0.0; R.th_esc = th; R.ph_esc = ph; }
+    return R;
 }
 
 // BL driver   main.cpp:343-434
@@ -574,12 +609,9 @@ static Result trace_bl(Ray s, const Spacetime& g, double rin, double rout, doubl
         int rej = 0;
         while (true) {
             hu = h;
-            if (take_step(g, s, h, method, fsal, c.tol)) { if (cnt) cnt->accepted++; break; }
+            if (take_step(g, s, h, method, fsal, c.tol, c.quirks)) { if (cnt) cnt->accepted++; break; }
             if (cnt) cnt->rejected++;
-            if (!std::isfinite(h) || ++rej > 64) {
-                Result R{ESC, s.r, 1.0}; R.ph_disk = s.th; R.th_esc = s.ph; R.ph_esc = 0.0;  // positional fill :360
-                return R;
-            }
+            if (!std::isfinite(h) || ++rej > 64) return esc_result(s.r, s.th, s.ph, c.quirks);   // :360
         }
         double best = 2.0;
         int ev = 0;   // 0 none, 1 disk, 2 horizon, 3 escape
@@ -618,18 +650,10 @@ static Result trace_bl(Ray s, const Spacetime& g, double rin, double rout, doubl
         }
         if (ev == 1) { Result R{DISK, d_r, d_g}; R.ph_disk = d_ph; return R; }
         if (ev == 2) { Result R{HOLE, sp.r + best * (s.r - sp.r), 0.0}; return R; }
-        if (ev == 3) {
-            // {ESCAPED, r_escape, 1.0, th_esc, ph_esc} fills TraceResult positionally (main.cpp:428-430):
-            // phi_disk <- theta at escape, theta_esc <- phi at escape, phi_esc <- 0
-            Result R{ESC, resc, 1.0};
-            R.ph_disk = sp.th + best * (s.th - sp.th);
-            R.th_esc = sp.ph + best * (s.ph - sp.ph);
-            R.ph_esc = 0.0;
-            return R;
-        }
+        if (ev == 3)    // {ESCAPED, r_escape, 1.0, th_esc, ph_esc} (main.cpp:428-430)
+            return esc_result(resc, sp.th + best * (s.th - sp.th), sp.ph + best * (s.ph - sp.ph), c.quirks);
     }
-    Result R{ESC, s.r, 1.0}; R.ph_disk = s.th; R.th_esc = s.ph; R.ph_esc = 0.0;      // positional fill :433
-    return R;
+    return esc_result(s.r, s.th, s.ph, c.quirks);      // :433
 }
 
 // ---- KS tracer (main.cpp:1303-1660) ----
@@ -704,9 +728,23 @@ static std::array<double, 3> covec_ks_to_bl(double r, double th, double ph, doub
     return out;
 }
 
-static bool ks_init(const Ray& b, const Spacetime& g, KSRay& k) {   // :1404-1430
+// exact = false (quirk QK_KS_INIT_NO_SHIFT, the reference): the BL spatial covector is
+// carried over as is and p_T re-solved from H = 0.  exact = true: the full covector map.
+// With T = v - r and phi~ the ingoing Kerr coordinates (dv = dt + (r^2+a^2)/Delta dr,
+// dphi~ = dphi + a/Delta dr), p_T = p_t and the radial component seen at fixed phi~ is
+//   p_r + p_t - ((r^2+a^2) p_t + a p_phi) / Delta,
+// which is what the reference's solve leaves out (its dT/dr and dphi~/dr terms).
+static bool ks_init(const Ray& b, const Spacetime& g, KSRay& k, bool exact = false) {   // :1404-1430
     if (std::abs(g.Lambda) > 1e-15) return false;
     Spacetime::bl_to_ks(b.r, b.th, b.ph, g.a, k.X, k.Y, k.Z);
+    if (exact) {
+        const double ra = b.r*b.r + g.a*g.a;
+        const double dl = ra - 2.0*g.M*b.r + g.Q*g.Q;
+        const double pr_T = b.pr + b.pt - (ra*b.pt + g.a*b.pp) / dl;
+        if (!covec_bl_to_ks(b.r, b.th, b.ph, g.a, pr_T, b.pth, b.pp, k.pX, k.pY, k.pZ)) return false;
+        k.pT = b.pt;
+        return std::isfinite(k.pT);
+    }
     if (!covec_bl_to_ks(b.r, b.th, b.ph, g.a, b.pr, b.pth, b.pp, k.pX, k.pY, k.pZ)) return false;
     double u[4][4];
     g.upper_ks(k.X, k.Y, k.Z, u);
@@ -804,7 +842,7 @@ static bool ks_doubling(const Spacetime& g, KSRay& s, double& h, double tol) {  
 static Result trace_ks(const Ray& b, const Spacetime& g, double rin, double rout, double resc, double rh,
                        int method, const Controls& c, Counters* cnt) {   // :1530-1660
     KSRay s{};
-    if (!ks_init(b, g, s)) return trace_bl(b, g, rin, rout, resc, rh, method, c, cnt);
+    if (!ks_init(b, g, s, (c.quirks & QK_KS_INIT_NO_SHIFT) == 0)) return trace_bl(b, g, rin, rout, resc, rh, method, c, cnt);
     const double rcut = rh * 1.03;
     double h = std::max(c.step_init, 1e-10);
     double rprev = Spacetime::ks_radius(s.X, s.Y, s.Z, g.a);
@@ -820,8 +858,7 @@ static Result trace_ks(const Ray& b, const Spacetime& g, double rin, double rout
             if (!std::isfinite(h) || ++rej > 64) {
                 double rr, tt, pp;
                 Spacetime::ks_to_bl(s.X, s.Y, s.Z, g.a, rr, tt, pp);
-                Result R{ESC, rr, 1.0}; R.ph_disk = tt; R.th_esc = pp; R.ph_esc = 0.0;   // positional fill :1572
-                return R;
+                return esc_result(rr, tt, pp, c.quirks);   // :1572
             }
         }
         const double rnow = Spacetime::ks_radius(s.X, s.Y, s.Z, g.a);
@@ -884,15 +921,13 @@ static Result trace_ks(const Ray& b, const Spacetime& g, double rin, double rout
             const double Ze = sp.Z + best * (s.Z - sp.Z);
             double re, te, pe;
             Spacetime::ks_to_bl(Xe, Ye, Ze, g.a, re, te, pe);
-            Result R{ESC, re, 1.0}; R.ph_disk = te; R.th_esc = pe; R.ph_esc = 0.0;   // positional fill :1650
-            return R;
+            return esc_result(re, te, pe, c.quirks);   // :1650
         }
         rprev = rnow;
     }
     double re, te, pe;
     Spacetime::ks_to_bl(s.X, s.Y, s.Z, g.a, re, te, pe);
-    Result R{ESC, re, 1.0}; R.ph_disk = te; R.th_esc = pe; R.ph_esc = 0.0;   // positional fill :1659
-    return R;
+    return esc_result(re, te, pe, c.quirks);   // :1659
 }
 
 // ---- Jacobi ray bundles (ray_bundle.hpp:56-436) ----
@@ -1062,9 +1097,11 @@ static Result trace_bundle(int px, int py, const Cam& cam, const Spacetime& g, d
             bl_rhs(g, bp.geo.r, bp.geo.th, bp.geo.pr, bp.geo.pth, bp.geo.pt, bp.geo.pp, dr0,dt0,dpr0,dpt0);
             bl_rhs(g, bs.geo.r, bs.geo.th, bs.geo.pr, bs.geo.pth, bs.geo.pt, bs.geo.pp, dr1,dt1,dpr1,dpt1);
             double a = 0.0;
-            if (!hermite_first_root(bp.geo.th, bs.geo.th, dt0, dt1, hu, PI/2.0, a, 8, 8)) continue;   // :360-364
-            const double rhit = hermite(bp.geo.r, bs.geo.r, dr0, dr1, hu, a);
-            if (rhit >= rin && rhit <= rout) {
+            const bool found = hermite_first_root(bp.geo.th, bs.geo.th, dt0, dt1, hu, PI/2.0, a, 8, 8);
+            // the reference skips the horizon and escape tests of this step (:360-364)
+            if (!found && (c.quirks & QK_BUNDLE_ROOT_SKIP)) continue;
+            const double rhit = found ? hermite(bp.geo.r, bs.geo.r, dr0, dr1, hu, a) : 0.0;
+            if (found && rhit >= rin && rhit <= rout) {
                 const double Om = g.omega_k(rhit);
                 const double b = -bs.geo.pp / (-bs.geo.pt);
                 double L[4][4];
@@ -1100,7 +1137,10 @@ static Result trace_bundle(int px, int py, const Cam& cam, const Spacetime& g, d
             if (ae < best) { best = ae; ev = 3; }
         }
         if (ev == 1) { Result R{DISK, d_r, d_g}; R.magnif = d_det; R.ph_disk = d_ph; return R; }
-        if (ev == 2) return none;
+        if (ev == 2) {   // the reference returns the empty BundleResult (:426-428): sky at the pole
+            if (c.quirks & QK_BUNDLE_HORIZON_ESC) return none;
+            return Result{HOLE, 0.0, 0.0};
+        }
         if (ev == 3) {
             Result R{ESC, resc, 1.0};
             R.th_esc = bp.geo.th + best * (bs.geo.th - bp.geo.th);
@@ -1266,6 +1306,8 @@ struct kto_frame {
     double  M, a, Q, Lambda;
     double  theta_deg, phi_deg, r_obs, fov_deg, disk_out;
     double  offset_x, offset_y;
+    uint32_t quirks;      // QK_* mask (QK_REFERENCE: every reference behaviour)
+    int32_t pad;
 };
 
 struct kto_frame_info {     // per-frame scalars (KGeoMeta subset + effective chart)
@@ -1292,7 +1334,7 @@ void kto_frame_info_get(const kto_frame* f, kto_frame_info* out) {
 static void fill_pixel(kto_geopixel& p, const Result& R, bool bundle) {   // main.cpp:2102-2147
     if (bundle) {
         const bool hit = (R.fate == DISK);
-        p.outcome = hit ? 1 : 0;
+        p.outcome = hit ? 1 : (R.fate == HOLE ? 2 : 0);   // HOLE only with QK_BUNDLE_HORIZON_ESC cleared
         p.r = hit ? (float)R.r : 0.0f;
         p.redshift = (float)R.g;
         p.magnif = (float)R.magnif;
@@ -1322,7 +1364,7 @@ int kto_trace_pixels(const kto_frame* f, const int32_t* px, const int32_t* py, i
     kto_frame_info fi;
     kto_frame_info_get(f, &fi);
     const Cam cam(f->r_obs, f->theta_deg, f->phi_deg, f->fov_deg, f->width, f->height);
-    const Controls c{std::max(1, f->max_steps), std::max(1e-10, f->step_init), std::max(1e-10, f->tol)};
+    const Controls c{std::max(1, f->max_steps), std::max(1e-10, f->step_init), std::max(1e-10, f->tol), f->quirks};
     const double rh = fi.r_horizon;   // hoisted: r_horizon() is a pure function of (M,a,Q,Lambda)
 #ifdef _OPENMP
     const int nt = n_threads > 0 ? n_threads : omp_get_max_threads();

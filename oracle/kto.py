@@ -26,7 +26,18 @@ class Frame(ctypes.Structure):
                 ("M", ctypes.c_double), ("a", ctypes.c_double), ("Q", ctypes.c_double),
                 ("Lambda", ctypes.c_double), ("theta_deg", ctypes.c_double), ("phi_deg", ctypes.c_double),
                 ("r_obs", ctypes.c_double), ("fov_deg", ctypes.c_double), ("disk_out", ctypes.c_double),
-                ("offset_x", ctypes.c_double), ("offset_y", ctypes.c_double)]
+                ("offset_x", ctypes.c_double), ("offset_y", ctypes.c_double),
+                ("quirks", ctypes.c_uint32), ("pad", ctypes.c_int32)]
+
+
+# Reference behaviours (same bit values as KT_QUIRK_* of include/kt_hip.h); a cleared bit makes
+# the oracle run the corrected behaviour.
+QUIRK_DOPRI5_PHI_FROZEN = 1 << 0     # geodesic.hpp:300-310
+QUIRK_ESCAPE_POSITIONAL = 1 << 1     # main.cpp:360,428-433,1572,1650,1659
+QUIRK_KS_INIT_NO_SHIFT = 1 << 2      # main.cpp:1373-1430
+QUIRK_BUNDLE_HORIZON_ESC = 1 << 3    # ray_bundle.hpp:426-428
+QUIRK_BUNDLE_ROOT_SKIP = 1 << 4      # ray_bundle.hpp:357-365
+QUIRKS_REFERENCE = 0x1F
 
 
 class FrameInfo(ctypes.Structure):
@@ -86,12 +97,16 @@ def _p(a):
 
 def make_frame(width, height, a, Q=0.0, Lambda=0.0, theta=80.0, phi=0.0, r_obs=40.0, fov=30.0, disk_out=12.0,
                chart="bl", integrator="rk4", bundles=False, max_steps=500000, step_init=1.0, tol=1e-7, M=1.0,
-               offset_x=0.0, offset_y=0.0):
-    """Frame with the reference CLI's parameter meaning (degrees, main.cpp:2548-2684 clamps)."""
+               offset_x=0.0, offset_y=0.0, quirks=QUIRKS_REFERENCE):
+    """Frame with the reference CLI's parameter meaning (degrees, main.cpp:2548-2684 clamps).
+    quirks: mask of QUIRK_* bits (default: every reference behaviour).  Bit 5 of the HIP
+    library (KS with Lambda != 0 run in BL) has no counterpart: the oracle always runs such
+    a frame in BL, the chart-invariant outputs (fate, r, g, theta) being the comparison."""
     return Frame(int(width), int(height), 1 if chart == "ks" else 0, 1 if integrator == "dopri5" else 0,
                  1 if bundles else 0, max(1, int(max_steps)), max(1e-10, float(step_init)),
                  max(1e-10, float(tol)), float(M), float(a), float(Q), float(Lambda), float(theta), float(phi),
-                 float(r_obs), float(fov), float(disk_out), float(offset_x), float(offset_y))
+                 float(r_obs), float(fov), float(disk_out), float(offset_x), float(offset_y),
+                 int(quirks) & QUIRKS_REFERENCE, 0)
 
 
 def frame_info(frame):

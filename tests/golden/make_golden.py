@@ -192,18 +192,23 @@ def images_only(names):
         json.dump(manifest, f, indent=1)
 
 
-def c3_bundle_sample(n=256, seed=3):
+def c3_bundle_sample(n=16384, seed=3):
     """Configuration 3 at full size (1920x1080, a=0.99, theta=89, BL, RK4 doubling, Jacobi
-    bundles): n seeded pixels traced by the oracle (its bundle path is bit-exact to the
-    reference binary on the 64x36 bundle frame; a full reference run of this frame would
-    take about a day on 8 cores)."""
+    bundles): n seeded pixels traced by the oracle with the bundle root-skip quirk cleared
+    (KT_QUIRK_BUNDLE_ROOT_SKIP, ray_bundle.hpp:357-365), the semantics the GPU runs this frame
+    with; every other reference behaviour is kept.  The oracle's bundle path is bit-exact to the
+    reference binary on the 64x36 bundle frame with every quirk set."""
     W, H = 1920, 1080
-    fr = kto.make_frame(W, H, a=0.99, theta=89, r_obs=40, disk_out=12, chart="bl", integrator="rk4", bundles=True)
+    quirks = kto.QUIRKS_REFERENCE & ~kto.QUIRK_BUNDLE_ROOT_SKIP
+    fr = kto.make_frame(W, H, a=0.99, theta=89, r_obs=40, disk_out=12, chart="bl", integrator="rk4", bundles=True,
+                        quirks=quirks)
     rng = np.random.default_rng(seed)
-    px = rng.integers(0, W, n).astype(np.int32)
-    py = rng.integers(0, H, n).astype(np.int32)
+    idx = np.sort(rng.choice(W * H, n, replace=False))
+    px = (idx % W).astype(np.int32)
+    py = (idx // W).astype(np.int32)
     g = kto.trace_pixels(fr, px, py)
-    np.savez_compressed(os.path.join(HERE, "c3_bundles_sample.npz"), px=px, py=py, pixels=g, W=W, H=H)
+    np.savez_compressed(os.path.join(HERE, "c3_bundles_sample.npz"), px=px, py=py, pixels=g, W=W, H=H,
+                        quirks=quirks)
     print("c3 bundle sample", n, np.bincount(g["outcome"], minlength=4))
 
 

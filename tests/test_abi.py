@@ -64,3 +64,58 @@ def test_product_has_no_cpu_fallback(monkeypatch):
     monkeypatch.setattr(kt, "LIB_PATH", "/nonexistent/libkt_hip.so")
     with pytest.raises(kt.KtError):
         kt.load_library()
+
+
+def _status(opts):
+    try:
+        kt.check_opts(opts)
+    except kt.KtError as e:
+        return e.status, str(e)
+    return _abi.KT_OK, ""
+
+
+def test_option_validation_without_gpu():
+    o = kt.default_opts()
+    assert _status(o)[0] == _abi.KT_OK
+    # Jacobi bundles are binary64 only: a mixed request is refused, not run in binary64
+    o.bundles, o.precision = 1, kt.KT_PRECISION_MIXED
+    st, msg = _status(o)
+    assert st == _abi.KT_E_UNSUPPORTED and "binary64" in msg
+    o.precision = kt.KT_PRECISION_FP64
+    assert _status(o)[0] == _abi.KT_OK
+    o = kt.default_opts()
+    o.rtol = 0.0
+    assert _status(o)[0] == _abi.KT_E_ARG
+
+
+@pytest.mark.parametrize("rings,sectors,sigma,ok", [
+    (7, 14, 0.5, True), (0, 0, 0.0, True), (150, 150, 10.0, True), (200, 200, 10.0, False),
+    (7, 14, 1e9, False), (7, 14, float("inf"), False), (7, 14, float("nan"), False),
+    (100000, 14, 0.5, False), (300, 300, 0.5, False)])
+def test_interstellar_palette_loop_bound(rings, sectors, sigma, ok):
+    """The shade kernel's cell loop is (rings + 2s + 1)(sectors + 2s + 1), s = ceil(3 sigma) + 1
+    (main.cpp:1717-1797, which has no bound): sizes beyond 65536 cells per pixel are KT_E_ARG."""
+    o = kt.default_opts()
+    o.palette = kt.KT_PALETTE_INTERSTELLAR
+    o.disk_rings, o.disk_sectors, o.disk_sigma = rings, sectors, sigma
+    assert (_status(o)[0] == _abi.KT_OK) == ok
+    o.palette = kt.KT_PALETTE_BLACKBODY      # the cell parameters are unused by the blackbody palette
+    assert _status(o)[0] == _abi.KT_OK
+
+
+def test_ppm_header_size_overflow_is_rejected(tmp_path):
+    """A P6 header whose width * height overflows a signed 64-bit product is refused, not read."""
+    lib = kt.load_library()
+    p = tmp_path / "big.ppm"
+    p.write_bytes(b"P6\n9999999999 9999999999\n255\n" + b"\0" * 64)
+    buf = np.zeros(64, np.uint8)
+    w, h = ctypes.c_int32(0), ctypes.c_int32(0)
+    assert lib.kt_read_image_rgb(str(p).encode(), buf.ctypes.data_as(ctypes.c_void_p), buf.size,
+                                 ctypes.byref(w), ctypes.byref(h)) != _abi.KT_OK
+    p.write_bytes(b"P6\n70000 2\n255\n" + b"\0" * 64)
+    assert lib.kt_read_image_rgb(str(p).encode(), buf.ctypes.data_as(ctypes.c_void_p), buf.size,
+                                 ctypes.byref(w), ctypes.byref(h)) != _abi.KT_OK
+    p.write_bytes(b"P6\n4 2\n255\n" + bytes(range(24)))
+    assert lib.kt_read_image_rgb(str(p).encode(), buf.ctypes.data_as(ctypes.c_void_p), buf.size,
+                                 ctypes.byref(w), ctypes.byref(h)) == _abi.KT_OK
+    assert (w.value, h.value) == (4, 2) and np.array_equal(buf[:24], np.arange(24))

@@ -273,6 +273,17 @@ def test_invalid_arguments_are_reported(gpu_ctx):
     o.chart = 7
     with pytest.raises(kt.KtError):
         gpu_ctx.render([kt.make_frame(4, 4, a=0.5)], o)
+    # Jacobi bundles are binary64 only: a mixed-precision bundle render is refused
+    with pytest.raises(kt.KtError) as e:
+        gpu_ctx.render([kt.make_frame(8, 8, a=0.5)], _opts(bundles=1, prec="mixed"))
+    assert e.value.status == kt._abi.KT_E_UNSUPPORTED
+    o = _opts(palette=kt.KT_PALETTE_INTERSTELLAR, disk_sigma=1e9)
+    with pytest.raises(kt.KtError) as e:
+        gpu_ctx.render([kt.make_frame(8, 8, a=0.5)], o)
+    assert e.value.status == kt._abi.KT_E_ARG
+    # the context stays usable after refused calls
+    out = gpu_ctx.render([kt.make_frame(8, 8, a=0.5)], _opts(bundles=1))
+    assert out["stats"]["rays"] == 64 and out["stats"]["rhs_fp32"] == 0
 
 
 def test_bundles_vs_reference(gpu_ctx):
@@ -297,19 +308,21 @@ def test_bundles_vs_reference(gpu_ctx):
 
 def test_c3_full_size_bundles_vs_oracle_sample(gpu_ctx):
     """Configuration 3 at full size (1920x1080, a=0.99, theta=89, BL, RK4 doubling, Jacobi
-    bundles): 256 seeded pixels against the oracle (tests/golden/c3_bundles_sample.npz).  The
-    band here is taken from the GPU image itself (fate changes among the 8 neighbours, and disk
-    pixels whose g differs from a neighbour's by more than 0.05).
+    bundles) against 16 384 seeded pixels of the oracle (tests/golden/c3_bundles_sample.npz), both
+    with the bundle root-skip quirk cleared (ray_bundle.hpp:357-365: with it, near-equatorial
+    steps without a disk root skip the horizon and escape tests, and about a quarter of this
+    frame's rays run to the 500 000-step cap) and every other reference behaviour kept, horizon
+    pixels included (outcome 0, angles 0: ray_bundle.hpp:426-428).
 
-    The GPU runs with the bundle root-skip quirk off: with it, near-equatorial steps without a
-    disk root skip the horizon and escape tests and 58 of these 256 rays run to the 500 000-step
-    cap (outcome 0, r = 0, angles 0), which makes the full frame take minutes.  The quirk also
-    moves the recorded escape point of every escaping ray (to the first step after r_escape that
-    is not skipped), so escapes are compared on fate only; disk pixels in full."""
+    Bars: fate on >= 99.9 % of the pixels off the band (taken from the GPU image: fate changes
+    among the 8 neighbours, steep g); on disk pixels r_hit, g and the magnification |det J|; on
+    escapes the escape angles (linear interpolation to r_escape between steps that differ
+    slightly, since the RHS here is analytic and the reference's is a finite difference)."""
     g = load_golden("c3_bundles_sample")
     W, H = int(g["W"]), int(g["H"])
     o = _opts(bundles=1)
-    o.quirks &= ~kt.KT_QUIRK_BUNDLE_ROOT_SKIP
+    o.quirks = int(g["quirks"])
+    assert not (o.quirks & kt.KT_QUIRK_BUNDLE_ROOT_SKIP)
     out = gpu_ctx.render([kt.make_frame(W, H, a=0.99, theta=89, r_obs=40, disk_out=12)], o, want_rgba=False)
     print("C3 bundles stats", out["stats"], flush=True)
     full = out["geo"]
@@ -317,13 +330,28 @@ def test_c3_full_size_bundles_vs_oracle_sample(gpu_ctx):
     got = full.reshape(H, W)[g["py"], g["px"]]
     ref = g["pixels"]
     keep = ~band
-    assert keep.sum() >= 200
-    assert (got["outcome"] == ref["outcome"])[keep].mean() >= 0.99
+    fate = (got["outcome"] == ref["outcome"])
+    print("C3 fate agreement all %.6f off-band %.6f (%d off-band)" % (fate.mean(), fate[keep].mean(), keep.sum()),
+          flush=True)
+    assert keep.sum() >= 14000
+    assert fate[keep].mean() >= 0.999
     d = keep & (got["outcome"] == 1) & (ref["outcome"] == 1)
-    assert d.sum() >= 50
-    assert np.abs(got["r"][d] - ref["r"][d]).max() < 2e-3
-    assert np.abs(got["redshift"][d] - ref["redshift"][d]).max() < 5e-4
+    assert d.sum() >= 6000
+    dr = np.abs(got["r"][d] - ref["r"][d])
+    dg = np.abs(got["redshift"][d] - ref["redshift"][d])
     rel = np.abs(got["magnif"][d] / ref["magnif"][d] - 1.0)
-    assert np.mean(rel < 0.10) >= 0.95
-    capped = (ref["outcome"] == 0) & (ref["r"] == 0) & (ref["theta_esc"] == 0) & (ref["phi_esc"] == 0)
-    assert capped.sum() >= 40 and np.all(got["outcome"][capped] == 0)
+    print("C3 disk max|dr| %.3g max|dg| %.3g magnif rel q50/q95/q99 %s" %
+          (dr.max(), dg.max(), np.quantile(rel, [0.5, 0.95, 0.99])), flush=True)
+    assert dr.max() < 2e-3
+    assert dg.max() < 1e-3 and np.quantile(dg, 0.999) < 2.5e-4
+    assert np.quantile(rel, 0.5) < 0.005
+    assert np.mean(rel < 0.05) >= 0.97
+    e = keep & (got["outcome"] == 0) & (ref["outcome"] == 0) & (ref["theta_esc"] != 0)
+    assert e.sum() >= 4000
+    dth = np.abs(got["theta_esc"][e] - ref["theta_esc"][e])
+    dph = np.abs(got["phi_esc"][e] - ref["phi_esc"][e])
+    print("C3 escape max|dtheta| %.3g max|dphi| %.3g" % (dth.max(), dph.max()), flush=True)
+    assert dth.max() < 2e-3 and dph.max() < 5e-3
+    hz = keep & (ref["outcome"] == 0) & (ref["theta_esc"] == 0)     # horizon: the empty bundle result
+    assert hz.sum() >= 2000
+    assert np.all((got["theta_esc"][hz] == 0) & (got["phi_esc"][hz] == 0) & (got["r"][hz] == 0))

@@ -159,6 +159,28 @@ def test_bundle_magnification_converged_vs_oracle():
     assert np.all(np.abs(got["magnif"] / ref["magnif"] - 1.0) < 0.01)
 
 
+def test_c3_bundle_sample_like_for_like():
+    """Configuration 3 (1920x1080, a=0.99, theta=89, bundles, root-skip quirk cleared on both
+    sides): the 16 384-pixel oracle sample that the full-size GPU test uses.  The host build of
+    the kernels' code agrees on every fate, with no band excluded."""
+    g = load_golden("c3_bundles_sample")
+    op = _abi.default_opts()
+    op.bundles = 1
+    op.quirks = int(g["quirks"])
+    got, _ = hc.trace_bundle_pixels(_abi.make_frame(int(g["W"]), int(g["H"]), a=0.99, theta=89, r_obs=40,
+                                                    disk_out=12), op, g["px"], g["py"])
+    ref = g["pixels"]
+    assert np.array_equal(got["outcome"], ref["outcome"])
+    d = ref["outcome"] == 1
+    assert np.abs(got["r"][d] - ref["r"][d]).max() < 1e-3
+    assert np.quantile(np.abs(got["redshift"][d] - ref["redshift"][d]), 0.999) < 2.5e-4
+    rel = np.abs(got["magnif"][d] / ref["magnif"][d] - 1.0)
+    assert np.quantile(rel, 0.5) < 0.005 and np.mean(rel < 0.05) >= 0.97
+    e = (ref["outcome"] == 0) & (ref["theta_esc"] != 0)
+    assert np.abs(got["theta_esc"][e] - ref["theta_esc"][e]).max() < 1e-3
+    assert np.abs(got["phi_esc"][e] - ref["phi_esc"][e]).max() < 2e-3
+
+
 def test_kernel_sincos_accuracy():
     """The kernels' inline sin/cos (Cody-Waite reduction + Taylor series) against x87
     extended precision: <= 2 ulp over the angles a ray meets, exact special values."""
@@ -282,3 +304,13 @@ def test_constraint_projection_restores_null_condition():
         assert cosang > 1 - 1e-6, (mp, cam, cosang)
         n_checked += 1
     assert n_checked >= 50
+
+
+@pytest.mark.parametrize("chart", [0, 1])
+def test_parked_lane_state_is_a_fixed_point(chart):
+    """Idle lanes of the persistent trace kernel run every attempt on a parked state (r at the
+    escape radius, theta 1 rad, every momentum zero): the right-hand side must vanish there
+    exactly, so the lane never moves (no infall through the horizon cut, no disk search)."""
+    for par in ((0.9, 0.3, 1e-5), (0.99, 0.0, 0.0), (-0.5, 0.2, 0.0)):
+        f = hc.rhs(chart, 1.0, par, [42.0, 1.0, 0.0, 0.0], 0.0, 0.0)
+        assert np.array_equal(f, np.zeros(4)), (chart, par, f)
