@@ -182,6 +182,23 @@ int kt_hip_render(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const k
 int kt_hip_render_device(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const kt_opts* opts,
                          void** d_rgba, void** d_geo, kt_stats* stats);
 
+/* Allocation-free variant of kt_hip_render_device.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The call allocates nothing: the caller passes device output
+ * buffers, d_geo (28 bytes per pixel, required) and d_rgba (4 bytes per pixel, or NULL to skip
+ * the shading).  The queue and statistics words, the frame table, the per-frame setup scratch and
+ * the tile orders live in one fixed device array of about 2.3 MB per GPU, held until the process
+ * ends; the trace kernel generates each camera ray itself (no per-pixel init records).  Limits:
+ * n_frames <= KT_NOALLOC_MAX_FRAMES and the frames' 8x8-tile padded ray count, the sum of
+ * ceil(W/8)*ceil(H/8)*64, <= KT_NOALLOC_MAX_RAYS; beyond them the call returns KT_E_ARG.  The
+ * launch uses the default register budget (tuning_occupancy is ignored).  Every call on a GPU
+ * shares that GPU's array, which keeps the last call's contents: this library serialises its own
+ * calls with a process-wide lock, and calls that overlap on one GPU by any other route race.   */
+#define KT_NOALLOC_MAX_FRAMES 64
+#define KT_NOALLOC_MAX_RAYS (1ll << 25)
+int kt_hip_render_device_noalloc(kt_hip_ctx* ctx, const kt_frame* frames, int n_frames, const kt_opts* opts,
+                                 void* d_rgba, void* d_geo, kt_stats* stats);
+
 /* Phase 2 on the device: GeoPixel records -> ABGR (colorize_buffer, main.cpp:1817-1866), with the
  * same scalars colorize_buffer takes (M, r_isco, r_disk_in, r_disk_out) and the palette of opts. */
 int kt_hip_colorize(kt_hip_ctx* ctx, const kt_geopixel* geo, int64_t n, double M, double r_isco,

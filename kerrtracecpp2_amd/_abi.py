@@ -4,6 +4,8 @@ import ctypes
 import numpy as np
 
 KT_OK = 0
+KT_NOALLOC_MAX_FRAMES = 64
+KT_NOALLOC_MAX_RAYS = 1 << 25
 KT_E_ARG, KT_E_DEVICE, KT_E_NOFP64, KT_E_ALLOC, KT_E_LAUNCH, KT_E_UNSUPPORTED = -1, -2, -3, -4, -5, -6
 KT_CHART_BL, KT_CHART_KS = 0, 1
 KT_INTEGRATOR_RK4, KT_INTEGRATOR_DOPRI5 = 0, 1
@@ -99,6 +101,7 @@ PROTOTYPES = {
     "kt_hip_set_background": (ctypes.c_int, [_P, _P]),
     "kt_hip_render": (ctypes.c_int, [_P, _P, _I, _P, _P, _P, _P, _P]),
     "kt_hip_render_device": (ctypes.c_int, [_P, _P, _I, _P, _P, _P, _P]),
+    "kt_hip_render_device_noalloc": (ctypes.c_int, [_P, _P, _I, _P, _P, _P, _P]),
     "kt_hip_colorize": (ctypes.c_int, [_P, _P, ctypes.c_int64, _D, _D, _D, _D, _P, _P]),
     "kt_hip_frame_info": (ctypes.c_int, [_P, _P, _I, _P, _P]),
     "kt_make_checker_background": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -373,12 +374,15 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // stops copying the whole ray state between registers on every loop trip (~1.5 % mixed);
 // the cost of the idle lanes is the same either way, since a wave issues for all 64.
 // Policies with side effects in advance() keep the divergent form.
-template <int CHART, int METHOD, int PREC, bool PHI>
+// INLINE_INIT: the lane runs the camera ray itself when it takes a ray (the allocation-free
+// entry point has no per-pixel init records); otherwise it loads the ray's init record.
+template <int CHART, int METHOD, int PREC, bool PHI, bool INLINE_INIT = false>
 struct GeodesicPolicy {
     typedef RayState State;
     static constexpr bool kUnconditional = true;
-    __device__ static void init(const TraceArgs& A, const FrameDev&, int, int, int64_t pix, State& S) {
-        ray_load_init(A.init[pix], A.O, S);
+    __device__ static void init(const TraceArgs& A, const FrameDev& F, int px, int py, int64_t pix, State& S) {
+        if (INLINE_INIT) ray_init<CHART, PREC_FP64>(F, A.O, px, py, S);
+        else ray_load_init(A.init[pix], A.O, S);
     }
     // A benign state for an idle lane: at the escape radius, 1 rad from the axis and the
     // equator, with every momentum zero.  The Hamiltonian is quadratic in the momenta, so
@@ -591,6 +595,14 @@ void kt_trace_kernel(TraceArgs A) {
     trace_loop<GeodesicPolicy<CHART, METHOD, PREC, PHI>, SINGLE>(A);
 }
 
+// Multi-frame trace kernel that generates each camera ray inside the loop (no init records):
+// the allocation-free entry point's kernel, at the default register budget.
+template <int CHART, int METHOD, int PREC, bool PHI>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(default_occ(PREC), 8)))
+void kt_trace_inline_kernel(TraceArgs A) {
+    trace_loop<GeodesicPolicy<CHART, METHOD, PREC, PHI, true>, false>(A);
+}
+
 // Jacobi ray bundles: BL chart, RK4 step doubling, binary64 (ray_bundle.hpp).  A bundle
 // attempt keeps the geodesic, the 4x2 deviation matrix, its derivative and the old state
 // for a reject live across dual-number right-hand sides of ~110 registers each: about
@@ -780,6 +792,23 @@ TraceKernelFn pick_kernel_occ(int chart, int method, int prec, bool phi) {
                              : pick_kernel_chart<CHART_BL, OCC, SINGLE>(method, prec, phi);
 }
 
+template <int CHART>
+TraceKernelFn pick_inline_chart(int method, int prec, bool phi) {
+    if (method == METHOD_RK4)
+        return prec ? kt_trace_inline_kernel<CHART, METHOD_RK4, PREC_MIXED, true>
+                    : kt_trace_inline_kernel<CHART, METHOD_RK4, PREC_FP64, true>;
+    if (phi)
+        return prec ? kt_trace_inline_kernel<CHART, METHOD_DP5, PREC_MIXED, true>
+                    : kt_trace_inline_kernel<CHART, METHOD_DP5, PREC_FP64, true>;
+    return prec ? kt_trace_inline_kernel<CHART, METHOD_DP5, PREC_MIXED, false>
+                : kt_trace_inline_kernel<CHART, METHOD_DP5, PREC_FP64, false>;
+}
+
+TraceKernelFn pick_inline_kernel(int chart, int method, int prec, bool phi) {
+    return chart == CHART_KS ? pick_inline_chart<CHART_KS>(method, prec, phi)
+                             : pick_inline_chart<CHART_BL>(method, prec, phi);
+}
+
 // occ: 0 built-in default for the precision, 2 or 3 waves per SIMD
 TraceKernelFn pick_kernel(int chart, int method, int prec, bool phi, int occ, bool single) {
     if (occ <= 0) occ = default_occ(prec);
@@ -831,6 +860,8 @@ struct kt_hip_ctx {
     // tile processing orders per frame shape (tiles_x, tiles_y), kept for the context's
     // lifetime: frame tables and captured graphs point into them
     std::map<uint64_t, uint32_t*> orders;
+    unsigned char* d_noalloc = nullptr;     // this device's allocation-free scratch (not owned)
+    std::string noalloc_err;                // why d_noalloc is null, if it is
 };
 
 static int fail(kt_hip_ctx* c, int code, const char* fmt, ...) {
@@ -849,6 +880,59 @@ static int fail(kt_hip_ctx* c, int code, const char* fmt, ...) {
         if (e_ != hipSuccess)                                                                   \
             return fail((ctx), KT_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));     \
     } while (0)
+
+// ============================================================================
+//  allocation-free render: one fixed scratch array per GPU
+// ============================================================================
+constexpr int kNoAllocMaxFrames = KT_NOALLOC_MAX_FRAMES;
+constexpr int64_t kNoAllocMaxRays = KT_NOALLOC_MAX_RAYS;
+constexpr int64_t kNoAllocMaxTiles = kNoAllocMaxRays / 64;
+
+// byte offsets into the scratch array
+struct NoAllocLayout {
+    static constexpr size_t words = 0;                                       // W_COUNT u64
+    static constexpr size_t table = 256;                                     // LaunchDesc + FrameDev[max]
+    static constexpr size_t table_bytes = sizeof(LaunchDesc) + kNoAllocMaxFrames * sizeof(FrameDev);
+    static constexpr size_t setup = (table + table_bytes + 255) / 256 * 256; // SetupScratch[max]
+    static constexpr size_t order = (setup + kNoAllocMaxFrames * sizeof(SetupScratch) + 255) / 256 * 256;
+    static constexpr size_t total = order + (size_t)kNoAllocMaxTiles * sizeof(uint32_t);
+};
+static_assert(W_COUNT * sizeof(unsigned long long) <= NoAllocLayout::table, "scratch layout");
+static_assert(NoAllocLayout::total < (size_t)5 << 19, "allocation-free scratch stays below 2.5 MB");
+
+// Device memory of the allocation-free path.  In host code this name is only a stand-in for
+// the symbol: the per-device address comes from hipGetSymbolAddress (noalloc_setup) and only
+// that address is passed to kernels and copies.
+static __device__ __align__(256) unsigned char noalloc_scratch[NoAllocLayout::total];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;                                       // guards everything below
+static unsigned char* g_noalloc_dev[kNoAllocMaxDevices];              // scratch address per device
+struct NoAllocShapes { int n = -1; int32_t tx[kNoAllocMaxFrames], ty[kNoAllocMaxFrames]; };
+static NoAllocShapes g_noalloc_shapes[kNoAllocMaxDevices];            // tile orders each scratch holds
+alignas(64) static unsigned char g_noalloc_table[NoAllocLayout::table_bytes];   // host staging
+static uint32_t g_noalloc_order[kNoAllocMaxTiles];
+static uint64_t g_noalloc_keys[kNoAllocMaxTiles];
+
+// Once per device, with the device current (kt_hip_create): the device address of the scratch.
+// A failed lookup leaves *out null and the message in err: the context still works and only the
+// allocation-free entry point refuses (KT_E_UNSUPPORTED, with that message).
+static void noalloc_setup(int device, unsigned char** out, std::string* err) {
+    *out = nullptr;
+    if (device < 0 || device >= kNoAllocMaxDevices) { *err = "device index beyond the scratch table"; return; }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            *err = std::string("hipGetSymbolAddress(noalloc_scratch) failed: ") + hipGetErrorString(e);
+            return;
+        }
+        g_noalloc_dev[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_noalloc_dev[device];
+}
 
 extern "C" {
 
@@ -914,6 +998,7 @@ int kt_hip_create(int device, kt_hip_ctx** out) {
         kt_hip_destroy(c);
         return KT_E_ALLOC;
     }
+    noalloc_setup(device, &c->d_noalloc, &c->noalloc_err);
     *out = c;
     return KT_OK;
 }
@@ -1036,20 +1121,28 @@ static int table_reserve(kt_hip_ctx* c, FrameTable& t, size_t n) {
 // persistent waves full until the queue drains instead of idling most lanes behind a
 // few long rays at the end.  Uploaded once per frame shape (never during a capture:
 // callers prepare frames before they enqueue).
+// Centre-out order of a tx x ty tile grid into ord[tx*ty]: sorted by (d^2, row-major index) as
+// one 64-bit key per tile (keys[tx*ty] is work space; d^2 < 2^32 for tiles_x, tiles_y <= 8192).
+static void centre_out_order(int tx, int ty, uint32_t* ord, uint64_t* keys) {
+    const size_t n = (size_t)tx * ty;
+    for (int y = 0; y < ty; ++y)
+        for (int x = 0; x < tx; ++x) {
+            const int64_t dx = 2 * (int64_t)x - (tx - 1), dy = 2 * (int64_t)y - (ty - 1);
+            const size_t i = (size_t)y * tx + x;
+            keys[i] = ((uint64_t)(dx * dx + dy * dy) << 32) | (uint64_t)i;
+        }
+    std::sort(keys, keys + n);
+    for (size_t i = 0; i < n; ++i) ord[i] = (uint32_t)(keys[i] & 0xFFFFFFFFu);
+}
+
 static int tile_order(kt_hip_ctx* c, int tx, int ty, const uint32_t** out) {
     const uint64_t key = ((uint64_t)(uint32_t)tx << 32) | (uint32_t)ty;
     auto it = c->orders.find(key);
     if (it != c->orders.end()) { *out = it->second; return KT_OK; }
     const size_t n = (size_t)tx * ty;
     std::vector<uint32_t> ord(n);
-    std::vector<int64_t> d2(n);
-    for (int y = 0; y < ty; ++y)
-        for (int x = 0; x < tx; ++x) {
-            const int64_t dx = 2 * (int64_t)x - (tx - 1), dy = 2 * (int64_t)y - (ty - 1);
-            d2[(size_t)y * tx + x] = dx * dx + dy * dy;
-        }
-    for (size_t i = 0; i < n; ++i) ord[i] = (uint32_t)i;
-    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t p, uint32_t q) { return d2[p] < d2[q]; });
+    std::vector<uint64_t> keys(n);
+    centre_out_order(tx, ty, ord.data(), keys.data());
     uint32_t* d = nullptr;
     if (hipMalloc(&d, n * sizeof(uint32_t)) != hipSuccess) return fail(c, KT_E_ALLOC, "tile order (%zu tiles)", n);
     if (hipMemcpy(d, ord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
@@ -1061,11 +1154,15 @@ static int tile_order(kt_hip_ctx* c, int tx, int ty, const uint32_t** out) {
     return KT_OK;
 }
 
-static int prepare_frames(kt_hip_ctx* c, FrameTable& t, const kt_frame* frames, int n_frames, const kt_opts* o,
-                          int64_t* total_rays, int64_t* total_pix, int* chart_out) {
+}  // extern "C"
+
+// Host copy of the frame table (descriptor + frames) for a frame list; order(tx, ty, &ptr)
+// supplies the device address of each frame's tile order.
+template <class OrderFn>
+static int fill_frames(kt_hip_ctx* c, LaunchDesc* d, FrameDev* hf, size_t cap, const kt_frame* frames, int n_frames,
+                       const kt_opts* o, OrderFn&& order, int64_t* total_rays, int64_t* total_pix, int* chart_out) {
     if (!frames || n_frames <= 0) return fail(c, KT_E_ARG, "no frames");
-    if ((size_t)n_frames > t.cap) return fail(c, KT_E_ARG, "frame table capacity %zu < %d frames", t.cap, n_frames);
-    FrameDev* hf = t.hframes();
+    if ((size_t)n_frames > cap) return fail(c, KT_E_ARG, "frame table capacity %zu < %d frames", cap, n_frames);
     int64_t rays = 0, pix = 0;
     int chart = -1;
     for (int i = 0; i < n_frames; ++i) {
@@ -1087,7 +1184,7 @@ static int prepare_frames(kt_hip_ctx* c, FrameTable& t, const kt_frame* frames, 
         F.W = f.width; F.H = f.height;
         F.tiles_x = (f.width + 7) / 8;
         F.tiles_y = (f.height + 7) / 8;
-        const int orc = tile_order(c, F.tiles_x, F.tiles_y, &F.tile_order);
+        const int orc = order(F.tiles_x, F.tiles_y, &F.tile_order);
         if (orc != KT_OK) return orc;
         F.ray_base = rays;
         F.pix_base = pix;
@@ -1098,7 +1195,6 @@ static int prepare_frames(kt_hip_ctx* c, FrameTable& t, const kt_frame* frames, 
         rays += (int64_t)F.tiles_x * F.tiles_y * 64;
         pix += (int64_t)f.width * f.height;
     }
-    LaunchDesc* d = t.hdesc();
     std::memset(d, 0, sizeof(*d));
     d->n_frames = n_frames;
     d->total_rays = rays;
@@ -1107,6 +1203,15 @@ static int prepare_frames(kt_hip_ctx* c, FrameTable& t, const kt_frame* frames, 
     *total_pix = pix;
     *chart_out = chart;
     return KT_OK;
+}
+
+extern "C" {
+
+static int prepare_frames(kt_hip_ctx* c, FrameTable& t, const kt_frame* frames, int n_frames, const kt_opts* o,
+                          int64_t* total_rays, int64_t* total_pix, int* chart_out) {
+    return fill_frames(c, t.hdesc(), t.hframes(), t.cap, frames, n_frames, o,
+                       [c](int tx, int ty, const uint32_t** out) { return tile_order(c, tx, ty, out); },
+                       total_rays, total_pix, chart_out);
 }
 
 static int reserve_pixels(kt_hip_ctx* c, uint32_t** geo, uint32_t** rgba, size_t* cap, int64_t pix) {
@@ -1221,8 +1326,11 @@ static void enqueue_setup(kt_hip_ctx* c, FrameDev* frames, const LaunchDesc* des
 static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, int chart, int setup_grid,
                           int64_t max_rays, bool single, bool shade, const Launch& L, hipEvent_t* ev) {
     const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
+    // no init records (the allocation-free path): the multi-frame kernel that runs the camera itself
+    const bool inline_init = !o->bundles && B.init == nullptr;
     TraceKernelFn fn = o->bundles ? pick_bundle_kernel(L.occ, single)
-                                  : pick_kernel(chart, o->integrator, o->precision, phi, L.occ, single);
+                       : inline_init ? pick_inline_kernel(chart, o->integrator, o->precision, phi)
+                                     : pick_kernel(chart, o->integrator, o->precision, phi, L.occ, single);
     if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");
     // persistent grid: blocks beyond the resident limit start when others retire and
     // find the queue drained, so over-subscription is harmless
@@ -1245,7 +1353,7 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
     hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, B.words, (int)W_COUNT);
     if (ev) KT_HIP(c, hipEventRecord(ev[0], c->stream));
     enqueue_setup(c, B.frames, B.desc, B.scratch, setup_grid);
-    if (!o->bundles) {
+    if (!o->bundles && !inline_init) {
         if (chart == KT_CHART_KS)
             hipLaunchKernelGGL(kt_init_kernel<CHART_KS>, dim3((unsigned)(c->n_cu * 8)), dim3(256), 0, c->stream,
                                B.frames, B.desc, B.init);
@@ -1322,6 +1430,82 @@ int kt_hip_render_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames, co
     if (d_rgba) *d_rgba = c->d_rgba;
     if (d_geo) *d_geo = c->d_geo;
     return KT_OK;
+}
+
+int kt_hip_render_device_noalloc(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o,
+                                 void* d_rgba, void* d_geo, kt_stats* st) {
+    if (!c) return KT_E_ARG;
+    if (!d_geo) return fail(c, KT_E_ARG, "d_geo (device memory, 28 bytes per pixel) is required");
+    if (!c->d_noalloc)
+        return fail(c, KT_E_UNSUPPORTED, "no allocation-free scratch on device %d: %s", c->device,
+                    c->noalloc_err.c_str());
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    if (!frames || n_frames <= 0) return fail(c, KT_E_ARG, "no frames");
+    if (n_frames > kNoAllocMaxFrames)
+        return fail(c, KT_E_ARG, "%d frames exceed the allocation-free maximum of %d", n_frames, kNoAllocMaxFrames);
+    int64_t need = 0;
+    for (int i = 0; i < n_frames; ++i) {
+        if (frames[i].width <= 0 || frames[i].height <= 0 || frames[i].width > 65536 || frames[i].height > 65536)
+            return fail(c, KT_E_ARG, "frame %d: bad size %dx%d", i, frames[i].width, frames[i].height);
+        need += (int64_t)((frames[i].width + 7) / 8) * ((frames[i].height + 7) / 8) * 64;
+    }
+    if (need > kNoAllocMaxRays)
+        return fail(c, KT_E_ARG, "%lld rays (8x8-tile padded) exceed the allocation-free maximum of %lld",
+                    (long long)need, (long long)kNoAllocMaxRays);
+    KT_HIP(c, hipSetDevice(c->device));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    unsigned char* base = c->d_noalloc;
+    uint32_t* d_order = reinterpret_cast<uint32_t*>(base + NoAllocLayout::order);
+    // one tile-order segment per distinct frame shape, in order of first appearance
+    int nshape = 0;
+    int32_t stx[kNoAllocMaxFrames], sty[kNoAllocMaxFrames];
+    int64_t soff[kNoAllocMaxFrames], used = 0;
+    auto order = [&](int tx, int ty, const uint32_t** out) -> int {
+        for (int k = 0; k < nshape; ++k)
+            if (stx[k] == tx && sty[k] == ty) { *out = d_order + soff[k]; return KT_OK; }
+        stx[nshape] = tx; sty[nshape] = ty; soff[nshape] = used;
+        *out = d_order + used;
+        used += (int64_t)tx * ty;
+        ++nshape;
+        return KT_OK;
+    };
+    LaunchDesc* hd = reinterpret_cast<LaunchDesc*>(g_noalloc_table);
+    FrameDev* hf = reinterpret_cast<FrameDev*>(g_noalloc_table + sizeof(LaunchDesc));
+    int64_t rays = 0, pix = 0;
+    int chart = 0;
+    rc = fill_frames(c, hd, hf, kNoAllocMaxFrames, frames, n_frames, o, order, &rays, &pix, &chart);
+    if (rc) return rc;
+    // the scratch keeps the last call's tile orders: upload only when the shapes changed
+    NoAllocShapes& held = g_noalloc_shapes[c->device];
+    bool same = held.n == nshape;
+    for (int k = 0; same && k < nshape; ++k) same = held.tx[k] == stx[k] && held.ty[k] == sty[k];
+    if (!same) {
+        held.n = -1;                                   // unknown until this call has synchronised
+        for (int k = 0; k < nshape; ++k) centre_out_order(stx[k], sty[k], g_noalloc_order + soff[k], g_noalloc_keys);
+        KT_HIP(c, hipMemcpyAsync(d_order, g_noalloc_order, (size_t)used * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    KT_HIP(c, hipMemcpyAsync(base + NoAllocLayout::table, g_noalloc_table,
+                             sizeof(LaunchDesc) + (size_t)n_frames * sizeof(FrameDev), hipMemcpyHostToDevice, c->stream));
+    Launch L = launch_config(c, o, chart);
+    L.occ = 0;                                         // default register budget (the inline kernels')
+    unsigned long long* d_words = reinterpret_cast<unsigned long long*>(base + NoAllocLayout::words);
+    RenderBufs B{reinterpret_cast<LaunchDesc*>(base + NoAllocLayout::table),
+                 reinterpret_cast<FrameDev*>(base + NoAllocLayout::table + sizeof(LaunchDesc)),
+                 reinterpret_cast<SetupScratch*>(base + NoAllocLayout::setup),
+                 static_cast<uint32_t*>(d_geo), static_cast<uint32_t*>(d_rgba), d_words, nullptr};
+    rc = enqueue_render(c, B, o, chart, n_frames, rays, false, d_rgba != nullptr, L, c->ev);
+    if (rc) return rc;
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    if (!same) {
+        held.n = nshape;
+        for (int k = 0; k < nshape; ++k) { held.tx[k] = stx[k]; held.ty[k] = sty[k]; }
+    }
+    float t_trace = 0.f, t_total = 0.f;
+    (void)hipEventElapsedTime(&t_trace, c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&t_total, c->ev[0], c->ev[3]);
+    return read_stats(c, d_words, t_trace, t_total, st);
 }
 
 int kt_hip_render(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const kt_opts* o, uint32_t* rgba_out,
