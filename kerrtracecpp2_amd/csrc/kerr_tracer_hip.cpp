@@ -291,8 +291,7 @@ int main(int argc, char** argv) {
                                              (camera_spp > 1 ? "_spp" + std::to_string(camera_spp) : "");
         const bool fp64 = gpu_fp64 || precision == "fp64";
 
-        auto render_frames = [&](const std::vector<Frame>& fr, std::vector<std::vector<uint8_t>>& images,
-                                 std::vector<kt_geopixel>* geo_out, kt_stats* st) {
+        auto to_kt_frames = [&](const std::vector<Frame>& fr) {
             std::vector<kt_frame> kf;
             for (const Frame& f : fr) {
                 kthip::KNdSParams_HIP kp{M_bh, f.a, arg_Q, arg_Lam, 0.0, 0.0, f.disk_out};
@@ -301,6 +300,11 @@ int main(int argc, char** argv) {
                 k.theta_deg = f.theta; k.phi_deg = f.phi; k.fov_deg = arg_fov;  // degrees as given
                 kf.push_back(k);
             }
+            return kf;
+        };
+        auto render_frames = [&](const std::vector<Frame>& fr, std::vector<std::vector<uint8_t>>& images,
+                                 std::vector<kt_geopixel>* geo_out, kt_stats* st) {
+            std::vector<kt_frame> kf = to_kt_frames(fr);
             if (gpu_fp64 && !ctx.has_fp64())
                 throw std::runtime_error("HIP FP64 strict mode requested, but the device does not expose native FP64.");
             const kt_opts o = kthip::make_opts(chart, ro, fp64);
@@ -429,6 +433,20 @@ int main(int argc, char** argv) {
         }
         const double t_total = now_s();
         int rendered = 0;
+        // Without camera-spp every batch goes through one plan: its render is captured in a HIP
+        // graph on the first batch (after autotuning the launch on it) and replayed for the others.
+        kt_hip_plan* plan = nullptr;
+        struct PlanGuard { kt_hip_plan*& p; ~PlanGuard() { if (p) kt_hip_plan_destroy(p); } } plan_guard{plan};
+        if (camera_spp <= 1 && !todo.empty()) {
+            if (gpu_fp64 && !ctx.has_fp64())
+                throw std::runtime_error("HIP FP64 strict mode requested, but the device does not expose native FP64.");
+            kt_background bg{ro.bg_rgb, ro.bg_w, ro.bg_h};
+            ctx.check(kt_hip_set_background(ctx.get(), ro.bg_rgb ? &bg : nullptr), "kt_hip_set_background");
+            const kt_opts o = kthip::make_opts(chart, ro, fp64);
+            const int nb = (int)std::min(todo.size(), (size_t)anim_batch);
+            ctx.check(kt_hip_plan_create(ctx.get(), &o, nb, (int64_t)nb * W * H, &plan), "kt_hip_plan_create");
+            std::cout << "Plan: " << nb << " frames per graph launch\n";
+        }
         for (size_t b = 0; b < todo.size(); b += (size_t)anim_batch) {
             const size_t e = std::min(todo.size(), b + (size_t)anim_batch);
             std::vector<Frame> fr;
@@ -436,7 +454,17 @@ int main(int argc, char** argv) {
             std::vector<std::vector<uint8_t>> images;
             kt_stats st{};
             const double t0 = now_s();
-            render_frames(fr, images, nullptr, &st);
+            if (plan) {
+                const std::vector<kt_frame> kf = to_kt_frames(fr);
+                const size_t npix = (size_t)W * H;
+                std::vector<uint32_t> px(npix * kf.size());
+                ctx.check(kt_hip_plan_render(plan, kf.data(), (int)kf.size(), px.data(), nullptr, nullptr, &st),
+                          "kt_hip_plan_render");
+                images.assign(fr.size(), std::vector<uint8_t>(npix * 3));
+                for (size_t f = 0; f < fr.size(); ++f) kthip::abgr_to_rgb(px.data() + f * npix, npix, images[f].data());
+            } else {
+                render_frames(fr, images, nullptr, &st);
+            }
             for (size_t j = b; j < e; ++j) {
                 char fname[1024];
                 std::snprintf(fname, sizeof(fname), "%s/frame_%05d.png", frames_dir.c_str(), todo[j].first);

@@ -108,6 +108,8 @@ IMAGES = {
     "ks_interstellar_rgb_160x90": "--ks --custom-res 160 90 --a 0.5 --theta 80 --phi 0 --r-obs 40 --fov 45 "
                                   "--disk-out 12 --disk-interstellar --disk-rings 5 --disk-sectors 9 "
                                   "--disk-sigma 0.8 --disk-hue-offset 0.3",
+    # Jacobi ray bundles: the disk flux is weighted by clamp(1/magnif, 0.05, 5) (main.cpp:1799-1814)
+    "bundles_rgb_64x36": "--bl --bundles --custom-res 64 36 --a 0.99 --theta 89 --r-obs 40 --disk-out 12",
 }
 
 

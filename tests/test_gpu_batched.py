@@ -132,10 +132,10 @@ def test_c4_batched_graph_vs_oracle(gpu_ctx):
         plan.close()
     assert out["stats"]["rays"] == npix
     assert tune["candidates"] == 13
-    # per-pixel parity on 6 frames, 300 seeded pixels each
+    # per-pixel parity on 12 frames, 2000 seeded pixels each (fewer for the smallest frames)
     rng = np.random.default_rng(7)
     offs = np.cumsum([0] + [f.width * f.height for f in frames])
-    picks = [0, 9, 21, 33, 47, 63]
+    picks = [0, 5, 9, 14, 21, 27, 33, 40, 47, 52, 58, 63]
     f64 = gpu_ctx.render([frames[i] for i in picks], _opts(rtol=1e-9, atol=1e-12), want_rgba=False)["geo"]
     foff = np.cumsum([0] + [frames[i].width * frames[i].height for i in picks])
     worst_dg, agree = 0.0, []
@@ -143,7 +143,7 @@ def test_c4_batched_graph_vs_oracle(gpu_ctx):
         p = params[i]
         W, H = p["width"], p["height"]
         band = photon_ring_band(f64[foff[j]:foff[j + 1]], W, H)
-        idx = rng.choice(W * H, size=300, replace=False)
+        idx = rng.choice(W * H, size=min(2000, W * H // 2), replace=False)
         px, py = (idx % W).astype(np.int32), (idx // W).astype(np.int32)
         fo = kto.make_frame(W, H, a=p["a"], Q=p["Q"], Lambda=p["Lambda"], theta=p["theta"], r_obs=p["r_obs"],
                             fov=p["fov"], disk_out=p["disk_out"], chart="ks", integrator="dopri5", tol=1e-9)
@@ -155,7 +155,7 @@ def test_c4_batched_graph_vs_oracle(gpu_ctx):
         if d.any():
             worst_dg = max(worst_dg, float(np.abs(got["redshift"][d] - ref["redshift"][d]).max()))
     print("C4 sample fate agreement", agree, "max |dg|", worst_dg, flush=True)
-    assert min(agree) >= 0.99 and np.mean(agree) >= 0.999
+    assert min(agree) >= 0.999
     assert worst_dg <= 1e-4
     # a second launch of the same graph reproduces the frame scalars; per-pixel results of the
     # mixed path may move within tolerance (binary64 islands are chosen per wave)

@@ -137,3 +137,15 @@ def test_oracle_spp_resolve_matches_reference_png():
         passes.append(kto.colorize(geo, 1.0, kto.frame_info(fr).r_isco, bg=bg))
     got = kto.spp_resolve(np.stack(passes))
     assert np.array_equal(got, want)
+
+
+def test_bundle_colorize_vs_reference_png():
+    """Phase 2 in bundle mode: the oracle's shading of the reference's bundle GeoPixels (flux
+    weighted by clamp(1/magnif, 0.05, 5), main.cpp:1799-1814) against the reference's own PNG of the
+    same frame, bit for bit."""
+    from tests.golden.make_golden import checker
+    g = load_golden("bundles_bl_64x36")
+    want = load_golden("bundles_rgb_64x36")["rgb"].reshape(-1, 3)
+    got = kto.colorize(g["pixels"], 1.0, float(g["meta"][4]), bg=checker())
+    assert np.array_equal(got, want)
+    assert (g["pixels"]["magnif"][g["pixels"]["outcome"] == 1] != 1.0).all()
