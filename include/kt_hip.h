@@ -106,7 +106,10 @@ typedef struct kt_opts {
     int32_t palette;        /* KT_PALETTE_*: disk colouring (ColorParams::palette, main.cpp:70-81) */
     int32_t disk_rings;     /* Interstellar palette: rings x sectors cells (main.cpp:76-77)   */
     int32_t disk_sectors;
-    int32_t pad2;
+    int32_t mixed_hold;     /* MIXED: > 0: a rejected binary32 attempt is retried at the same step
+                               size in binary64 and the ray stays in binary64 for this many
+                               accepted steps (fewer attempts, far fewer binary32 steps);
+                               0 or < 0: no retry (the built-in policy)                     */
     double disk_sigma;      /* Interstellar Gaussian blend width in cells (main.cpp:78).  With
                                s = ceil(3 sigma) + 1, (rings + 2s + 1)(sectors + 2s + 1) must
                                stay within 65536 cells per pixel, else KT_E_ARG            */

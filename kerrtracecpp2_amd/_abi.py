@@ -41,7 +41,7 @@ class KtOpts(ctypes.Structure):
                 ("tuning_chunk", ctypes.c_int32), ("tuning_priority_steps", ctypes.c_int32),
                 ("mixed_noise", ctypes.c_double), ("mixed_r_island", ctypes.c_double),
                 ("palette", ctypes.c_int32), ("disk_rings", ctypes.c_int32), ("disk_sectors", ctypes.c_int32),
-                ("pad2", ctypes.c_int32), ("disk_sigma", ctypes.c_double), ("disk_hue_offset", ctypes.c_double)]
+                ("mixed_hold", ctypes.c_int32), ("disk_sigma", ctypes.c_double), ("disk_hue_offset", ctypes.c_double)]
 
 
 class KtBackground(ctypes.Structure):

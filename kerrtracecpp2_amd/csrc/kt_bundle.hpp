@@ -145,7 +145,7 @@ KT_HD void bundle_init(const FrameDev& F, const RunOpts& O, int px, int py, Bund
     kt_sincos(S.y[1], &S.sc.s, &S.sc.c);
     bundle_rhs(F.P, S.y, S.sc, S.pt, S.pp, B.W, S.k1, B.dW);
     S.n_acc = 0; S.n_rej = 0;
-    S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0;
+    S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0; S.hold = 0;
 }
 
 // One RK4-doubling attempt of the bundle + the reference's event logic.

@@ -221,8 +221,11 @@ enum { CHART_BL = 0, CHART_KS = 1 };
 
 // T: type of the state variables (double, float, or the dual numbers of the
 // bundle kernel); S: type of the spacetime constants and the conserved p_t, p_phi.
-template <int CHART, class T, class S = T>
-KT_HD void rhs(const Knds<S>& P, T r, const SinCos<T>& sc, T pr, T pth, S pt, S pp, T out[5]) {
+// SCALED: every output multiplied by hscale (the integrator's step size), folded into the
+// final scaling by 1/Sigma, so that stage derivatives come out as h k at the cost of one
+// multiply instead of one per component.
+template <int CHART, class T, class S = T, bool SCALED = false>
+KT_HD void rhs_impl(const Knds<S>& P, T r, const SinCos<T>& sc, T pr, T pth, S pt, S pp, T hscale, T out[5]) {
     const T s = sc.s, c = sc.c;
     const T s2 = s*s, c2 = c*c, sc2 = s*c;
     const T r2 = r*r;
@@ -243,38 +246,51 @@ KT_HD void rhs(const Knds<S>& P, T r, const SinCos<T>& sc, T pr, T pth, S pt, S 
         const T p12 = sig*dR;
         const T inv = kt_rcp(p12*dTs2);
         const T inv_sig = inv*dR*dTs2;
+        const T inv_out = SCALED ? inv_sig*hscale : inv_sig;
         const T inv_dR = inv*sig*dTs2;
         const T inv_ts = inv*p12;
         const T A1 = P.xi2*K*inv_dR;                          // Xi^2 K / Dr
         const T B1 = P.xi2*L*inv_ts;                          // Xi^2 L / (Dt s^2)
         const T N = fma(dR*pr, pr, fma(dT*pth, pth, fma(-A1, K, B1*L)));
-        out[0] = dR*pr*inv_sig;
-        out[1] = dT*pth*inv_sig;
-        out[4] = fma(-A1, P.a, B1)*inv_sig;
+        out[0] = dR*pr*inv_out;
+        out[1] = dT*pth*inv_out;
+        out[4] = fma(-A1, P.a, B1)*inv_out;
         // dN/dr = dR_r pr^2 - 2 A1 Kr + A1 K dR_r / Dr
         const T dNr = fma(dR_r*pr, pr, fma(S(-2)*A1, Kr, A1*K*inv_dR*dR_r));
         // dN/dth = dT_th pth^2 + 2 B1 Lth - B1 L (dT_th s^2 + 2 Dt s c)/(Dt s^2)
         const T dNth = fma(dT_th*pth, pth, fma(S(2)*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, S(2)*dT*sc2)));
-        const T hs = S(0.5)*inv_sig;
+        const T hs = S(0.5)*inv_out;
         out[2] = -fma(-N*(S(2)*r), inv_sig, dNr)*hs;
         out[3] = -fma(N*(S(2)*P.a2)*sc2, inv_sig, dNth)*hs;
     } else {
         const T inv = kt_rcp(sig*dTs2);
         const T inv_sig = inv*dTs2;
+        const T inv_out = SCALED ? inv_sig*hscale : inv_sig;
         const T inv_ts = inv*sig;
         const T B1 = P.xi2*L*inv_ts;                          // Xi^2 L / (Dt s^2)
         const T XK = P.xi*K;
         const T N = fma(dR*pr, pr, fma(S(2)*XK, pr, fma(dT*pth, pth, B1*L)));
-        out[0] = fma(dR, pr, XK)*inv_sig;
-        out[1] = dT*pth*inv_sig;
-        out[4] = fma(P.xi*P.a, pr, B1)*inv_sig;
+        out[0] = fma(dR, pr, XK)*inv_out;
+        out[1] = dT*pth*inv_out;
+        out[4] = fma(P.xi*P.a, pr, B1)*inv_out;
         // dN/dr = dR_r pr^2 + 2 Xi pr Kr
         const T dNr = fma(dR_r*pr, pr, (S(2)*P.xi)*pr*Kr);
         const T dNth = fma(dT_th*pth, pth, fma(S(2)*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, S(2)*dT*sc2)));
-        const T hs = S(0.5)*inv_sig;
+        const T hs = S(0.5)*inv_out;
         out[2] = -fma(-N*(S(2)*r), inv_sig, dNr)*hs;
         out[3] = -fma(N*(S(2)*P.a2)*sc2, inv_sig, dNth)*hs;
     }
+}
+
+template <int CHART, class T, class S = T>
+KT_HD void rhs(const Knds<S>& P, T r, const SinCos<T>& sc, T pr, T pth, S pt, S pp, T out[5]) {
+    rhs_impl<CHART, T, S, false>(P, r, sc, pr, pth, pt, pp, T(), out);   // scale unused
+}
+
+// the right-hand side times h: (h dr, h dtheta, h dp_r, h dp_theta, h dphi)
+template <int CHART, class T, class S = T>
+KT_HD void rhs_h(const Knds<S>& P, T r, const SinCos<T>& sc, T pr, T pth, S pt, S pp, T h, T out[5]) {
+    rhs_impl<CHART, T, S, true>(P, r, sc, pr, pth, pt, pp, h, out);
 }
 
 // 2 Sigma H (the numerator N above), for constraint monitoring / renormalisation

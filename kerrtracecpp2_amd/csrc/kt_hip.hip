@@ -396,7 +396,7 @@ struct GeodesicPolicy {
         for (int i = 0; i < 5; ++i) S.k1[i] = 0.0;
         S.pt = 0.0; S.pp = 0.0;
         S.h = 1.0;
-        S.n_acc = 0; S.n_rej = 0;
+        S.n_acc = 0; S.n_rej = 0; S.hold = 0;
     }
     template <bool SINGLE>
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
@@ -1267,6 +1267,8 @@ static RunOpts run_opts(const kt_opts* o) {
     R.atol = o->atol < 1.1754943508222875e-38 ? 1.1754943508222875e-38 : o->atol;   // FLT_MIN
     R.isl_noise = o->mixed_noise > 0.0 ? o->mixed_noise : kMixedNoiseDefault;
     R.isl_r = o->mixed_r_island > 0.0 ? o->mixed_r_island : kMixedRIslandDefault;
+    R.isl_hold = o->mixed_hold > 0 ? o->mixed_hold : kMixedHoldDefault;
+    R.pad = 0;
     return R;
 }
 

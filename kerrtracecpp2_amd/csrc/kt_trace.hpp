@@ -57,6 +57,7 @@ struct FrameDev {
 // default island parameters of the mixed-precision path
 static constexpr double kMixedNoiseDefault = 1e-9;
 static constexpr double kMixedRIslandDefault = 2.5;
+static constexpr int kMixedHoldDefault = 0;   // retry off unless asked for
 
 struct RunOpts {
     int32_t max_steps;
@@ -64,6 +65,9 @@ struct RunOpts {
     double step_init, rtol, atol;
     double isl_noise;      // mixed: FP32 stage noise coefficient for the island test
     double isl_r;          // mixed: binary64 below r < isl_r * r_horizon
+    int32_t isl_hold = 0;  // mixed: accepted binary64 steps owed after a binary32 rejection
+                           // (which is retried at the same step size); 0: no retry
+    int32_t pad = 0;
 };
 
 // result fields of one ray before GeoPixel narrowing (TraceResult, main.cpp:187-191)
@@ -348,6 +352,7 @@ struct RayState {
                            // per lane in registers, not re-read from the frame table
     int32_t n_acc, n_rej;
     int32_t rhs32, rhs64, hermites, rej_total;
+    int32_t hold;          // mixed: binary64 steps still owed after a binary32 rejection
 };
 
 // Wave-level vote policy: the kernel passes one backed by __ballot, the host check
@@ -429,6 +434,25 @@ KT_HD void stage_given(const Knds<double>& P, const Knds<float>& Pf, bool use64,
         rhs<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
                    ptf, ppf, kf);
         for (int i = 0; i < NC; ++i) k[i] = (double)kf[i];
+    }
+}
+
+// stage derivative times h at y0 + d with the sin/cos of theta given (integrated stage values)
+template <int CHART, int PREC, int NC>
+KT_HD void stage_given_h(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
+                         const double d[4], const SinCos<double>& sc, double pt, double pp, double h, double hk[NC]) {
+    if (PREC == PREC_FP64 || use64) {
+        double o[5];
+        rhs_h<CHART>(P, y0[0] + d[0], sc, y0[2] + d[2], y0[3] + d[3], pt, pp, h, o);
+        for (int i = 0; i < NC; ++i) hk[i] = o[i];
+    } else {
+        const SinCos<float> scf = {(float)sc.s, (float)sc.c};
+        float kf[5];
+        float ptf = (float)pt, ppf = (float)pp;
+        opaque_f32(ptf, ppf);
+        rhs_h<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
+                     ptf, ppf, (float)h, kf);
+        for (int i = 0; i < NC; ++i) hk[i] = (double)kf[i];
     }
 }
 
@@ -579,9 +603,19 @@ KT_HD int after_accept(const FrameDev& F, const Knds<double>& P, const RunOpts& 
     // binary32 steps only: a step the wave took in binary64 drifts off H = 0 no more than
     // the binary64 path, which projects nothing
     if (PREC == PREC_MIXED && !step64) renormalise<CHART>(P, S);
+    if (PREC == PREC_MIXED && step64 && S.hold > 0) S.hold--;
     if (step_events<CHART>(F, r0, th0, ph0, dr0, dth0, S, hu, O.quirks, o, S.hermites)) return STEP_DONE;
     if (S.n_acc >= O.max_steps) { stopped_result<CHART>(S, O.quirks, o); return STEP_DONE; }
     return STEP_ACCEPTED;
+}
+
+// Mixed precision: a rejected binary32 attempt says little about the step size (its error
+// estimate may be binary32 stage noise, which does not shrink with h as the truncation error
+// does), so the same step is retried in binary64 and the lane stays in binary64 for
+// O.isl_hold accepted steps, instead of shrinking h through a cascade of noisy rejections.
+template <int PREC>
+KT_HD void mixed_retry(const RunOpts& O, bool use64, double h, RayState& S) {
+    if (PREC == PREC_MIXED && !use64 && O.isl_hold > 0) { S.h = h; S.hold = O.isl_hold; }
 }
 
 KT_HD int after_reject(const RunOpts& O, RayState& S) {
@@ -693,6 +727,7 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     if (!isfinite(err2)) {
         const double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
         S.h = hn < 1e-10 ? 1e-10 : hn;
+        mixed_retry<PREC>(O, use64, h, S);
         return after_reject(O, S);
     }
     if (err2 <= 1.0 || h < 1e-10) {
@@ -710,6 +745,110 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     double hn = h * 0.9 * pow_neg(err2, 0.125f);
     if (!isfinite(hn)) hn = h*0.5;
     S.h = hn < 1e-10 ? 1e-10 : hn;
+    mixed_retry<PREC>(O, use64, h, S);
+    return after_reject(O, S);
+}
+
+// DOPRI5 attempt of the trace kernels: the dp5_step arithmetic with integrated stage sin/cos,
+// stages 2-6 carried as h k_i (h folded into each right-hand side's final scaling) and the
+// k1 terms taking their tableau coefficient times h, so that no stage combination, 5th-order
+// sum or error sum needs a final multiply by h per component.  k1 and the FSAL stage stay
+// unscaled: they are the event derivatives and the next attempt's (or a retry's) k1.
+template <int CHART, int PREC, int NC>
+KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
+                     RayState& S, RayOut& o) {
+    typedef DP5 C;
+    const double h = S.h;
+    const double* y = S.y;
+    const double* k1 = S.k1;                    // f(y), kept for a rejected attempt
+    double k2[NC], k3[NC], k4[NC];              // h k_i
+    double d[4], a5[NC], ae[4];
+    // stage sin/cos and their h-scaled derivatives u = cos h theta', v = -sin h theta'
+    const double s1 = S.sc.s, c1 = S.sc.c;
+    const double hth1 = h*k1[1];
+    const double u1 = c1*hth1, v1 = -s1*hth1;
+    double u2, v2, u3, v3, u4, v4;
+    SinCos<double> sc;
+    // the k1 terms take the tableau coefficient times h (one multiply per coefficient)
+    double ch = C::a21*h;
+    for (int i = 0; i < 4; ++i) d[i] = ch*k1[i];
+    sc.s = fma(C::a21, u1, s1); sc.c = fma(C::a21, v1, c1);
+    stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k2);
+    u2 = sc.c*k2[1]; v2 = -sc.s*k2[1];
+    ch = C::a31*h;
+    for (int i = 0; i < 4; ++i) d[i] = fma(ch, k1[i], C::a32*k2[i]);
+    sc.s = s1 + (C::a31*u1 + C::a32*u2); sc.c = c1 + (C::a31*v1 + C::a32*v2);
+    stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k3);
+    u3 = sc.c*k3[1]; v3 = -sc.s*k3[1];
+    ch = C::a41*h;
+    for (int i = 0; i < 4; ++i) d[i] = fma(ch, k1[i], C::a42*k2[i] + C::a43*k3[i]);
+    sc.s = s1 + (C::a41*u1 + C::a42*u2 + C::a43*u3); sc.c = c1 + (C::a41*v1 + C::a42*v2 + C::a43*v3);
+    stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k4);
+    u4 = sc.c*k4[1]; v4 = -sc.s*k4[1];
+    {
+        double k5[NC];
+        ch = C::a51*h;
+        for (int i = 0; i < 4; ++i) d[i] = fma(ch, k1[i], C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
+        sc.s = s1 + (C::a51*u1 + C::a52*u2 + C::a53*u3 + C::a54*u4);
+        sc.c = c1 + (C::a51*v1 + C::a52*v2 + C::a53*v3 + C::a54*v4);
+        stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k5);
+        const double u5 = sc.c*k5[1], v5 = -sc.s*k5[1];
+        sc.s = s1 + (C::a61*u1 + C::a62*u2 + C::a63*u3 + C::a64*u4 + C::a65*u5);
+        sc.c = c1 + (C::a61*v1 + C::a62*v2 + C::a63*v3 + C::a64*v4 + C::a65*v5);
+        ch = C::a61*h;
+        for (int i = 0; i < 4; ++i)
+            d[i] = fma(ch, k1[i], C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
+        ch = C::b1*h;
+        for (int i = 0; i < NC; ++i) a5[i] = fma(ch, k1[i], C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i]);
+        ch = C::e1*h;
+        for (int i = 0; i < 4; ++i) ae[i] = fma(ch, k1[i], C::e3*k3[i] + C::e4*k4[i] + C::e5*k5[i]);
+    }
+    {
+        double k6[NC];
+        stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k6);
+        for (int i = 0; i < NC; ++i) a5[i] += C::b6*k6[i];
+        for (int i = 0; i < 4; ++i) ae[i] += C::e6*k6[i];
+    }
+    for (int i = 0; i < 4; ++i) d[i] = a5[i];
+    // FSAL stage at y5 in the stage precision, unscaled; its sin/cos becomes the next base
+    double k7[5];
+    SinCos<double> sc7;
+    if (PREC == PREC_FP64 || use64) {
+        stage_f64<CHART, 5>(P, y, d, S.sc, S.pt, S.pp, k7, sc7);
+    } else {
+        stage_f32<CHART, 5>(Pf, y, d, S.sc, S.pt, S.pp, k7);
+        sc7 = sc_at(S.sc, y[1], d[1]);
+    }
+    count_rhs<PREC>(S, use64, 6);      // stages 2-6 and the FSAL stage
+    // err^2 = mean_i (dy_i / (atol + rtol |y_i|))^2 in binary64 (as dp5_step)
+    const double e7h = C::e7*h;
+    double e2 = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        const double q = fma(e7h, k7[i], ae[i]) * kt_rcp(O.atol + O.rtol*fabs(y[i]));
+        e2 = fma(q, q, e2);
+    }
+    const double err2 = e2 * 0.25;
+    if (!isfinite(err2)) {
+        const double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
+        S.h = hn < 1e-10 ? 1e-10 : hn;
+        mixed_retry<PREC>(O, use64, h, S);
+        return after_reject(O, S);
+    }
+    if (err2 <= 1.0 || h < 1e-10) {
+        const double r0 = y[0], th0 = y[1], ph0 = S.ph, dr0 = S.k1[0], dth0 = S.k1[1];
+        if (NC == 5) S.ph += a5[NC - 1];
+        for (int i = 0; i < 4; ++i) S.y[i] += d[i];
+        for (int i = 0; i < 5; ++i) S.k1[i] = k7[i];
+        S.sc = sc7;
+        double hn = h * ((err2 > 1e-24) ? 0.9*pow_neg(err2, 0.1f) : 4.0);
+        if (!isfinite(hn)) hn = h;
+        S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
+        return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o, use64);
+    }
+    double hn = h * 0.9 * pow_neg(err2, 0.125f);
+    if (!isfinite(hn)) hn = h*0.5;
+    S.h = hn < 1e-10 ? 1e-10 : hn;
+    mixed_retry<PREC>(O, use64, h, S);
     return after_reject(O, S);
 }
 
@@ -754,6 +893,7 @@ KT_HD int rk4_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     if (!isfinite(err)) {
         const double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
         S.h = hn < 1e-10 ? 1e-10 : hn;
+        mixed_retry<PREC>(O, use64, h, S);
         return after_reject(O, S);
     }
     if (err < tol || h < 1e-10) {
@@ -776,6 +916,7 @@ KT_HD int rk4_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     if (!isfinite(hn)) hn = half;
     if (hn > half) hn = half;
     S.h = hn < 1e-10 ? 1e-10 : hn;
+    mixed_retry<PREC>(O, use64, h, S);
     return after_reject(O, S);
 }
 
@@ -793,6 +934,7 @@ KT_HD void ray_init(const FrameDev& F, const RunOpts& O, int px, int py, RayStat
     S.r_cut = F.r_cut; S.r_esc = F.r_esc;
     S.n_acc = 0; S.n_rej = 0;
     S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0;
+    S.hold = 0;
 }
 
 // Initial state of a ray as the init kernel leaves it in HBM: one 128-byte record per
@@ -821,6 +963,7 @@ KT_HD void ray_load_init(const RayInit& R, const RunOpts& O, RayState& S) {
     S.h = O.step_init < 1e-10 ? 1e-10 : O.step_init;
     S.n_acc = 0; S.n_rej = 0;
     S.rhs32 = 0; S.rhs64 = 1; S.hermites = 0; S.rej_total = 0;
+    S.hold = 0;
 }
 
 // Does this lane need a binary64 step in the mixed-precision path?  The FP32
@@ -828,6 +971,7 @@ KT_HD void ray_load_init(const RayInit& R, const RunOpts& O, RayState& S) {
 // well below the error scale atol + rtol |y|; near the horizon and the polar
 // axis the FP32 Hamiltonian loses too many digits.
 KT_HD bool wants_fp64(const FrameDev& F, const RunOpts& O, const RayState& S) {
+    if (S.hold > 0) return true;
     if (S.y[0] < O.isl_r * F.r_h) return true;
     if (S.sc.s*S.sc.s < 1e-3) return true;
     for (int i = 0; i < 4; ++i) {
@@ -863,7 +1007,10 @@ template <int CHART, int METHOD, int PREC, bool PHI, bool SCI = true>
 KT_HD bool ray_advance(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O,
                        bool use64, RayState& S, RayOut& o) {
     int rc;
-    if (METHOD == METHOD_DP5) rc = dp5_step<CHART, PREC, PHI ? 5 : 4, SCI>(F, P, Pf, O, use64, S, o);
+    if (METHOD == METHOD_DP5) {
+        if constexpr (SCI) rc = dp5_step_h<CHART, PREC, PHI ? 5 : 4>(F, P, Pf, O, use64, S, o);
+        else rc = dp5_step<CHART, PREC, PHI ? 5 : 4, false>(F, P, Pf, O, use64, S, o);
+    }
     else rc = rk4_step<CHART, PREC>(F, P, Pf, O, use64, S, o);
     if (rc != STEP_DONE) return false;
     if (S.n_rej > 64 || !isfinite(S.h)) stopped_result<CHART>(S, O.quirks, o);   // reject cap

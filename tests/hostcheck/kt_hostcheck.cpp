@@ -83,6 +83,7 @@ extern "C" int ktc_trace_pixels(const kt_frame* f, const kt_opts* o, const int32
     O.rtol = o->rtol; O.atol = o->atol;
     O.isl_noise = o->mixed_noise > 0.0 ? o->mixed_noise : kMixedNoiseDefault;
     O.isl_r = o->mixed_r_island > 0.0 ? o->mixed_r_island : kMixedRIslandDefault;
+    O.isl_hold = o->mixed_hold > 0 ? o->mixed_hold : kMixedHoldDefault;
     if (info5) { info5[0] = F.r_h; info5[1] = F.r_isco; info5[2] = F.r_esc; info5[3] = F.chart; info5[4] = F.ks_ref_init; }
     const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
     if (F.chart == CHART_KS) run_chart<CHART_KS>(o->integrator, o->precision, phi, F, O, px, py, n, out, acc, rhs32, rhs64);

@@ -126,6 +126,25 @@ def test_c2_ks_lambda_vs_reference(gpu_ctx, prec):
         assert st["rhs_fp32"] + st["rhs_fp64"] > 0
 
 
+@pytest.mark.parametrize("hold", [2, 8])
+def test_c2_mixed_binary64_retry(gpu_ctx, hold):
+    """Mixed precision with kt_opts.mixed_hold: a rejected binary32 attempt is retried at the same
+    step size in binary64 and the ray stays in binary64 for `hold` accepted steps.  Same parity bars
+    against the reference; fewer rejected attempts and fewer binary32 stages than without it."""
+    g = load_golden("c2_ks_dopri5_lambda_160x90")
+    W, H = 160, 90
+    fr = kt.make_frame(W, H, a=0.9, Q=0.3, Lambda=1e-5, theta=85, r_obs=40, disk_out=12)
+    base = gpu_ctx.render([fr], _opts(chart="ks", method="dopri5", prec="mixed", rtol=1e-9, atol=1e-12))["stats"]
+    out = gpu_ctx.render([fr], _opts(chart="ks", method="dopri5", prec="mixed", rtol=1e-9, atol=1e-12,
+                                     mixed_hold=hold))
+    _check(g["pixels"], out["geo"], W, H, dr_tol=1e-3)
+    st = out["stats"]
+    print("hold", hold, "rejected", st["steps_rejected"], "vs", base["steps_rejected"], "binary32 stages",
+          st["rhs_fp32"], "vs", base["rhs_fp32"], flush=True)
+    assert st["steps_rejected"] < base["steps_rejected"]
+    assert 0 < st["rhs_fp32"] < base["rhs_fp32"]
+
+
 @pytest.mark.parametrize("name,a", [("ks_rk4_spin_pos_160x90", 0.5), ("ks_rk4_spin_neg_160x90", -0.5)])
 def test_ks_lambda0_vs_reference(gpu_ctx, name, a):
     g = load_golden(name)
