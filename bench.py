@@ -50,6 +50,24 @@ F_ROT = 31
 F_HERMITE = 370
 PEAK_VALU_TFLOPS = 78.6   # MI355X vector FP64; FP32 without packed math issues at the same rate
 
+# Adjoint reverse pass (kt_adjoint_backward_kernel, KS chart), per recorded accepted step:
+#   segment replay: 6 stages x (right-hand side + sin/cos rotation) + combinations (stage sums
+#   120, 5th-order sum 36, update 8, stage inputs 18 = 182);  the step's stage points again
+#   (same count);  6 reverse-mode vector-Jacobian products (F_VJP, counted by
+#   tests/hostcheck/flopcount.cpp) + the adjoint stage combinations (168 + 24 + 30 = 222).
+F_VJP = {"bl": 367, "ks": 348}
+F_ADJ_STEP = 2 * (6 * (F_RHS["ks"] + F_ROT) + 182) + 6 * F_VJP["ks"] + 222
+# Jacobi bundles (kt_bundle_kernel, BL, RK4 doubling), per attempt: the full step (3 plain
+# right-hand sides + rotations + sums: 3 x (98 + 31 + 3) + 54), the two half steps (3 stages
+# each of one dual-number right-hand side F_DUAL_BL (counted by flopcount.cpp) + 4 for its sin/cos
+# tangents + rotation + state/W updates 24 + stage sums 26, then 14 for the increments), the
+# derivative at the midpoint (F_DUAL_BL + 4 + 31 + 12) and the error norm (22); an accepted
+# attempt adds the derivative at the new point (F_DUAL_BL + 4 + 31 + 21).
+F_DUAL_BL = 459
+F_BUNDLE_ATTEMPT = 3 * (F_RHS["bl"] + F_ROT + 3) + 54 + 2 * (3 * (F_DUAL_BL + 4 + F_ROT + 24 + 26) + 14) \
+    + (F_DUAL_BL + 4 + F_ROT + 12) + 22
+F_BUNDLE_ACCEPT = F_DUAL_BL + 4 + F_ROT + 21
+
 
 def flops_of(st, chart="ks"):
     attempts = st["steps_accepted"] + st["steps_rejected"]
@@ -103,10 +121,16 @@ def c5_run(ctx, kt, reps):
     outs = [ctx.render_adjoint(fr, o) for _ in range(reps)]
     wall = (time.perf_counter() - t0) / reps
     st = outs[-1]["stats"]
+    bw_ms = float(np.mean([o["stats"]["backward_ms"] for o in outs]))
+    fl = st["steps"] * F_ADJ_STEP
+    ach = fl / (bw_ms * 1e-3) / 1e12
     return {"workload": "256x256 KN a=0.7 Q=0.2 KS theta=60 DOPRI5, dI/d(a,Q,Lambda,theta,r_obs) per pixel",
             "ms_per_call": 1e3 * wall, "forward_ms": st["forward_ms"], "record_ms": st["record_ms"],
-            "backward_ms": st["backward_ms"], "rays_differentiated": st["rays_differentiated"],
-            "steps_recorded": st["steps"], "checkpoints": st["checkpoints"], "record_bytes": st["record_bytes"]}
+            "backward_ms": bw_ms, "rays_differentiated": st["rays_differentiated"],
+            "steps_recorded": st["steps"], "checkpoints": st["checkpoints"], "record_bytes": st["record_bytes"],
+            "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_VALU_TFLOPS, "kernel": "kt_adjoint_backward_kernel<KS>",
+                         "flops_per_launch": fl, "flops_per_step": F_ADJ_STEP}}
 
 
 def c3_run(ctx, kt, reps):
@@ -124,11 +148,19 @@ def c3_run(ctx, kt, reps):
     t0 = time.perf_counter()
     sts = [ctx.render_device([fr], o) for _ in range(reps)]
     wall = (time.perf_counter() - t0) / reps
+    tr = float(np.mean([s["trace_ms"] for s in sts]))
+    last = sts[-1]
+    fl = ((last["steps_accepted"] + last["steps_rejected"]) * F_BUNDLE_ATTEMPT
+          + last["steps_accepted"] * F_BUNDLE_ACCEPT + last["events"] * F_HERMITE)
+    ach = fl / (tr * 1e-3) / 1e12
     return {"workload": "1920x1080 Kerr a=0.99 BL theta=89 RK4 doubling tol 1e-7, Jacobi bundles, FP64, "
                         "bundle root-skip quirk off",
-            "pixels": W3 * H3, "ms_per_frame": 1e3 * wall, "trace_ms": float(np.mean([s["trace_ms"] for s in sts])),
-            "pixels_per_s": W3 * H3 / wall, "steps_accepted": sts[-1]["steps_accepted"],
-            "steps_rejected": sts[-1]["steps_rejected"]}
+            "pixels": W3 * H3, "ms_per_frame": 1e3 * wall, "trace_ms": tr,
+            "pixels_per_s": W3 * H3 / wall, "steps_accepted": last["steps_accepted"],
+            "steps_rejected": last["steps_rejected"],
+            "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_VALU_TFLOPS, "kernel": "kt_bundle_kernel<single,1>",
+                         "flops_per_launch": fl, "flops_per_attempt": F_BUNDLE_ATTEMPT}}
 
 
 def accuracy(geo, ref, px, py, band_geo):
@@ -240,7 +272,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
-    ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--only", choices=["both", "mixed", "fp64", "c3", "c5"], default="both")
     ap.add_argument("--copy-calibration", action="store_true")
     a = ap.parse_args()
     if a.copy_calibration:
@@ -254,6 +286,11 @@ def main():
     ref_cpu = cpu_reference(threads) if (not a.no_cpu and a.only == "both") else None
     import kerrtracecpp2_amd as kt
     ctx = kt.Context(0)
+    if a.only in ("c3", "c5"):   # profiling runs of the bundle / adjoint kernels, no JSON contract
+        r = c3_run(ctx, kt, a.steps) if a.only == "c3" else c5_run(ctx, kt, a.steps)
+        print(json.dumps({"only": a.only, **r}))
+        ctx.close()
+        return
     if a.only != "both":   # profiling runs: one kernel variant, no JSON contract
         prec = kt.KT_PRECISION_MIXED if a.only == "mixed" else kt.KT_PRECISION_FP64
         wall, st, _, _ = gpu_run(ctx, kt, prec, a.steps, a.warmup)

@@ -19,8 +19,9 @@
 //
 //  with the accepted step sizes held fixed (the discrete adjoint of the taken
 //  steps; the controller's dependence on p enters only at the tolerance level).
-//  Local Jacobians come from the templated right-hand side evaluated in dual
-//  numbers.  The seed differentiates the disk event: the Hermite root alpha of
+//  The products J^T mu come from a hand-written reverse mode of the right-hand side
+//  (rhs_vjp), pinned against the forward-mode Jacobian in dual numbers
+//  (rhs_jacobian).  The seed differentiates the disk event: the Hermite root alpha of
 //  theta(alpha) = pi/2 by the implicit function theorem, r_hit = Hermite r at
 //  alpha, then redshift and intensity.  The differentiable path converges that
 //  root (Q_EXACT_ROOT); the reference's 8 bisections stop 2^-12 short, which
@@ -253,6 +254,198 @@ KT_HD void rhs_jacobian(double M, const double par[3], const double y[4], double
     }
 }
 
+// Vector-Jacobian product of the right-hand side (rhs<CHART>, kt_device.hpp), hand-written
+// reverse mode: g[k] = sum_i mu[i] d out_i / d x_k over x = (r, theta, p_r, p_theta, p_t,
+// p_phi, a, Q, Lambda), the columns of rhs_jacobian, for about three right-hand sides of work
+// instead of nine forward directions.  The forward sweep below restates rhs<CHART>'s
+// intermediates (1/Sigma, 1/Delta_r and 1/(Delta_theta sin^2) taken as such); the reverse
+// sweep visits them in the opposite order.
+//   P: the frame's constants (knds_t of a, Q, Lambda); sc: sin, cos of y[1].
+template <int CHART, class T = double>
+KT_HD void rhs_vjp_sc(const Knds<T>& P, T Q, T Lam, const T y[4], const SinCos<T>& sc, T pt, T pp, const T mu[4],
+                      T g[9]) {
+    const T a = P.a;
+    const T r = y[0], pr = y[2], pth = y[3];
+    const T s = sc.s, c = sc.c;
+    // ---- forward ----
+    const T s2 = s*s, c2 = c*c, sc2 = s*c, r2 = r*r;
+    const T ra = r2 + P.a2;
+    const T sig = fma(P.a2, c2, r2);
+    const T one_m = fma(-P.lam3, r2, 1.0);
+    const T dR = fma(ra, one_m, fma(-P.M2, r, P.Q2));
+    const T dR_r = fma(2.0*r, one_m, fma(-2.0*P.lam3*r, ra, -P.M2));
+    const T dT = fma(P.lam3a2, c2, 1.0);
+    const T dT_th = -2.0*P.lam3a2*sc2;
+    const T K = fma(ra, pt, P.a*pp);
+    const T L = fma(P.a*s2, pt, pp);
+    const T Kr = 2.0*pt*r;
+    const T Lth = 2.0*P.a*pt*sc2;
+    const T dTs2 = dT*s2;
+    const T inv_sig = kt_rcp(sig);
+    const T inv_ts = kt_rcp(dTs2);
+    const T B1 = P.xi2*L*inv_ts;
+    const T W = fma(dT_th, s2, 2.0*dT*sc2);
+    const T dNth = fma(dT_th*pth, pth, fma(2.0*B1, Lth, -B1*L*inv_ts*W));
+    const T hs = 0.5*inv_sig;
+    T N, dNr, T0, inv_dR = 0.0, A1 = 0.0, XK = 0.0;
+    if (CHART == CHART_BL) {
+        inv_dR = kt_rcp(dR);
+        A1 = P.xi2*K*inv_dR;
+        N = fma(dR*pr, pr, fma(dT*pth, pth, fma(-A1, K, B1*L)));
+        dNr = fma(dR_r*pr, pr, fma(-2.0*A1, Kr, A1*K*inv_dR*dR_r));
+        T0 = dR*pr;
+    } else {
+        XK = P.xi*K;
+        N = fma(dR*pr, pr, fma(2.0*XK, pr, fma(dT*pth, pth, B1*L)));
+        dNr = fma(dR_r*pr, pr, 2.0*P.xi*pr*Kr);
+        T0 = fma(dR, pr, XK);
+    }
+    const T T2 = 2.0*r*N*inv_sig - dNr;            // out2 = hs T2
+    const T T3 = 2.0*P.a2*sc2*N*inv_sig + dNth;    // out3 = -hs T3
+    // ---- reverse ----
+    T b_r = 0.0, b_s = 0.0, b_c = 0.0, b_pr = 0.0, b_pth = 0.0, b_pt = 0.0, b_pp = 0.0;
+    T b_a = 0.0, b_a2 = 0.0, b_Q2 = 0.0, b_lam3 = 0.0, b_lam3a2 = 0.0, b_xi = 0.0, b_xi2 = 0.0;
+    T b_inv_sig = 0.0, b_inv_ts = 0.0, b_N = 0.0, b_dR = 0.0, b_dT = 0.0, b_K = 0.0, b_L = 0.0;
+    T b_sc2 = 0.0, b_s2 = 0.0, b_dT_th = 0.0, b_Kr = 0.0, b_Lth = 0.0, b_dR_r = 0.0, b_B1 = 0.0;
+    // out0 = T0 inv_sig, out1 = dT pth inv_sig, out2 = hs T2, out3 = -hs T3
+    const T b_hs = mu[2]*T2 - mu[3]*T3;
+    const T b_T2 = mu[2]*hs, b_T3 = -mu[3]*hs;
+    const T b_T0 = mu[0]*inv_sig;
+    b_inv_sig += mu[0]*T0 + mu[1]*dT*pth + 0.5*b_hs;
+    b_dT += mu[1]*pth*inv_sig;
+    b_pth += mu[1]*dT*inv_sig;
+    // T2 = 2 r N inv_sig - dNr
+    b_N += b_T2*2.0*r*inv_sig;
+    b_r += b_T2*2.0*N*inv_sig;
+    b_inv_sig += b_T2*2.0*r*N;
+    const T b_dNr = -b_T2;
+    // T3 = 2 a2 sc2 N inv_sig + dNth
+    b_a2 += b_T3*2.0*sc2*N*inv_sig;
+    b_sc2 += b_T3*2.0*P.a2*N*inv_sig;
+    b_N += b_T3*2.0*P.a2*sc2*inv_sig;
+    b_inv_sig += b_T3*2.0*P.a2*sc2*N;
+    const T b_dNth = b_T3;
+    // dNth = dT_th pth^2 + 2 B1 Lth - B1 L inv_ts W
+    b_dT_th += b_dNth*pth*pth;
+    b_pth += b_dNth*2.0*dT_th*pth;
+    b_B1 += b_dNth*(2.0*Lth - L*inv_ts*W);
+    b_Lth += b_dNth*2.0*B1;
+    b_L -= b_dNth*B1*inv_ts*W;
+    b_inv_ts -= b_dNth*B1*L*W;
+    const T b_W = -b_dNth*B1*L*inv_ts;
+    // W = dT_th s2 + 2 dT sc2
+    b_dT_th += b_W*s2;
+    b_s2 += b_W*dT_th;
+    b_dT += b_W*2.0*sc2;
+    b_sc2 += b_W*2.0*dT;
+    if (CHART == CHART_BL) {
+        T b_A1 = 0.0, b_inv_dR = 0.0;
+        // T0 = dR pr
+        b_dR += b_T0*pr;
+        b_pr += b_T0*dR;
+        // dNr = dR_r pr^2 - 2 A1 Kr + A1 K inv_dR dR_r
+        b_dR_r += b_dNr*(pr*pr + A1*K*inv_dR);
+        b_pr += b_dNr*2.0*dR_r*pr;
+        b_A1 += b_dNr*(K*inv_dR*dR_r - 2.0*Kr);
+        b_Kr -= b_dNr*2.0*A1;
+        b_K += b_dNr*A1*inv_dR*dR_r;
+        b_inv_dR += b_dNr*A1*K*dR_r;
+        // N = dR pr^2 + dT pth^2 - A1 K + B1 L
+        b_dR += b_N*pr*pr;
+        b_pr += b_N*2.0*dR*pr;
+        b_dT += b_N*pth*pth;
+        b_pth += b_N*2.0*dT*pth;
+        b_A1 -= b_N*K;
+        b_K -= b_N*A1;
+        b_B1 += b_N*L;
+        b_L += b_N*B1;
+        // A1 = xi2 K inv_dR, inv_dR = 1/dR
+        b_xi2 += b_A1*K*inv_dR;
+        b_K += b_A1*P.xi2*inv_dR;
+        b_inv_dR += b_A1*P.xi2*K;
+        b_dR -= b_inv_dR*inv_dR*inv_dR;
+    } else {
+        // T0 = dR pr + XK
+        b_dR += b_T0*pr;
+        b_pr += b_T0*dR;
+        T b_XK = b_T0;
+        // dNr = dR_r pr^2 + 2 xi pr Kr
+        b_dR_r += b_dNr*pr*pr;
+        b_pr += b_dNr*(2.0*dR_r*pr + 2.0*P.xi*Kr);
+        b_xi += b_dNr*2.0*pr*Kr;
+        b_Kr += b_dNr*2.0*P.xi*pr;
+        // N = dR pr^2 + 2 XK pr + dT pth^2 + B1 L
+        b_dR += b_N*pr*pr;
+        b_pr += b_N*(2.0*dR*pr + 2.0*XK);
+        b_XK += b_N*2.0*pr;
+        b_dT += b_N*pth*pth;
+        b_pth += b_N*2.0*dT*pth;
+        b_B1 += b_N*L;
+        b_L += b_N*B1;
+        // XK = xi K
+        b_xi += b_XK*K;
+        b_K += b_XK*P.xi;
+    }
+    // B1 = xi2 L inv_ts
+    b_xi2 += b_B1*L*inv_ts;
+    b_L += b_B1*P.xi2*inv_ts;
+    b_inv_ts += b_B1*P.xi2*L;
+    // inv_ts = 1/dTs2, inv_sig = 1/sig, dTs2 = dT s2
+    const T b_dTs2 = -b_inv_ts*inv_ts*inv_ts;
+    const T b_sig = -b_inv_sig*inv_sig*inv_sig;
+    b_dT += b_dTs2*s2;
+    b_s2 += b_dTs2*dT;
+    // Lth = 2 a pt sc2, Kr = 2 pt r
+    b_a += b_Lth*2.0*pt*sc2;
+    b_pt += b_Lth*2.0*P.a*sc2 + b_Kr*2.0*r;
+    b_sc2 += b_Lth*2.0*P.a*pt;
+    b_r += b_Kr*2.0*pt;
+    // L = a s2 pt + pp, K = ra pt + a pp
+    b_a += b_L*s2*pt + b_K*pp;
+    b_s2 += b_L*P.a*pt;
+    b_pt += b_L*P.a*s2 + b_K*ra;
+    b_pp += b_L + b_K*P.a;
+    T b_ra = b_K*pt;
+    // dT_th = -2 lam3a2 sc2, dT = 1 + lam3a2 c2
+    b_lam3a2 += -2.0*b_dT_th*sc2 + b_dT*c2;
+    b_sc2 -= 2.0*b_dT_th*P.lam3a2;
+    T b_c2 = b_dT*P.lam3a2;
+    // dR_r = 2 r one_m - 2 lam3 r ra - M2, dR = ra one_m - M2 r + Q2
+    b_r += b_dR_r*(2.0*one_m - 2.0*P.lam3*ra) - b_dR*P.M2;
+    T b_one_m = b_dR_r*2.0*r + b_dR*ra;
+    b_lam3 -= b_dR_r*2.0*r*ra;
+    b_ra += -b_dR_r*2.0*P.lam3*r + b_dR*one_m;
+    b_Q2 += b_dR;
+    // one_m = 1 - lam3 r2, sig = a2 c2 + r2, ra = r2 + a2
+    b_lam3 -= b_one_m*r2;
+    T b_r2 = -b_one_m*P.lam3 + b_sig + b_ra;
+    b_a2 += b_sig*c2 + b_ra;
+    b_c2 += b_sig*P.a2;
+    // r2 = r r, sc2 = s c, c2 = c c, s2 = s s
+    b_r += 2.0*r*b_r2;
+    b_s += b_sc2*c + 2.0*s*b_s2;
+    b_c += b_sc2*s + 2.0*c*b_c2;
+    // (s, c) = (sin, cos) theta
+    const T b_th = b_s*c - b_c*s;
+    // constants: a2 = a^2, Q2 = Q^2, lam3 = L/3, lam3a2 = L a^2/3, xi = 1 + L a^2/3, xi2 = xi^2
+    b_xi += 2.0*P.xi*b_xi2;
+    g[0] = b_r; g[1] = b_th; g[2] = b_pr; g[3] = b_pth; g[4] = b_pt; g[5] = b_pp;
+    const T third = 1.0/3.0;
+    g[6] = b_a + 2.0*a*b_a2 + (b_lam3a2 + b_xi)*(2.0*third*Lam*a);
+    g[7] = 2.0*Q*b_Q2;
+    g[8] = (b_lam3 + (b_lam3a2 + b_xi)*(a*a))*third;
+}
+
+// rhs_vjp_sc from (M, a, Q, Lambda) and theta itself
+template <int CHART>
+KT_HD void rhs_vjp(double M, const double par[3], const double y[4], double pt, double pp, const double mu[4],
+                   double g[9]) {
+    const Knds<double> P = knds_t<double>(M, par[0], par[1], par[2]);
+    SinCos<double> sc;
+    kt_sincos(y[1], &sc.s, &sc.c);
+    rhs_vjp_sc<CHART>(P, par[1], par[2], y, sc, pt, pp, mu, g);
+}
+
 // initial state y0 = (r, theta, p_r, p_theta), c = (p_t, p_phi) of a pixel and their
 // derivatives with respect to p (theta_obs in degrees)
 template <int CHART>
@@ -324,9 +517,9 @@ KT_HD void dp5_step_adjoint(double M, const double par[3], const Knds<double>& P
                             const SinCos<double>& sc, double h, double pt, double pp, double lam[4], double gc[2],
                             double gP[3]) {
     typedef DP5 C;
-    // stage points Y_1..Y_6 (forward arithmetic)
+    // stage points Y_1..Y_6 and their sin/cos theta (forward arithmetic)
     double Y[6][4], k[6][4], d[4];
-    SinCos<double> scx;
+    SinCos<double> scs[6];
     const double A[6][6] = {{0, 0, 0, 0, 0, 0},
                             {C::a21, 0, 0, 0, 0, 0},
                             {C::a31, C::a32, 0, 0, 0, 0},
@@ -336,7 +529,7 @@ KT_HD void dp5_step_adjoint(double M, const double par[3], const Knds<double>& P
     const double B[6] = {C::b1, 0.0, C::b3, C::b4, C::b5, C::b6};
     const double z[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = 0; i < 4; ++i) Y[0][i] = y[i];
-    stage_f64<CHART, 4>(P, y, z, sc, pt, pp, k[0], scx);
+    stage_f64<CHART, 4>(P, y, z, sc, pt, pp, k[0], scs[0]);
     for (int s = 1; s < 6; ++s) {
         for (int i = 0; i < 4; ++i) {
             double acc = 0.0;
@@ -344,7 +537,7 @@ KT_HD void dp5_step_adjoint(double M, const double par[3], const Knds<double>& P
             d[i] = h*acc;
             Y[s][i] = y[i] + d[i];
         }
-        stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k[s], scx);
+        stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k[s], scs[s]);
     }
     // reverse over the stages
     double nu[6][4];
@@ -356,21 +549,17 @@ KT_HD void dp5_step_adjoint(double M, const double par[3], const Knds<double>& P
             for (int m = s + 1; m < 6; ++m) acc += A[m][s]*nu[m][i];
             mu[i] = h*acc;
         }
-        double J[4][9];
-        rhs_jacobian<CHART>(M, par, Y[s], pt, pp, J);
+        double gv[9];                       // J^T mu over (y, p_t, p_phi, a, Q, Lambda)
+        rhs_vjp_sc<CHART>(P, par[1], par[2], Y[s], scs[s], pt, pp, mu, gv);
         for (int k2 = 0; k2 < 4; ++k2) {
-            double acc = 0.0;
-            for (int i = 0; i < 4; ++i) acc += J[i][k2]*mu[i];
-            nu[s][k2] = acc;
-            lam_out[k2] += acc;
+            nu[s][k2] = gv[k2];
+            lam_out[k2] += gv[k2];
         }
-        for (int i = 0; i < 4; ++i) {
-            gc[0] += J[i][4]*mu[i];
-            gc[1] += J[i][5]*mu[i];
-            gP[0] += J[i][6]*mu[i];
-            gP[1] += J[i][7]*mu[i];
-            gP[2] += J[i][8]*mu[i];
-        }
+        gc[0] += gv[4];
+        gc[1] += gv[5];
+        gP[0] += gv[6];
+        gP[1] += gv[7];
+        gP[2] += gv[8];
     }
     for (int i = 0; i < 4; ++i) lam[i] = lam_out[i];
 }
@@ -428,17 +617,17 @@ KT_HD bool event_seed(const FrameDev& F, const double y0[4], const double y1[4],
     double e1[4] = {lr*hb[2], la*hb[2], 0.0, 0.0};          // direct on y1
     const double df0[2] = {lr*hb[1]*h, la*hb[1]*h};          // on f0 (r, theta components)
     const double df1[2] = {lr*hb[3]*h, la*hb[3]*h};
-    double J0[4][9], J1[4][9];
-    rhs_jacobian<CHART>(F.M, par, y0, pt, pp, J0);
-    rhs_jacobian<CHART>(F.M, par, y1, pt, pp, J1);
+    // through the end-point derivatives f0, f1 (their r and theta rows): J^T (df, 0, 0)
+    const double m0[4] = {df0[0], df0[1], 0.0, 0.0}, m1[4] = {df1[0], df1[1], 0.0, 0.0};
+    double g0[9], g1[9];
+    rhs_vjp<CHART>(F.M, par, y0, pt, pp, m0, g0);
+    rhs_vjp<CHART>(F.M, par, y1, pt, pp, m1, g1);
     for (int k = 0; k < 4; ++k) {
-        lam0[k] = e0[k] + J0[0][k]*df0[0] + J0[1][k]*df0[1];
-        lam1[k] = e1[k] + J1[0][k]*df1[0] + J1[1][k]*df1[1];
+        lam0[k] = e0[k] + g0[k];
+        lam1[k] = e1[k] + g1[k];
     }
-    for (int k = 0; k < 2; ++k)
-        gc[k] += J0[0][4 + k]*df0[0] + J0[1][4 + k]*df0[1] + J1[0][4 + k]*df1[0] + J1[1][4 + k]*df1[1];
-    for (int k = 0; k < 3; ++k)
-        gP[k] += J0[0][6 + k]*df0[0] + J0[1][6 + k]*df0[1] + J1[0][6 + k]*df1[0] + J1[1][6 + k]*df1[1];
+    for (int k = 0; k < 2; ++k) gc[k] += g0[4 + k] + g1[4 + k];
+    for (int k = 0; k < 3; ++k) gP[k] += g0[6 + k] + g1[6 + k];
     return true;
 }
 

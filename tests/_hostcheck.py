@@ -39,6 +39,7 @@ def lib():
         D = ctypes.c_double
         L.ktc_rhs_jacobian.argtypes = [ctypes.c_int, D, P, P, D, D, P]
         L.ktc_rhs.argtypes = [ctypes.c_int, D, P, P, D, D, P]
+        L.ktc_rhs_vjp.argtypes = [ctypes.c_int, D, P, P, D, D, P, P]
         L.ktc_replay_intensity.argtypes = [P, P, P, P, D, D, P, ctypes.c_int, P, P]
         L.ktc_ray_steps.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P, P]
         L.ktc_event_filters.argtypes = [P, ctypes.c_int64, P]
@@ -126,6 +127,13 @@ def rhs_jacobian(chart, M, par, y, pt, pp):
     J = np.zeros((4, 9))
     lib().ktc_rhs_jacobian(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), pt, pp, _p(J))
     return J
+
+
+def rhs_vjp(chart, M, par, y, pt, pp, mu):
+    g = np.zeros(9)
+    lib().ktc_rhs_vjp(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), pt, pp,
+                      _p(np.asarray(mu, np.float64)), _p(g))
+    return g
 
 
 def rhs(chart, M, par, y, pt, pp):

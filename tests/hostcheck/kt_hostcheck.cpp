@@ -254,6 +254,14 @@ extern "C" int ktc_rhs_jacobian(int chart, double M, const double* par3, const d
     return 0;
 }
 
+// vector-Jacobian product of the right-hand side (reverse mode), g[9] = J^T mu
+extern "C" int ktc_rhs_vjp(int chart, double M, const double* par3, const double* y4, double pt, double pp,
+                           const double* mu4, double* g9) {
+    if (chart == CHART_KS) rhs_vjp<CHART_KS>(M, par3, y4, pt, pp, mu4, g9);
+    else rhs_vjp<CHART_BL>(M, par3, y4, pt, pp, mu4, g9);
+    return 0;
+}
+
 // rhs value at one point with sin/cos from the argument (for finite differences)
 extern "C" int ktc_rhs(int chart, double M, const double* par3, const double* y4, double pt, double pp, double* f4) {
     const Knds<double> P = knds_t<double>(M, par3[0], par3[1], par3[2]);
