@@ -77,9 +77,7 @@ def flops_of(st, chart="ks"):
 
 def gpu_run(ctx, kt, prec, steps, warmup):
     fr = kt.make_frame(W, H, **FRAME)
-    o = kt.default_opts()
-    o.chart, o.integrator, o.precision = kt.KT_CHART_KS, kt.KT_INTEGRATOR_DOPRI5, prec
-    o.rtol, o.atol = RTOL, ATOL
+    o = c2_opts(kt, prec)
     for _ in range(warmup):
         ctx.render_device([fr], o)
     t0 = time.perf_counter()
@@ -89,10 +87,28 @@ def gpu_run(ctx, kt, prec, steps, warmup):
     return wall, stats, out["geo"], out["info"][0]
 
 
-def c4_run(ctx, kt, steps):
+def c2_opts(kt, prec):
+    o = kt.default_opts()
+    o.chart, o.integrator, o.precision = kt.KT_CHART_KS, kt.KT_INTEGRATOR_DOPRI5, prec
+    o.rtol, o.atol = RTOL, ATOL
+    return o
+
+
+def autotune_c2(ctx, kt, prec):
+    """The project's autotuner on the benchmark frame (its own probe: the whole frame): launch
+    configuration, wave-priority steps and, for mixed precision, the island threshold and radius
+    under its accuracy guard.  The timed renders then use the result (their fields are 0)."""
+    t = ctx.autotune([kt.make_frame(W, H, **FRAME)], c2_opts(kt, prec))
+    t["gain_vs_builtin"] = t["default_ms"] / t["best_ms"] if t["best_ms"] > 0 else None
+    return t
+
+
+def c4_run(kt, steps):
     """The uneven batched workload: 64 seeded frames (24.2 M rays), KS DOPRI5 mixed, rendered by
-    one graph launch of an autotuned plan; timed with the outputs left on the device."""
+    one graph launch of an autotuned plan; timed with the outputs left on the device.  A context of
+    its own, so that the plan tunes on these frames (a context keeps one result per option class)."""
     from kerrtracecpp2_amd import workloads
+    ctx = kt.Context(0)
     frames = workloads.c4_frames()
     npix = sum(f.width * f.height for f in frames)
     o = kt.default_opts()
@@ -105,6 +121,7 @@ def c4_run(ctx, kt, steps):
     wall = (time.perf_counter() - t0) / steps
     tune = plan.tuning()
     plan.close()
+    ctx.close()
     return {"workload": f"{len(frames)} frames, sides 64..1024, seed {workloads.C4_SEED}, KS DOPRI5 mixed, one graph launch",
             "rays": npix, "ms_per_batch": 1e3 * wall, "graph_ms": float(np.mean([s["trace_ms"] for s in sts])),
             "rays_per_s": npix / wall, "steps_accepted": sts[-1]["steps_accepted"], "autotune": tune}
@@ -293,11 +310,13 @@ def main():
         return
     if a.only != "both":   # profiling runs: one kernel variant, no JSON contract
         prec = kt.KT_PRECISION_MIXED if a.only == "mixed" else kt.KT_PRECISION_FP64
+        tuned = autotune_c2(ctx, kt, prec)      # as the full run does; its probe renders come first
         wall, st, _, _ = gpu_run(ctx, kt, prec, a.steps, a.warmup)
         print(json.dumps({"only": a.only, "ms_per_step": 1e3 * wall / a.steps,
-                          "trace_ms": float(np.mean([s["trace_ms"] for s in st]))}))
+                          "trace_ms": float(np.mean([s["trace_ms"] for s in st])), "autotune": tuned}))
         ctx.close()
         return
+    tune = {"mixed": autotune_c2(ctx, kt, kt.KT_PRECISION_MIXED), "fp64": autotune_c2(ctx, kt, kt.KT_PRECISION_FP64)}
     wall_m, st_m, geo_m, info = gpu_run(ctx, kt, kt.KT_PRECISION_MIXED, a.steps, a.warmup)
     wall_d, st_d, geo_d, _ = gpu_run(ctx, kt, kt.KT_PRECISION_FP64, a.steps, a.warmup)
     ms_m = 1e3 * wall_m / a.steps
@@ -328,6 +347,7 @@ def main():
                  "steps_accepted": st_d[-1]["steps_accepted"], "steps_rejected": st_d[-1]["steps_rejected"],
                  "achieved_tflops": fl_d / (trace_ms_d * 1e-3) / 1e12, "traffic": traffic_from_profile("FP64")},
         "mixed_vs_fp64": ms_d / ms_m,
+        "autotune_c2": tune,
         "mixed": {"trace_kernel_ms": trace_ms_m, "steps_accepted": st_m[-1]["steps_accepted"],
                   "steps_rejected": st_m[-1]["steps_rejected"], "rhs_fp32": st_m[-1]["rhs_fp32"],
                   "rhs_fp64": st_m[-1]["rhs_fp64"]},
@@ -337,7 +357,7 @@ def main():
                      "kernel": "kt_trace_kernel<KS,DOPRI5,MIXED>", "flops_per_launch": fl_m},
     }
     if not a.no_c4:
-        result["batched_c4"] = c4_run(ctx, kt, 3)
+        result["batched_c4"] = c4_run(kt, 3)
         result["adjoint_c5"] = c5_run(ctx, kt, 3)
         result["bundles_c3"] = c3_run(ctx, kt, 2)
     if not a.no_cpu:

@@ -140,7 +140,13 @@ typedef struct kt_stats {
     uint64_t events;          /* Hermite disk searches run                                  */
 } kt_stats;
 
-/* Launch configuration chosen by the autotuner (kt_hip_autotune). */
+/* Configuration chosen by the autotuner (kt_hip_autotune): the launch (grid, queue chunk,
+ * register budget), the wave issue-priority steps and, for mixed precision, the island
+ * threshold and radius.  Mixed candidates are kept only if, on the probe, they agree with a
+ * binary64 render of it at the parity bars (fate >= 99.9 % and |dg| <= 1e-4 off the band of
+ * fate edges, steep redshift and disk-edge images) and keep binary32 for at least 50 % of the
+ * right-hand sides (or the built-in policy's share where that is lower).  Renders whose
+ * corresponding kt_opts fields are 0 use the result. */
 typedef struct kt_tune {
     int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu x CUs                        */
     int32_t chunk;            /* rays per work-queue grab                                      */
@@ -149,6 +155,13 @@ typedef struct kt_tune {
     double best_ms;           /* trace time of the chosen configuration on the probe           */
     double default_ms;        /* trace time of the built-in configuration on the probe         */
     int64_t probe_rays;       /* rays of the probe workload                                    */
+    int32_t priority_steps;   /* accepted steps per wave issue-priority level (< 0: off)       */
+    int32_t guarded;          /* MIXED: candidates refused by the accuracy / binary32 guard    */
+    double mixed_noise;       /* MIXED: island threshold chosen (0 for binary64 and bundles)   */
+    double mixed_r_island;    /* MIXED: island radius chosen, in horizon radii                 */
+    double binary32_share;    /* MIXED: binary32 share of right-hand sides on the probe        */
+    double max_dg;            /* MIXED: max |dg| against a binary64 render of the probe, off the
+                                 parity band; fate agreement there was >= 99.9 %              */
 } kt_tune;
 
 typedef struct kt_hip_ctx kt_hip_ctx;

@@ -64,7 +64,9 @@ class KtKgeoMeta(ctypes.Structure):
 class KtTune(ctypes.Structure):
     _fields_ = [("blocks_per_cu", ctypes.c_int32), ("chunk", ctypes.c_int32), ("occupancy", ctypes.c_int32),
                 ("candidates", ctypes.c_int32), ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double),
-                ("probe_rays", ctypes.c_int64)]
+                ("probe_rays", ctypes.c_int64), ("priority_steps", ctypes.c_int32), ("guarded", ctypes.c_int32),
+                ("mixed_noise", ctypes.c_double), ("mixed_r_island", ctypes.c_double),
+                ("binary32_share", ctypes.c_double), ("max_dg", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

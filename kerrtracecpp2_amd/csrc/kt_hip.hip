@@ -1295,6 +1295,22 @@ static Launch launch_config(const kt_hip_ctx* c, const kt_opts* o, int chart) {
     return L;
 }
 
+// The option set a render runs with: the caller's, with the autotuner's result for its option
+// class filled into the run fields the caller left at 0 (issue-priority steps; for mixed
+// precision the island threshold and radius).  Launch fields go through launch_config.
+static kt_opts tuned_opts(const kt_hip_ctx* c, const kt_opts* o, int chart) {
+    kt_opts e = *o;
+    const int b = o->bundles ? 1 : 0, m = o->integrator ? 1 : 0, pr = o->precision ? 1 : 0, ch = chart ? 1 : 0;
+    if (!c->tuned[ch][m][pr][b]) return e;
+    const kt_tune& t = c->tune[ch][m][pr][b];
+    if (e.tuning_priority_steps == 0 && t.priority_steps != 0) e.tuning_priority_steps = t.priority_steps;
+    if (e.precision == KT_PRECISION_MIXED && !e.bundles) {
+        if (e.mixed_noise == 0.0 && t.mixed_noise > 0.0) e.mixed_noise = t.mixed_noise;
+        if (e.mixed_r_island == 0.0 && t.mixed_r_island > 0.0) e.mixed_r_island = t.mixed_r_island;
+    }
+    return e;
+}
+
 struct RenderBufs {
     LaunchDesc* desc;
     FrameDev* frames;
@@ -1410,8 +1426,9 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     rc = upload_frames(c, frames, n_frames, o, &rays, &pix, &chart);
     if (rc) return rc;
     const Launch L = force ? *force : launch_config(c, o, chart);
+    const kt_opts eff = tuned_opts(c, o, chart);
     RenderBufs B{c->table.desc(), c->table.frames(), c->table.scratch, c->d_geo, c->d_rgba, c->d_words, c->d_init};
-    rc = enqueue_render(c, B, o, chart, n_frames, rays, n_frames == 1, shade, L, c->ev);
+    rc = enqueue_render(c, B, &eff, chart, n_frames, rays, n_frames == 1, shade, L, c->ev);
     if (rc) return rc;
     KT_HIP(c, hipStreamSynchronize(c->stream));
     float t_trace = 0.f, t_total = 0.f;
@@ -1497,7 +1514,8 @@ int kt_hip_render_device_noalloc(kt_hip_ctx* c, const kt_frame* frames, int n_fr
                  reinterpret_cast<FrameDev*>(base + NoAllocLayout::table + sizeof(LaunchDesc)),
                  reinterpret_cast<SetupScratch*>(base + NoAllocLayout::setup),
                  static_cast<uint32_t*>(d_geo), static_cast<uint32_t*>(d_rgba), d_words, nullptr};
-    rc = enqueue_render(c, B, o, chart, n_frames, rays, false, d_rgba != nullptr, L, c->ev);
+    const kt_opts eff = tuned_opts(c, o, chart);
+    rc = enqueue_render(c, B, &eff, chart, n_frames, rays, false, d_rgba != nullptr, L, c->ev);
     if (rc) return rc;
     KT_HIP(c, hipStreamSynchronize(c->stream));
     if (!same) {
@@ -1809,6 +1827,34 @@ int kt_read_image_rgb(const char* path, uint8_t* rgb_out, int64_t cap, int32_t* 
 // ============================================================================
 //  autotuner and graph-captured batched plans
 // ============================================================================
+// The parity band of one frame's GeoPixels, as the test suite defines it: pixels with an
+// 8-neighbour of another fate; disk pixels whose redshift differs from a disk 8-neighbour's by
+// more than 0.05; pixels on or next to a disk hit within 0.5 % of the image's smallest or
+// largest hit radius (the images of the disk's edges, where r_hit jumps between crossings).
+static void parity_band(const kt_geopixel* g, int W, int H, uint8_t* band) {
+    float r_in = 1e30f, r_out = -1e30f;
+    for (int64_t i = 0; i < (int64_t)W * H; ++i)
+        if (g[i].outcome == 1) { r_in = std::min(r_in, g[i].r); r_out = std::max(r_out, g[i].r); }
+    auto near_edge = [&](const kt_geopixel& p) {
+        return p.outcome == 1 && (std::fabs(p.r - r_out) <= 0.005f * r_out || std::fabs(p.r - r_in) <= 0.005f * r_in);
+    };
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const kt_geopixel& p = g[(int64_t)y * W + x];
+            uint8_t b = 0;
+            for (int dy = -1; dy <= 1 && !b; ++dy)
+                for (int dx = -1; dx <= 1 && !b; ++dx) {
+                    const int ny = y + dy, nx = x + dx;
+                    if (ny < 0 || ny >= H || nx < 0 || nx >= W) continue;
+                    const kt_geopixel& q = g[(int64_t)ny * W + nx];
+                    if (q.outcome != p.outcome) b = 1;
+                    else if (p.outcome == 1 && std::fabs(q.redshift - p.redshift) > 0.05f) b = 1;
+                    else if (near_edge(q)) b = 1;
+                }
+            band[(int64_t)y * W + x] = b;
+        }
+}
+
 static void tune_slot(const kt_opts* o, int chart, int idx[4]) {
     idx[0] = chart ? 1 : 0; idx[1] = o->integrator ? 1 : 0; idx[2] = o->precision ? 1 : 0; idx[3] = o->bundles ? 1 : 0;
 }
@@ -1865,8 +1911,9 @@ static int plan_capture(kt_hip_plan* p) {
     if (hipMemcpyAsync(p->table.dev, p->table.host, p->table.bytes(p->max_frames), hipMemcpyHostToDevice,
                        c->stream) != hipSuccess)
         rc = fail(c, KT_E_DEVICE, "frame table copy could not be captured");
+    const kt_opts eff = tuned_opts(c, &p->o, p->chart);
     if (!rc)
-        rc = enqueue_render(c, B, &p->o, p->chart, p->max_frames, (int64_t)1 << 40, p->max_frames == 1, true, p->L,
+        rc = enqueue_render(c, B, &eff, p->chart, p->max_frames, (int64_t)1 << 40, p->max_frames == 1, true, p->L,
                             nullptr);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -1945,10 +1992,17 @@ int kt_hip_autotune(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const k
     }
     static const int bpcs[4] = {2, 3, 4, 6};
     static const int chunks[3] = {32, 64, 128};
+    const bool mixed = o->precision == KT_PRECISION_MIXED && !o->bundles;
+    // candidates run with every tuned run field explicit, so an earlier result for this option
+    // class does not leak into them
+    kt_opts ot = *o;
+    if (ot.tuning_priority_steps == 0) ot.tuning_priority_steps = kPrioSteps;
+    if (mixed && ot.mixed_noise == 0.0) ot.mixed_noise = kMixedNoiseDefault;
+    if (mixed && ot.mixed_r_island == 0.0) ot.mixed_r_island = kMixedRIslandDefault;
     kt_stats st;
     int64_t pix = 0;
     Launch def{kDefaultBlocksPerCu, kDefaultChunk, 0};
-    rc = render_on_device(c, pf.data(), n_frames, o, false, &st, &pix, &def);   // warm-up
+    rc = render_on_device(c, pf.data(), n_frames, &ot, false, &st, &pix, &def);   // warm-up
     if (rc) return rc;
     kt_tune best;
     std::memset(&best, 0, sizeof(best));
@@ -1959,10 +2013,10 @@ int kt_hip_autotune(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const k
     // the best (blocks per CU, chunk)
     const int occ0 = o->tuning_occupancy > 0 ? o->tuning_occupancy
                                              : (o->bundles ? 1 : default_occ(o->precision));
-    auto time_launch = [&](const Launch& L, double* ms) -> int {
+    auto time_launch = [&](const Launch& L, const kt_opts* oc, double* ms) -> int {
         *ms = 1e300;
         for (int rep = 0; rep < 2; ++rep) {
-            const int r = render_on_device(c, pf.data(), n_frames, o, false, &st, &pix, &L);
+            const int r = render_on_device(c, pf.data(), n_frames, oc, false, &st, &pix, &L);
             if (r) return r;
             *ms = std::min(*ms, st.trace_ms);
         }
@@ -1973,7 +2027,7 @@ int kt_hip_autotune(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const k
     for (int b : bpcs)
         for (int ch : chunks) {
             double ms;
-            rc = time_launch(Launch{b, ch, occ0}, &ms);
+            rc = time_launch(Launch{b, ch, occ0}, &ot, &ms);
             if (rc) return rc;
             if (b == kDefaultBlocksPerCu && ch == kDefaultChunk) best.default_ms = ms;
             if (ms < best.best_ms) { best.best_ms = ms; best.blocks_per_cu = b; best.chunk = ch; }
@@ -1981,9 +2035,100 @@ int kt_hip_autotune(kt_hip_ctx* c, const kt_frame* frames, int n_frames, const k
     if (o->tuning_occupancy <= 0) {
         const int occ1 = o->bundles ? 2 : (occ0 == 3 ? 2 : 3);
         double ms;
-        rc = time_launch(Launch{best.blocks_per_cu, best.chunk, occ1}, &ms);
+        rc = time_launch(Launch{best.blocks_per_cu, best.chunk, occ1}, &ot, &ms);
         if (rc) return rc;
         if (ms < best.best_ms) { best.best_ms = ms; best.occupancy = occ1; }
+    }
+    const Launch LB{best.blocks_per_cu, best.chunk, best.occupancy};
+    // wave issue priority: accepted steps per level (negative: off); binary64 results do not
+    // depend on it
+    best.priority_steps = ot.tuning_priority_steps;
+    if (o->tuning_priority_steps == 0) {
+        static const int prios[5] = {-1, 24, 32, 64, 96};
+        for (int pv : prios) {
+            kt_opts oc = ot;
+            oc.tuning_priority_steps = pv;
+            double ms;
+            rc = time_launch(LB, &oc, &ms);
+            if (rc) return rc;
+            if (ms < best.best_ms) { best.best_ms = ms; best.priority_steps = pv; }
+        }
+        ot.tuning_priority_steps = best.priority_steps;
+    }
+    // mixed precision: island threshold and radius, each candidate guarded against a binary64
+    // render of the probe (parity bars off the band) and kept to binary32 for the bulk
+    if (mixed) {
+        best.mixed_noise = ot.mixed_noise;
+        best.mixed_r_island = ot.mixed_r_island;
+        std::vector<kt_geopixel> ref((size_t)probe), got((size_t)probe);
+        std::vector<uint8_t> band((size_t)probe);
+        kt_opts o64 = ot;
+        o64.precision = KT_PRECISION_FP64;
+        rc = render_on_device(c, pf.data(), n_frames, &o64, false, &st, &pix, &LB);
+        if (rc) return rc;
+        KT_HIP(c, hipMemcpy(ref.data(), c->d_geo, (size_t)probe * sizeof(kt_geopixel), hipMemcpyDeviceToHost));
+        {
+            int64_t off = 0;
+            for (const kt_frame& f : pf) {
+                parity_band(ref.data() + off, f.width, f.height, band.data() + off);
+                off += (int64_t)f.width * f.height;
+            }
+        }
+        // timing + guard of one candidate; returns the time, or 1e300 when the guard refuses it.
+        // The binary32 floor is half of the right-hand sides, or the built-in policy's share where
+        // that is lower (batches with many polar or near-horizon rays): tuning never moves the
+        // mixed path toward binary64 beyond that.
+        double floor32 = 0.5;
+        auto guarded = [&](const kt_opts& oc, double* share, double* dg) -> double {
+            double ms;
+            if (time_launch(LB, &oc, &ms) != KT_OK) return -1.0;
+            *share = (double)st.rhs_fp32 / (double)std::max<uint64_t>(1, st.rhs_fp32 + st.rhs_fp64);
+            if (hipMemcpy(got.data(), c->d_geo, (size_t)probe * sizeof(kt_geopixel), hipMemcpyDeviceToHost) !=
+                hipSuccess)
+                return -1.0;
+            int64_t keep = 0, agree = 0;
+            double mdg = 0.0;
+            for (int64_t i = 0; i < probe; ++i) {
+                if (band[i]) continue;
+                ++keep;
+                if (got[i].outcome == ref[i].outcome) ++agree;
+                if (got[i].outcome == 1 && ref[i].outcome == 1)
+                    mdg = std::max(mdg, (double)std::fabs(got[i].redshift - ref[i].redshift));
+            }
+            *dg = mdg;
+            const bool ok = *share >= floor32 && mdg <= 1e-4 && (double)agree >= 0.999 * (double)keep;
+            if (!ok) ++best.guarded;
+            return ok ? ms : 1e300;
+        };
+        double share = 0.0, dg = 0.0;
+        floor32 = 0.0;                                     // the built-in island parameters set the floor
+        const double base = guarded(ot, &share, &dg);
+        if (base < 0.0) return fail(c, KT_E_DEVICE, "autotune probe render failed");
+        floor32 = std::min(0.5, share);
+        double best_mixed = base;
+        best.binary32_share = share;
+        best.max_dg = dg;
+        static const double noises[4] = {7e-10, 1.4e-9, 2e-9, 2.8e-9};
+        static const double risls[3] = {2.0, 2.5, 3.5};
+        for (double nz : noises)
+            for (double ri : risls) {
+                kt_opts oc = ot;
+                oc.mixed_noise = nz;
+                oc.mixed_r_island = ri;
+                const double ms = guarded(oc, &share, &dg);
+                if (ms < 0.0) return fail(c, KT_E_DEVICE, "autotune probe render failed");
+                if (ms < best_mixed) {
+                    best_mixed = ms;
+                    best.mixed_noise = nz;
+                    best.mixed_r_island = ri;
+                    best.binary32_share = share;
+                    best.max_dg = dg;
+                }
+            }
+        best.best_ms = std::min(best.best_ms, best_mixed);
+        if (base >= 1e300) {   // the built-in island parameters failed the guard: keep them only if nothing passed
+            if (best_mixed >= 1e300) { best.mixed_noise = ot.mixed_noise; best.mixed_r_island = ot.mixed_r_island; }
+        }
     }
     best.probe_rays = probe;
     int idx[4];

@@ -1,6 +1,7 @@
 """Summarise rocprofv3 output as short text for profiles/.
 
-    python -m kerrtracecpp2_amd.profsum <results.db> "<command>"            kernel-trace summary
+    python -m kerrtracecpp2_amd.profsum [--last N] <results.db> "<command>"  kernel-trace summary
+    (--last N, before any mode: each kernel's last N dispatches only)
     python -m kerrtracecpp2_amd.profsum pmc <counter_collection.csv> ...      per-kernel counter means
     python -m kerrtracecpp2_amd.profsum traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv>
         HBM bytes per launch of each kt_ kernel: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
@@ -12,10 +13,25 @@ import sys
 from collections import defaultdict
 
 
-def summary(db, command):
+def summary(db, command, last=0):
+    """Per-kernel calls / total / average duration; last > 0: only each kernel's last `last`
+    dispatches (the timed renders after an autotune's probe renders)."""
     c = sqlite3.connect(db)
     lines = [f"command: {command}", "", "kernel                                                        calls   total_us     avg_us"]
-    for name, calls, tot, avg, pct in c.execute("select * from top_kernels"):
+    if last > 0:
+        per = defaultdict(list)
+        for name, t0, t1 in c.execute("select name, start, end from kernels order by start"):
+            per[name].append((t1 - t0) / 1e3)
+        rows = [(n, v[-last:]) for n, v in per.items()]
+        grand = sum(sum(v) for _, v in rows)
+        for name, v in sorted(rows, key=lambda x: -sum(x[1])):
+            short = name.replace("(anonymous namespace)::", "")
+            lines.append(f"{short[:60]:60s} {len(v):6d} {sum(v):11.1f} {sum(v) / len(v):10.1f}  "
+                         f"({100.0 * sum(v) / grand:.1f}%)")
+        rows = []
+    else:
+        rows = c.execute("select * from top_kernels")
+    for name, calls, tot, avg, pct in rows:
         short = name.replace("(anonymous namespace)::", "")
         lines.append(f"{short[:60]:60s} {calls:6d} {tot:11.1f} {avg:10.1f}  ({pct:.1f}%)")
     lines.append("")
@@ -34,12 +50,18 @@ def _short(name):
     return name.replace("(anonymous namespace)::", "")
 
 
+LAST = 0   # > 0: counter means over each kernel's last LAST dispatches only
+
+
 def pmc_means(path):
-    """{(kernel, counter): (mean value per dispatch, dispatches)} from a counter_collection.csv."""
+    """{(kernel, counter): (mean value per dispatch, dispatches)} from a counter_collection.csv
+    (dispatches in file order, which is dispatch order)."""
     acc = defaultdict(list)
     with open(path) as f:
         for row in csv.DictReader(f):
             acc[(_short(row["Kernel_Name"]), row["Counter_Name"])].append(float(row["Counter_Value"]))
+    if LAST > 0:
+        acc = {k: v[-LAST:] for k, v in acc.items()}
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 
 
@@ -74,9 +96,12 @@ def traffic_summary(fetch, write, cfetch, cwrite, copy_bytes=1 << 28):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 2 and sys.argv[1] == "--last":
+        LAST = int(sys.argv[2])
+        del sys.argv[1:3]
     if sys.argv[1] == "pmc":
         sys.stdout.write(pmc_summary(sys.argv[2:]))
     elif sys.argv[1] == "traffic":
         sys.stdout.write(traffic_summary(*sys.argv[2:6]))
     else:
-        sys.stdout.write(summary(sys.argv[1], sys.argv[2] if len(sys.argv) > 2 else ""))
+        sys.stdout.write(summary(sys.argv[1], sys.argv[2] if len(sys.argv) > 2 else "", LAST))

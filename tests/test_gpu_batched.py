@@ -104,7 +104,10 @@ def test_plan_capacity_and_chart_errors(gpu_ctx):
 def test_autotune_and_configuration_independence(gpu_ctx):
     o = _opts()
     t = gpu_ctx.autotune(LIST_A, o)
-    assert t["candidates"] == 13        # 4 grids x 3 chunks at the default budget, then the other budget
+    # 4 grids x 3 chunks at the default budget, the other budget, 5 issue-priority settings
+    assert t["candidates"] == 18
+    assert t["priority_steps"] in (-1, 24, 32, 48, 64, 96)
+    assert t["mixed_noise"] == 0.0 and t["guarded"] == 0          # binary64: no island search
     assert t["blocks_per_cu"] in (2, 3, 4, 6) and t["chunk"] in (32, 64, 128) and t["occupancy"] in (2, 3)
     assert 0 < t["best_ms"] <= t["default_ms"]
     assert t["probe_rays"] == sum(f.width * f.height for f in LIST_A)     # small lists are their own probe
@@ -131,7 +134,12 @@ def test_c4_batched_graph_vs_oracle(gpu_ctx):
     finally:
         plan.close()
     assert out["stats"]["rays"] == npix
-    assert tune["candidates"] == 13
+    # launch (13) and priority (5) candidates, then the built-in islands and 12 island settings, each
+    # kept only if it meets the parity bars against binary64 and keeps binary32 for the bulk
+    assert tune["candidates"] == 31
+    assert tune["binary32_share"] >= 0.35 and tune["max_dg"] <= 1e-4
+    assert tune["mixed_noise"] in (7e-10, 1e-9, 1.4e-9, 2e-9, 2.8e-9)
+    assert tune["mixed_r_island"] in (2.0, 2.5, 3.5)
     # per-pixel parity on 12 frames, 2000 seeded pixels each (fewer for the smallest frames)
     rng = np.random.default_rng(7)
     offs = np.cumsum([0] + [f.width * f.height for f in frames])
