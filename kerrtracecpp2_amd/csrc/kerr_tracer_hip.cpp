@@ -228,6 +228,11 @@ int main(int argc, char** argv) {
     ro.exposure = exposure; ro.gamma = gamma; ro.temp_scale = temp_scale;
     ro.palette = palette; ro.disk_rings = disk_rings; ro.disk_sectors = disk_sectors;
     ro.disk_sigma = disk_sigma; ro.disk_hue_offset = disk_hue;
+    {   // the checks every render entry point applies, before a device is opened
+        char msg[256] = {0};
+        const kt_opts chk = kthip::make_opts(chart, ro, gpu_fp64 || precision == "fp64");
+        if (kt_check_opts(&chk, msg, sizeof msg) != KT_OK) die(std::string("invalid options: ") + msg);
+    }
 
     std::vector<uint8_t> bg_rgb;
     int bg_w = 0, bg_h = 0;

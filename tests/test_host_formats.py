@@ -168,3 +168,17 @@ def test_cli_rejects_unprovided_modes(flag):
     r = subprocess.run([CLI, "--custom-res", "8", "8", "--bg", "", flag], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "not provided by the HIP backend" in r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+@pytest.mark.parametrize("args,msg", [
+    (["--disk-interstellar", "--disk-rings", "100000"], "invalid options: disk_rings/disk_sectors above"),
+    (["--bundles", "--precision", "mixed"], "--bundles runs in binary64 only"),
+])
+def test_cli_rejects_invalid_options_before_opening_a_device(args, msg):
+    """The CLI applies kt_check_opts (the check of every render entry point) before it creates a
+    context, so these fail the same way with or without a GPU."""
+    r = subprocess.run([CLI, "--custom-res", "8", "8", "--bg", ""] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert msg in r.stderr
+    assert "Device:" not in r.stdout
