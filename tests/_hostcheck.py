@@ -15,14 +15,26 @@ _lib = None
 
 
 def build():
+    tmp = "%s.%d.tmp" % (LIB, os.getpid())      # test workers may build at once: rename atomically
     subprocess.check_call(["g++", "-std=c++17", "-O2", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared",
-                           "-o", LIB, SRC])
+                           "-o", tmp, SRC])
+    os.replace(tmp, LIB)
+
+
+def _stale():
+    """The library is missing or older than its source or any header of the device code."""
+    if not os.path.exists(LIB):
+        return True
+    csrc = os.path.join(os.path.dirname(HERE), "kerrtracecpp2_amd", "csrc")
+    deps = [SRC] + [os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith(".hpp")]
+    t = os.path.getmtime(LIB)
+    return any(os.path.getmtime(d) > t for d in deps)
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        if _stale():
             build()
         L = ctypes.CDLL(LIB)
         P = ctypes.c_void_p
