@@ -129,7 +129,8 @@ def c4_run(kt, steps):
 
 def c5_run(ctx, kt, reps):
     """Differentiable render: 256x256 Kerr-Newman a=0.7 Q=0.2 KS theta=60 DOPRI5; one adjoint call =
-    counting forward + recording forward (step sizes, checkpoints every 16 steps) + reverse pass."""
+    one forward pass (step sizes, checkpoints every 16 steps, at the record stride the context keeps
+    from its earlier calls) + reverse pass.  The untimed first call sizes that stride."""
     fr = kt.make_frame(256, 256, a=0.7, Q=0.2, Lambda=0.0, theta=60.0, r_obs=40.0, fov=30.0, disk_out=12.0)
     o = kt.default_opts()
     o.chart, o.integrator, o.rtol, o.atol = kt.KT_CHART_KS, kt.KT_INTEGRATOR_DOPRI5, RTOL, ATOL

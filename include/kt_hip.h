@@ -271,8 +271,9 @@ int kt_hip_render_intensity(kt_hip_ctx* ctx, const kt_frame* frame, const kt_opt
                             kt_stats* stats);
 
 typedef struct kt_adjoint_stats {
-    double forward_ms;        /* counting forward pass                                       */
-    double record_ms;         /* recording forward pass (step sizes + checkpoints)             */
+    double forward_ms;        /* forward pass: step counts, intensities, step sizes, checkpoints */
+    double record_ms;         /* the forward pass again, when a ray outgrew the record stride
+                                 (0 otherwise)                                                 */
     double backward_ms;       /* reverse pass                                                  */
     int64_t steps;            /* accepted steps recorded                                       */
     int64_t checkpoints;      /* state checkpoints stored (one per 16 accepted steps)          */
@@ -282,13 +283,15 @@ typedef struct kt_adjoint_stats {
 } kt_adjoint_stats;
 
 /* Hand-written adjoint of the adaptive integrator + disk shading.  Gradients of the per-pixel
- * intensity with respect to p = (a, Q, Lambda, theta_obs [degrees], r_obs): a counting forward
- * pass sizes the records, a recording pass stores every accepted step size and a checkpoint
- * every 16 accepted steps, and the reverse pass replays each segment from its checkpoint and
- * propagates the adjoint back through the DOPRI5 stages (accepted step sizes held fixed).
+ * intensity with respect to p = (a, Q, Lambda, theta_obs [degrees], r_obs): one forward pass
+ * stores every accepted step size and a checkpoint every 16 accepted steps at a fixed stride per
+ * pixel (the context keeps these buffers between calls; a ray longer than the stride makes the
+ * stride grow and the pass run once more), and the reverse pass replays each segment from its
+ * checkpoint and propagates the adjoint back through the DOPRI5 stages (accepted step sizes
+ * held fixed).
  *   weights:       W*H values w (dL/dI), NULL for all ones
  *   intensity_out: W*H (may be NULL);  dI_dp_out: W*H*5, pixel-major (may be NULL)
- *   grad_out:      5 values, sum_p w_p dI_p/dp (summed on the host in pixel order)          */
+ *   grad_out:      5 values, sum_p w_p dI_p/dp (summed on the device in a fixed order)      */
 int kt_hip_render_adjoint(kt_hip_ctx* ctx, const kt_frame* frame, const kt_opts* opts, const double* weights,
                           double* intensity_out, double* dI_dp_out, double* grad_out, kt_adjoint_stats* stats);
 

@@ -320,14 +320,14 @@ __global__ void kt_setup_final_kernel(FrameDev* frames, const LaunchDesc* desc, 
 //  persistent trace kernel
 // ---------------------------------------------------------------------------
 
-// per-pixel records of the adjoint's forward passes
+// per-pixel records of the adjoint's forward pass, at a fixed stride per pixel: pixel p's step
+// sizes are hs[p cap ..], its checkpoints ckpt[6 (p cap/16) ..]; a ray longer than cap steps
+// stops recording (its count tells the host, which enlarges cap and runs the pass again)
 struct AdjRecord {
     int32_t* n_acc;          // accepted steps of the ray (written when it ends)
     double* inten;           // disk intensity, 0 off the disk
-    const int32_t* n_cap;    // recording pass: step count of the counting pass
-    const int64_t* hoff;     // recording pass: first step-size slot of the pixel (nullptr: counting)
+    int32_t cap;             // step slots per pixel, a multiple of kCkptEvery (0: no records)
     double* hs;              // accepted step sizes
-    const int64_t* coff;     // first checkpoint slot of the pixel
     double* ckpt;            // r, theta, p_r, p_theta, sin theta, cos theta per checkpoint
 };
 
@@ -427,9 +427,8 @@ struct BundlePolicy {
     __device__ static const RayState& geo(const State& B) { return B.g; }
 };
 
-// Forward pass of the adjoint: the binary64 DOPRI5 geodesic, plus per-pixel records.
-// Counting pass (rec->hoff == nullptr): accepted-step count and disk intensity.
-// Recording pass: also every accepted step size and a checkpoint every kCkptEvery steps.
+// Forward pass of the adjoint: the binary64 DOPRI5 geodesic, its accepted-step count and disk
+// intensity, and (rec.cap > 0) every accepted step size and a checkpoint every kCkptEvery steps.
 template <int CHART>
 struct AdjointForwardPolicy {
     struct State { RayState g; int64_t pix; };
@@ -443,7 +442,7 @@ struct AdjointForwardPolicy {
         ray_init<CHART, PREC_FP64>(F, A.O, px, py, S.g);
         S.pix = pix;
         const AdjRecord& R = A.rec;
-        if (R.hoff) checkpoint(R, R.coff[S.pix], S.g);
+        if (R.cap) checkpoint(R, S.pix * (R.cap / kCkptEvery), S.g);
     }
     template <bool SINGLE>
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>& P,
@@ -453,10 +452,11 @@ struct AdjointForwardPolicy {
         magnif = 1.0;
         const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false, false>(F, P, Pf, A.O, true, S.g, o);
         const AdjRecord& R = A.rec;
-        if (R.hoff && S.g.n_acc > n0 && n0 < R.n_cap[S.pix]) {
-            R.hs[R.hoff[S.pix] + n0] = h;
+        if (R.cap && S.g.n_acc > n0 && n0 < R.cap) {
+            R.hs[S.pix * R.cap + n0] = h;
             const int n1 = S.g.n_acc;
-            if (!done && n1 % kCkptEvery == 0 && n1 < R.n_cap[S.pix]) checkpoint(R, R.coff[S.pix] + n1 / kCkptEvery, S.g);
+            if (!done && n1 % kCkptEvery == 0 && n1 < R.cap)
+                checkpoint(R, S.pix * (R.cap / kCkptEvery) + n1 / kCkptEvery, S.g);
         }
         if (done) {
             R.n_acc[S.pix] = S.g.n_acc;
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(256) void kt_init_kernel(const FrameDev* frames, co
     }
 }
 
-// Adjoint forward passes (counting / recording), one frame, binary64 DOPRI5.
+// Adjoint forward pass (counts, intensities, records), one frame, binary64 DOPRI5.
 template <int CHART>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void kt_adjoint_forward_kernel(TraceArgs A) {
@@ -646,9 +646,8 @@ struct AdjBackArgs {
     int64_t n_pix;
     const int32_t* n_acc;
     const double* inten;
-    const int64_t* hoff;
+    int32_t cap;                   // record stride (AdjRecord.cap)
     const double* hs;
-    const int64_t* coff;
     const double* ckpt;
     double* grad;                  // AP_N per pixel: dI/dp
     unsigned long long* words;     // [0] rays differentiated, [1] seeds without a disk root
@@ -666,13 +665,38 @@ __global__ __launch_bounds__(256) void kt_adjoint_backward_kernel(AdjBackArgs B)
         const int N = B.n_acc[p];
         if (!(B.inten[p] > 0.0) || N < 1) continue;
         double g[AP_N];
-        if (!adjoint_pixel<CHART>(F, (int)(p % F.W), (int)(p / F.W), N, B.hs + B.hoff[p], B.ckpt + 6*B.coff[p], g)) {
+        if (!adjoint_pixel<CHART>(F, (int)(p % F.W), (int)(p / F.W), N, B.hs + p*B.cap,
+                                  B.ckpt + 6*p*(B.cap / kCkptEvery), g)) {
             atomicAdd(&B.words[1], 1ull);
             continue;
         }
         for (int q = 0; q < AP_N; ++q) gout[q] = g[q];
         atomicAdd(&B.words[0], 1ull);
     }
+}
+
+// sum over pixels of w_p dI_p/dp (w = nullptr: all ones; pixels with w_p = 0 skipped) in a fixed
+// order: thread t takes pixels t, t + 1024, ... in turn, then a fixed pairwise tree in LDS, so the
+// result does not depend on the launch or on timing
+__global__ __launch_bounds__(1024) void kt_adjoint_sum_kernel(const double* grad, const double* w, int64_t n,
+                                                               double* out) {
+    __shared__ double part[AP_N][1024];
+    const int t = threadIdx.x;
+    double acc[AP_N] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = t; i < n; i += 1024) {
+        const double wi = w ? w[i] : 1.0;
+        if (wi == 0.0) continue;
+        for (int q = 0; q < AP_N; ++q) acc[q] = fma(wi, grad[i*AP_N + q], acc[q]);
+    }
+    for (int q = 0; q < AP_N; ++q) part[q][t] = acc[q];
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < AP_N; ++q) part[q][t] += part[q][t + s];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < AP_N; ++q) out[q] = part[q][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -862,6 +886,12 @@ struct kt_hip_ctx {
     std::map<uint64_t, uint32_t*> orders;
     unsigned char* d_noalloc = nullptr;     // this device's allocation-free scratch (not owned)
     std::string noalloc_err;                // why d_noalloc is null, if it is
+    // adjoint buffers, kept between calls: per-pixel counts, intensities, dI/dp, and the records
+    // at adj_cap step slots per pixel; adj_next_cap is the stride the next call starts from
+    int32_t* d_adj_nacc = nullptr; double* d_adj_inten = nullptr; double* d_adj_grad = nullptr;
+    double* d_adj_hs = nullptr; double* d_adj_ckpt = nullptr;
+    double* d_adj_w = nullptr; double* d_adj_sum = nullptr;       // pixel weights, weighted sum
+    int64_t adj_pix = 0; int32_t adj_cap = 0; int32_t adj_next_cap = 256;
 };
 
 static int fail(kt_hip_ctx* c, int code, const char* fmt, ...) {
@@ -1018,6 +1048,8 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     (void)hipFree(c->d_rgb);
     (void)hipFree(c->d_tab);
     for (auto& kv : c->orders) (void)hipFree(kv.second);
+    (void)hipFree(c->d_adj_nacc); (void)hipFree(c->d_adj_inten); (void)hipFree(c->d_adj_grad);
+    (void)hipFree(c->d_adj_hs); (void)hipFree(c->d_adj_ckpt); (void)hipFree(c->d_adj_w); (void)hipFree(c->d_adj_sum);
     for (int i = 0; i < 4; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2231,6 +2263,29 @@ int dev_alloc(kt_hip_ctx* c, DevBuf& b, size_t bytes) {
     return KT_OK;
 }
 
+// The adjoint's buffers for pix pixels at cap step slots per pixel, kept in the context and
+// grown (never shrunk) as calls need.
+int adjoint_buffers(kt_hip_ctx* c, int64_t pix, int32_t cap) {
+    if (pix <= c->adj_pix && cap <= c->adj_cap) return KT_OK;
+    const int64_t np = std::max(pix, c->adj_pix);
+    const int32_t nc = std::max(cap, c->adj_cap);
+    (void)hipFree(c->d_adj_nacc); (void)hipFree(c->d_adj_inten); (void)hipFree(c->d_adj_grad);
+    (void)hipFree(c->d_adj_hs); (void)hipFree(c->d_adj_ckpt); (void)hipFree(c->d_adj_w); (void)hipFree(c->d_adj_sum);
+    c->d_adj_nacc = nullptr; c->d_adj_inten = nullptr; c->d_adj_grad = nullptr;
+    c->d_adj_hs = nullptr; c->d_adj_ckpt = nullptr; c->d_adj_w = nullptr; c->d_adj_sum = nullptr;
+    c->adj_pix = 0; c->adj_cap = 0;
+    const size_t hs_bytes = (size_t)np * nc * 8, ck_bytes = (size_t)np * (nc / kCkptEvery) * 48;
+    if (hipMalloc(&c->d_adj_nacc, np * 4) != hipSuccess || hipMalloc(&c->d_adj_inten, np * 8) != hipSuccess ||
+        hipMalloc(&c->d_adj_grad, np * 8 * AP_N) != hipSuccess || hipMalloc(&c->d_adj_hs, hs_bytes) != hipSuccess ||
+        hipMalloc(&c->d_adj_ckpt, ck_bytes) != hipSuccess || hipMalloc(&c->d_adj_w, np * 8) != hipSuccess ||
+        hipMalloc(&c->d_adj_sum, AP_N * 8) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "adjoint records for %lld pixels x %d steps (%zu bytes)", (long long)np, nc,
+                    hs_bytes + ck_bytes);
+    c->adj_pix = np;
+    c->adj_cap = nc;
+    return KT_OK;
+}
+
 typedef void (*AdjFwdFn)(TraceArgs);
 
 // One forward pass of the adjoint (counting when rec.hoff is null); times it with HIP events.
@@ -2315,60 +2370,52 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     int rc = adjoint_prepare(c, f, oin, &o, &chart, &rays, &pix);
     if (rc) return rc;
     if (!grad_out && !dI_out) return fail(c, KT_E_ARG, "grad_out and dI_dp_out are both NULL");
-    DevBuf n_acc, n_cap, inten, hoff, coff, grad, hs, ckpt;
-    if ((rc = dev_alloc(c, n_acc, pix * 4)) || (rc = dev_alloc(c, n_cap, pix * 4)) ||
-        (rc = dev_alloc(c, inten, pix * 8)) || (rc = dev_alloc(c, hoff, pix * 8)) ||
-        (rc = dev_alloc(c, coff, pix * 8)) || (rc = dev_alloc(c, grad, pix * 8 * AP_N)))
-        return rc;
-    AdjRecord rec;
-    std::memset(&rec, 0, sizeof(rec));
-    rec.n_acc = n_acc.as<int32_t>();
-    rec.inten = inten.as<double>();
     kt_adjoint_stats S;
     std::memset(&S, 0, sizeof(S));
-    float ms = 0.f;
-    // 1. counting pass: accepted steps per ray -> exact record layout
-    rc = adjoint_forward(c, &o, chart, rays, rec, &ms);
-    if (rc) return rc;
-    S.forward_ms = ms;
+    // 1. one forward pass: counts, intensities and the records at the context's current stride;
+    //    a ray longer than the stride makes the host enlarge it and run the pass again
+    int32_t cap = c->adj_next_cap;
     std::vector<int32_t> cnt((size_t)pix);
-    KT_HIP(c, hipMemcpy(cnt.data(), rec.n_acc, pix * 4, hipMemcpyDeviceToHost));
-    std::vector<int64_t> ho((size_t)pix), co((size_t)pix);
+    for (int pass = 0; pass < 2; ++pass) {
+        if ((rc = adjoint_buffers(c, pix, cap))) return rc;
+        AdjRecord rec;
+        rec.n_acc = c->d_adj_nacc;
+        rec.inten = c->d_adj_inten;
+        rec.cap = cap;
+        rec.hs = c->d_adj_hs;
+        rec.ckpt = c->d_adj_ckpt;
+        float ms = 0.f;
+        rc = adjoint_forward(c, &o, chart, rays, rec, &ms);
+        if (rc) return rc;
+        if (pass == 0) S.forward_ms = ms; else S.record_ms = ms;
+        KT_HIP(c, hipMemcpy(cnt.data(), c->d_adj_nacc, pix * 4, hipMemcpyDeviceToHost));
+        int32_t nmax = 0;
+        for (int64_t i = 0; i < pix; ++i) nmax = std::max(nmax, cnt[i]);
+        if (nmax <= cap) break;
+        if (pass == 1) return fail(c, KT_E_DEVICE, "adjoint records: %d steps after enlarging to %d", nmax, cap);
+        cap = (nmax + kCkptEvery - 1) / kCkptEvery * kCkptEvery;
+        c->adj_next_cap = cap;
+    }
     int64_t nh = 0, nc = 0;
     for (int64_t i = 0; i < pix; ++i) {
-        ho[i] = nh;
-        co[i] = nc;
         const int64_t n = cnt[i] > 0 ? cnt[i] : 0;
         nh += n;
-        nc += (n + kCkptEvery - 1) / kCkptEvery > 0 ? (n + kCkptEvery - 1) / kCkptEvery : 1;
+        nc += n > 0 ? (n + kCkptEvery - 1) / kCkptEvery : 1;
     }
     S.steps = nh;
     S.checkpoints = nc;
     S.record_bytes = nh * 8 + nc * 48;
-    if ((rc = dev_alloc(c, hs, (size_t)nh * 8)) || (rc = dev_alloc(c, ckpt, (size_t)nc * 48))) return rc;
-    KT_HIP(c, hipMemcpy(hoff.p, ho.data(), pix * 8, hipMemcpyHostToDevice));
-    KT_HIP(c, hipMemcpy(coff.p, co.data(), pix * 8, hipMemcpyHostToDevice));
-    KT_HIP(c, hipMemcpy(n_cap.p, rec.n_acc, pix * 4, hipMemcpyDeviceToDevice));
-    // 2. recording pass (the binary64 path is deterministic: the same steps as pass 1)
-    rec.n_cap = n_cap.as<int32_t>();
-    rec.hoff = hoff.as<int64_t>();
-    rec.hs = hs.as<double>();
-    rec.coff = coff.as<int64_t>();
-    rec.ckpt = ckpt.as<double>();
-    rc = adjoint_forward(c, &o, chart, rays, rec, &ms);
-    if (rc) return rc;
-    S.record_ms = ms;
+    float ms = 0.f;
     // 3. reverse pass
     AdjBackArgs B;
     B.frame = c->table.frames();
     B.n_pix = pix;
-    B.n_acc = rec.n_cap;
-    B.inten = rec.inten;
-    B.hoff = rec.hoff;
-    B.hs = rec.hs;
-    B.coff = rec.coff;
-    B.ckpt = rec.ckpt;
-    B.grad = grad.as<double>();
+    B.n_acc = c->d_adj_nacc;
+    B.inten = c->d_adj_inten;
+    B.cap = cap;
+    B.hs = c->d_adj_hs;
+    B.ckpt = c->d_adj_ckpt;
+    B.grad = c->d_adj_grad;
     B.words = c->d_words;
     hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
     KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
@@ -2386,19 +2433,17 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     KT_HIP(c, hipMemcpy(words, c->d_words, sizeof(words), hipMemcpyDeviceToHost));
     S.rays_differentiated = (int64_t)words[0];
     S.seed_failures = (int64_t)words[1];
-    std::vector<double> g((size_t)pix * AP_N);
-    KT_HIP(c, hipMemcpy(g.data(), grad.p, pix * 8 * AP_N, hipMemcpyDeviceToHost));
-    if (I_out) KT_HIP(c, hipMemcpy(I_out, rec.inten, pix * 8, hipMemcpyDeviceToHost));
-    if (dI_out) std::memcpy(dI_out, g.data(), g.size() * 8);
     if (grad_out) {
-        double acc[AP_N] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int64_t i = 0; i < pix; ++i) {
-            const double wi = w ? w[i] : 1.0;
-            if (wi == 0.0) continue;
-            for (int q = 0; q < AP_N; ++q) acc[q] += wi * g[(size_t)i*AP_N + q];
-        }
-        for (int q = 0; q < AP_N; ++q) grad_out[q] = acc[q];
+        if (w) KT_HIP(c, hipMemcpyAsync(c->d_adj_w, w, pix * 8, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(kt_adjoint_sum_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_grad,
+                           w ? c->d_adj_w : nullptr, pix, c->d_adj_sum);
+        KT_HIP(c, hipGetLastError());
+        // the context stream does not synchronise with the null stream: copy on it and wait
+        KT_HIP(c, hipMemcpyAsync(grad_out, c->d_adj_sum, AP_N * 8, hipMemcpyDeviceToHost, c->stream));
+        KT_HIP(c, hipStreamSynchronize(c->stream));
     }
+    if (dI_out) KT_HIP(c, hipMemcpy(dI_out, c->d_adj_grad, pix * 8 * AP_N, hipMemcpyDeviceToHost));
+    if (I_out) KT_HIP(c, hipMemcpy(I_out, c->d_adj_inten, pix * 8, hipMemcpyDeviceToHost));
     if (ast) *ast = S;
     return KT_OK;
 }

@@ -64,6 +64,48 @@ def test_adjoint_records_and_counts(gpu_ctx, base):
     assert np.array_equal(I2, I)                  # the intensity pass and the adjoint's forward agree
 
 
+def test_adjoint_record_stride_kept_between_calls(base):
+    """The records sit at a per-pixel stride that the context keeps: a fresh context may run the
+    forward pass twice on its first call (a ray longer than the starting stride), never on the
+    next; a larger frame then grows the buffers.  Every call gives the same gradients bit for
+    bit, since the binary64 path takes the same steps whatever the stride."""
+    ctx = kt.Context(0)
+    try:
+        first = ctx.render_adjoint(_frame(), _opts(), want_pixel_grads=True)
+        again = ctx.render_adjoint(_frame(), _opts(), want_pixel_grads=True)
+        print("strides: first call", first["stats"], "second", again["stats"], flush=True)
+        assert again["stats"]["record_ms"] == 0.0
+        for out in (first, again):
+            assert np.array_equal(out["dI_dp"], base["dI_dp"])
+            assert np.array_equal(out["intensity"], base["intensity"])
+            assert out["stats"]["steps"] == base["stats"]["steps"]
+        big = kt.make_frame(320, 288, fov=30.0, disk_out=12.0, **C5)
+        grown = ctx.render_adjoint(big, _opts(), want_pixel_grads=True)
+    finally:
+        ctx.close()
+    ctx2 = kt.Context(0)
+    try:
+        fresh = ctx2.render_adjoint(big, _opts(), want_pixel_grads=True)
+    finally:
+        ctx2.close()
+    assert np.array_equal(grown["dI_dp"], fresh["dI_dp"])
+    assert grown["stats"]["rays_differentiated"] == fresh["stats"]["rays_differentiated"] > 10000
+
+
+def test_grad_is_the_weighted_sum_of_pixel_grads(gpu_ctx, base):
+    """grad_out is summed on the device in a fixed order: it equals the per-pixel dI/dp summed with
+    the weights (to rounding), and repeats bit for bit."""
+    dI = base["dI_dp"].reshape(-1, 5)
+    np.testing.assert_allclose(base["grad"], dI.sum(axis=0), rtol=1e-12, atol=1e-12 * np.abs(dI).sum(axis=0).max())
+    w = np.random.default_rng(5).standard_normal(W * H)
+    w[::7] = 0.0
+    out = gpu_ctx.render_adjoint(_frame(), _opts(), weights=w, want_pixel_grads=True)
+    want = (w[:, None] * out["dI_dp"].reshape(-1, 5)).sum(axis=0)
+    np.testing.assert_allclose(out["grad"], want, rtol=1e-11, atol=1e-12 * np.abs(out["dI_dp"]).sum(axis=(0, 1)).max())
+    again = gpu_ctx.render_adjoint(_frame(), _opts(), weights=w)
+    assert np.array_equal(again["grad"], out["grad"])
+
+
 def test_gpu_adjoint_equals_host_adjoint(base):
     o = _opts()
     o.quirks |= 1 << 30
