@@ -43,7 +43,7 @@ RTOL, ATOL = 1e-9, 1e-12
 #   their tableau combinations for stages 2-6),
 #   one sin/cos rotation per attempt (the FSAL stage; |d| <= 1/8 series),
 #   Hermite 8-bin/8-bisection disk search.
-F_RHS = {"bl": 98, "ks": 93}
+F_RHS = {"bl": 83, "ks": 78}
 F_DP5_COMB = 256
 F_SCI = 80
 F_ROT = 31
@@ -58,12 +58,12 @@ PEAK_VALU_TFLOPS = 78.6   # MI355X vector FP64; FP32 without packed math issues 
 F_VJP = {"bl": 367, "ks": 348}
 F_ADJ_STEP = 2 * (6 * (F_RHS["ks"] + F_ROT) + 182) + 6 * F_VJP["ks"] + 222
 # Jacobi bundles (kt_bundle_kernel, BL, RK4 doubling), per attempt: the full step (3 plain
-# right-hand sides + rotations + sums: 3 x (98 + 31 + 3) + 54), the two half steps (3 stages
+# right-hand sides F_RHS["bl"] + rotations + sums: 3 x (F_RHS + 31 + 3) + 54), the two half steps (3 stages
 # each of one dual-number right-hand side F_DUAL_BL (counted by flopcount.cpp) + 4 for its sin/cos
 # tangents + rotation + state/W updates 24 + stage sums 26, then 14 for the increments), the
 # derivative at the midpoint (F_DUAL_BL + 4 + 31 + 12) and the error norm (22); an accepted
 # attempt adds the derivative at the new point (F_DUAL_BL + 4 + 31 + 21).
-F_DUAL_BL = 459
+F_DUAL_BL = 392
 F_BUNDLE_ATTEMPT = 3 * (F_RHS["bl"] + F_ROT + 3) + 54 + 2 * (3 * (F_DUAL_BL + 4 + F_ROT + 24 + 26) + 14) \
     + (F_DUAL_BL + 4 + F_ROT + 12) + 22
 F_BUNDLE_ACCEPT = F_DUAL_BL + 4 + F_ROT + 21

@@ -216,7 +216,15 @@ template <class T, class S> KT_HD Knds<T> knds_cast(const Knds<S>& k) {
 // reference's Cartesian Kerr-Schild chart (knds_metric.hpp:135-230) with
 // X + iY = (r + i a) sin(theta) e^{i phi~}, Z = r cos(theta), T = v - r.
 //
-// Arithmetic: one reciprocal of Sigma * Dr * (Dt s^2) (BL) / Sigma * Dt s^2 (KS).
+// Arithmetic: one reciprocal of Sigma * Dr * (Dt s^2) (BL) / Sigma * Dt s^2 (KS).  The
+// derivatives are formed at half scale, which takes the factors 2 of the quadratic form out
+// of every product:  with N = 2 Sigma H and N_s = N / Sigma,
+//   dr/dl = (1/2 dN/dp_r) / Sigma,            dtheta/dl = Dt pth / Sigma,
+//   -dp_r/dl = (1/2 dN/dr - r N_s) / Sigma,   -dp_theta/dl = s c (X + a^2 N_s) / Sigma,
+// where 1/2 dN/dtheta = s c X (every theta-derivative of the metric functions carries s c):
+//   X = -(Lambda a^2/3) pth^2 + 2 a p_t B1 - B1 L/(Dt s^2) (Dt - (Lambda a^2/3) s^2),
+//   B1 = Xi^2 L / (Dt s^2),
+// and 1/2 dDelta_r/dr = r (1 - Lambda r^2/3) - (Lambda/3) r (r^2+a^2) - M.
 enum { CHART_BL = 0, CHART_KS = 1 };
 
 // T: type of the state variables (double, float, or the dual numbers of the
@@ -233,53 +241,47 @@ KT_HD void rhs_impl(const Knds<S>& P, T r, const SinCos<T>& sc, T pr, T pth, S p
     const T sig = fma(P.a2, c2, r2);
     const T one_m = fma(-P.lam3, r2, S(1));                  // 1 - Lambda r^2/3
     const T dR = fma(ra, one_m, fma(-P.M2, r, P.Q2));        // Delta_r
-    // dDelta_r/dr = 2r(1 - L r^2/3) - (r^2+a^2) 2 L r/3 - 2M
-    const T dR_r = fma(S(2)*r, one_m, fma((S(-2)*P.lam3)*r, ra, -P.M2));
+    const T hdR_r = fma(r, one_m, fma(-(P.lam3*r), ra, -P.M));   // 1/2 dDelta_r/dr
     const T dT = fma(P.lam3a2, c2, S(1));                    // Delta_theta
-    const T dT_th = (S(-2)*P.lam3a2)*sc2;                     // dDelta_theta/dtheta
     const T K = fma(ra, pt, P.a*pp);
     const T L = fma(P.a*s2, pt, pp);
-    const T Kr = (S(2)*pt)*r;                                 // dK/dr
-    const T Lth = (S(2)*P.a*pt)*sc2;                          // dL/dtheta
     const T dTs2 = dT*s2;
+    const T dTpth = dT*pth;
+    T inv_sig, inv_ts, B1, N, o0, hdNr;
     if (CHART == CHART_BL) {
         const T p12 = sig*dR;
         const T inv = kt_rcp(p12*dTs2);
-        const T inv_sig = inv*dR*dTs2;
-        const T inv_out = SCALED ? inv_sig*hscale : inv_sig;
+        inv_sig = inv*dR*dTs2;
         const T inv_dR = inv*sig*dTs2;
-        const T inv_ts = inv*p12;
+        inv_ts = inv*p12;
         const T A1 = P.xi2*K*inv_dR;                          // Xi^2 K / Dr
-        const T B1 = P.xi2*L*inv_ts;                          // Xi^2 L / (Dt s^2)
-        const T N = fma(dR*pr, pr, fma(dT*pth, pth, fma(-A1, K, B1*L)));
-        out[0] = dR*pr*inv_out;
-        out[1] = dT*pth*inv_out;
-        out[4] = fma(-A1, P.a, B1)*inv_out;
-        // dN/dr = dR_r pr^2 - 2 A1 Kr + A1 K dR_r / Dr
-        const T dNr = fma(dR_r*pr, pr, fma(S(-2)*A1, Kr, A1*K*inv_dR*dR_r));
-        // dN/dth = dT_th pth^2 + 2 B1 Lth - B1 L (dT_th s^2 + 2 Dt s c)/(Dt s^2)
-        const T dNth = fma(dT_th*pth, pth, fma(S(2)*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, S(2)*dT*sc2)));
-        const T hs = S(0.5)*inv_out;
-        out[2] = -fma(-N*(S(2)*r), inv_sig, dNr)*hs;
-        out[3] = -fma(N*(S(2)*P.a2)*sc2, inv_sig, dNth)*hs;
+        B1 = P.xi2*L*inv_ts;                                  // Xi^2 L / (Dt s^2)
+        N = fma(dR*pr, pr, fma(dTpth, pth, fma(-A1, K, B1*L)));
+        o0 = dR*pr;
+        out[4] = fma(-A1, P.a, B1);
+        // 1/2 dN/dr = 1/2 dDelta_r/dr (pr^2 + Xi^2 K^2/Dr^2) - 2 A1 p_t r
+        hdNr = fma(hdR_r, fma(pr, pr, A1*K*inv_dR), -(((pt + pt)*r)*A1));
     } else {
         const T inv = kt_rcp(sig*dTs2);
-        const T inv_sig = inv*dTs2;
-        const T inv_out = SCALED ? inv_sig*hscale : inv_sig;
-        const T inv_ts = inv*sig;
-        const T B1 = P.xi2*L*inv_ts;                          // Xi^2 L / (Dt s^2)
+        inv_sig = inv*dTs2;
+        inv_ts = inv*sig;
+        B1 = P.xi2*L*inv_ts;                                  // Xi^2 L / (Dt s^2)
         const T XK = P.xi*K;
-        const T N = fma(dR*pr, pr, fma(S(2)*XK, pr, fma(dT*pth, pth, B1*L)));
-        out[0] = fma(dR, pr, XK)*inv_out;
-        out[1] = dT*pth*inv_out;
-        out[4] = fma(P.xi*P.a, pr, B1)*inv_out;
-        // dN/dr = dR_r pr^2 + 2 Xi pr Kr
-        const T dNr = fma(dR_r*pr, pr, (S(2)*P.xi)*pr*Kr);
-        const T dNth = fma(dT_th*pth, pth, fma(S(2)*B1, Lth, -B1*L*inv_ts*fma(dT_th, s2, S(2)*dT*sc2)));
-        const T hs = S(0.5)*inv_out;
-        out[2] = -fma(-N*(S(2)*r), inv_sig, dNr)*hs;
-        out[3] = -fma(N*(S(2)*P.a2)*sc2, inv_sig, dNth)*hs;
+        N = fma(dR*pr, pr, fma(XK + XK, pr, fma(dTpth, pth, B1*L)));
+        o0 = fma(dR, pr, XK);
+        out[4] = fma(P.xi*P.a, pr, B1);
+        // 1/2 dN/dr = 1/2 dDelta_r/dr pr^2 + 2 Xi p_t r pr
+        hdNr = pr*fma(hdR_r, pr, ((P.xi + P.xi)*pt)*r);
     }
+    const T inv_out = SCALED ? inv_sig*hscale : inv_sig;
+    const T Ns = N*inv_sig;
+    const T X = fma(-(P.lam3a2*pth), pth,
+                    fma((P.a + P.a)*pt, B1, -(B1*L*inv_ts)*fma(-P.lam3a2, s2, dT)));
+    out[0] = o0*inv_out;
+    out[1] = dTpth*inv_out;
+    out[4] = out[4]*inv_out;
+    out[2] = -fma(-r, Ns, hdNr)*inv_out;
+    out[3] = -(fma(P.a2, Ns, X)*sc2)*inv_out;
 }
 
 template <int CHART, class T, class S = T>

@@ -437,25 +437,6 @@ KT_HD void stage_given(const Knds<double>& P, const Knds<float>& Pf, bool use64,
     }
 }
 
-// stage derivative times h at y0 + d with the sin/cos of theta given (integrated stage values)
-template <int CHART, int PREC, int NC>
-KT_HD void stage_given_h(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
-                         const double d[4], const SinCos<double>& sc, double pt, double pp, double h, double hk[NC]) {
-    if (PREC == PREC_FP64 || use64) {
-        double o[5];
-        rhs_h<CHART>(P, y0[0] + d[0], sc, y0[2] + d[2], y0[3] + d[3], pt, pp, h, o);
-        for (int i = 0; i < NC; ++i) hk[i] = o[i];
-    } else {
-        const SinCos<float> scf = {(float)sc.s, (float)sc.c};
-        float kf[5];
-        float ptf = (float)pt, ppf = (float)pp;
-        opaque_f32(ptf, ppf);
-        rhs_h<CHART>(Pf, (float)(y0[0] + d[0]), scf, (float)(y0[2] + d[2]), (float)(y0[3] + d[3]),
-                     ptf, ppf, (float)h, kf);
-        for (int i = 0; i < NC; ++i) hk[i] = (double)kf[i];
-    }
-}
-
 // Step-size factor x^(-q) for a finite x > 0 of any binary64 magnitude: the binary exponent
 // is split off exactly (x = m 2^e, m in [0.5, 1)), log2 of the mantissa and the final exp2
 // are binary32 (~1e-7 relative).  The factor only scales the next trial step; the accept
@@ -749,10 +730,52 @@ KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& 
     return after_reject(O, S);
 }
 
+// stage derivative times h at the stage point z (r, -, p_r, p_theta; z[1] is not read), the
+// sin/cos of theta given (integrated stage values)
+template <int CHART, int PREC, int NC>
+KT_HD void stage_at_h(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double z[4],
+                      const SinCos<double>& sc, double pt, double pp, double h, double hk[NC]) {
+    if (PREC == PREC_FP64 || use64) {
+        double o[5];
+        rhs_h<CHART>(P, z[0], sc, z[2], z[3], pt, pp, h, o);
+        for (int i = 0; i < NC; ++i) hk[i] = o[i];
+    } else {
+        const SinCos<float> scf = {(float)sc.s, (float)sc.c};
+        float kf[5];
+        float ptf = (float)pt, ppf = (float)pp;
+        opaque_f32(ptf, ppf);
+        rhs_h<CHART>(Pf, (float)z[0], scf, (float)z[2], (float)z[3], ptf, ppf, (float)h, kf);
+        for (int i = 0; i < NC; ++i) hk[i] = (double)kf[i];
+    }
+}
+
+// FSAL stage at the 5th-order point z (theta = th0 + dth): f(z) unscaled, and sin/cos of its theta
+template <int CHART, int PREC>
+KT_HD void fsal_stage(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double z[4], double th0,
+                      double dth, const SinCos<double>& sc0, double pt, double pp, double k[5], SinCos<double>& sc) {
+    if (PREC == PREC_FP64 || use64) {
+        if (fabs(dth) <= kRotateMax) sc = rotate(sc0, dth);
+        else kt_sincos(th0 + dth, &sc.s, &sc.c);
+        rhs<CHART>(P, z[0], sc, z[2], z[3], pt, pp, k);
+    } else {
+        SinCos<float> scf;
+        if (fabs(dth) <= kRotateMax) scf = rotate(SinCos<float>{(float)sc0.s, (float)sc0.c}, (float)dth);
+        else kt_sincos((float)(th0 + dth), &scf.s, &scf.c);
+        float kf[5];
+        float ptf = (float)pt, ppf = (float)pp;
+        opaque_f32(ptf, ppf);
+        rhs<CHART>(Pf, (float)z[0], scf, (float)z[2], (float)z[3], ptf, ppf, kf);
+        for (int i = 0; i < 5; ++i) k[i] = (double)kf[i];
+        sc = sc_at(sc0, th0, dth);
+    }
+}
+
 // DOPRI5 attempt of the trace kernels: the dp5_step arithmetic with integrated stage sin/cos,
 // stages 2-6 carried as h k_i (h folded into each right-hand side's final scaling) and the
-// k1 terms taking their tableau coefficient times h, so that no stage combination, 5th-order
-// sum or error sum needs a final multiply by h per component.  k1 and the FSAL stage stay
+// k1 terms taking their tableau coefficient times h.  Each stage point, and the 5th-order
+// solution, is accumulated onto y by fused multiply-adds (one per tableau term, no separate
+// sum and add); theta enters the stages only through its sin/cos, so its stage values are not
+// formed, and its 5th-order increment feeds the FSAL rotation.  k1 and the FSAL stage stay
 // unscaled: they are the event derivatives and the next attempt's (or a retry's) k1.
 template <int CHART, int PREC, int NC>
 KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
@@ -762,63 +785,66 @@ KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>
     const double* y = S.y;
     const double* k1 = S.k1;                    // f(y), kept for a rejected attempt
     double k2[NC], k3[NC], k4[NC];              // h k_i
-    double d[4], a5[NC], ae[4];
+    double z[4], ae[4];                         // stage point / 5th-order solution; error sum
+    double dth5, dph5 = 0.0;                    // theta (and phi) increments of the 5th-order solution
+    z[1] = 0.0;
     // stage sin/cos and their h-scaled derivatives u = cos h theta', v = -sin h theta'
     const double s1 = S.sc.s, c1 = S.sc.c;
     const double hth1 = h*k1[1];
     const double u1 = c1*hth1, v1 = -s1*hth1;
     double u2, v2, u3, v3, u4, v4;
     SinCos<double> sc;
-    // the k1 terms take the tableau coefficient times h (one multiply per coefficient)
     double ch = C::a21*h;
-    for (int i = 0; i < 4; ++i) d[i] = ch*k1[i];
+    for (int i = 0; i < 4; ++i) if (i != 1) z[i] = fma(ch, k1[i], y[i]);
     sc.s = fma(C::a21, u1, s1); sc.c = fma(C::a21, v1, c1);
-    stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k2);
+    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k2);
     u2 = sc.c*k2[1]; v2 = -sc.s*k2[1];
     ch = C::a31*h;
-    for (int i = 0; i < 4; ++i) d[i] = fma(ch, k1[i], C::a32*k2[i]);
-    sc.s = s1 + (C::a31*u1 + C::a32*u2); sc.c = c1 + (C::a31*v1 + C::a32*v2);
-    stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k3);
+    for (int i = 0; i < 4; ++i) if (i != 1) z[i] = fma(C::a32, k2[i], fma(ch, k1[i], y[i]));
+    sc.s = fma(C::a32, u2, fma(C::a31, u1, s1)); sc.c = fma(C::a32, v2, fma(C::a31, v1, c1));
+    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k3);
     u3 = sc.c*k3[1]; v3 = -sc.s*k3[1];
     ch = C::a41*h;
-    for (int i = 0; i < 4; ++i) d[i] = fma(ch, k1[i], C::a42*k2[i] + C::a43*k3[i]);
-    sc.s = s1 + (C::a41*u1 + C::a42*u2 + C::a43*u3); sc.c = c1 + (C::a41*v1 + C::a42*v2 + C::a43*v3);
-    stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k4);
+    for (int i = 0; i < 4; ++i) if (i != 1) z[i] = fma(C::a43, k3[i], fma(C::a42, k2[i], fma(ch, k1[i], y[i])));
+    sc.s = fma(C::a43, u3, fma(C::a42, u2, fma(C::a41, u1, s1)));
+    sc.c = fma(C::a43, v3, fma(C::a42, v2, fma(C::a41, v1, c1)));
+    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k4);
     u4 = sc.c*k4[1]; v4 = -sc.s*k4[1];
     {
         double k5[NC];
         ch = C::a51*h;
-        for (int i = 0; i < 4; ++i) d[i] = fma(ch, k1[i], C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
-        sc.s = s1 + (C::a51*u1 + C::a52*u2 + C::a53*u3 + C::a54*u4);
-        sc.c = c1 + (C::a51*v1 + C::a52*v2 + C::a53*v3 + C::a54*v4);
-        stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k5);
+        for (int i = 0; i < 4; ++i)
+            if (i != 1) z[i] = fma(C::a54, k4[i], fma(C::a53, k3[i], fma(C::a52, k2[i], fma(ch, k1[i], y[i]))));
+        sc.s = fma(C::a54, u4, fma(C::a53, u3, fma(C::a52, u2, fma(C::a51, u1, s1))));
+        sc.c = fma(C::a54, v4, fma(C::a53, v3, fma(C::a52, v2, fma(C::a51, v1, c1))));
+        stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k5);
         const double u5 = sc.c*k5[1], v5 = -sc.s*k5[1];
-        sc.s = s1 + (C::a61*u1 + C::a62*u2 + C::a63*u3 + C::a64*u4 + C::a65*u5);
-        sc.c = c1 + (C::a61*v1 + C::a62*v2 + C::a63*v3 + C::a64*v4 + C::a65*v5);
+        sc.s = fma(C::a65, u5, fma(C::a64, u4, fma(C::a63, u3, fma(C::a62, u2, fma(C::a61, u1, s1)))));
+        sc.c = fma(C::a65, v5, fma(C::a64, v4, fma(C::a63, v3, fma(C::a62, v2, fma(C::a61, v1, c1)))));
         ch = C::a61*h;
         for (int i = 0; i < 4; ++i)
-            d[i] = fma(ch, k1[i], C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
+            if (i != 1)
+                z[i] = fma(C::a65, k5[i], fma(C::a64, k4[i], fma(C::a63, k3[i], fma(C::a62, k2[i], fma(ch, k1[i], y[i])))));
+        // stage 6 is evaluated at z; the 5th-order solution is accumulated in a5 meanwhile
+        double a5[4];
         ch = C::b1*h;
-        for (int i = 0; i < NC; ++i) a5[i] = fma(ch, k1[i], C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i]);
+        for (int i = 0; i < 4; ++i)
+            if (i != 1) a5[i] = fma(C::b5, k5[i], fma(C::b4, k4[i], fma(C::b3, k3[i], fma(ch, k1[i], y[i]))));
+        dth5 = fma(ch, k1[1], C::b3*k3[1] + C::b4*k4[1] + C::b5*k5[1]);
+        if (NC == 5) dph5 = fma(ch, k1[NC - 1], C::b3*k3[NC - 1] + C::b4*k4[NC - 1] + C::b5*k5[NC - 1]);
         ch = C::e1*h;
         for (int i = 0; i < 4; ++i) ae[i] = fma(ch, k1[i], C::e3*k3[i] + C::e4*k4[i] + C::e5*k5[i]);
-    }
-    {
         double k6[NC];
-        stage_given_h<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, h, k6);
-        for (int i = 0; i < NC; ++i) a5[i] += C::b6*k6[i];
-        for (int i = 0; i < 4; ++i) ae[i] += C::e6*k6[i];
+        stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k6);
+        for (int i = 0; i < 4; ++i) if (i != 1) z[i] = fma(C::b6, k6[i], a5[i]);
+        dth5 = fma(C::b6, k6[1], dth5);
+        if (NC == 5) dph5 = fma(C::b6, k6[NC - 1], dph5);
+        for (int i = 0; i < 4; ++i) ae[i] = fma(C::e6, k6[i], ae[i]);
     }
-    for (int i = 0; i < 4; ++i) d[i] = a5[i];
-    // FSAL stage at y5 in the stage precision, unscaled; its sin/cos becomes the next base
+    // FSAL stage at the 5th-order point, unscaled; its sin/cos becomes the next base
     double k7[5];
     SinCos<double> sc7;
-    if (PREC == PREC_FP64 || use64) {
-        stage_f64<CHART, 5>(P, y, d, S.sc, S.pt, S.pp, k7, sc7);
-    } else {
-        stage_f32<CHART, 5>(Pf, y, d, S.sc, S.pt, S.pp, k7);
-        sc7 = sc_at(S.sc, y[1], d[1]);
-    }
+    fsal_stage<CHART, PREC>(P, Pf, use64, z, y[1], dth5, S.sc, S.pt, S.pp, k7, sc7);
     count_rhs<PREC>(S, use64, 6);      // stages 2-6 and the FSAL stage
     // err^2 = mean_i (dy_i / (atol + rtol |y_i|))^2 in binary64 (as dp5_step)
     const double e7h = C::e7*h;
@@ -836,8 +862,8 @@ KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>
     }
     if (err2 <= 1.0 || h < 1e-10) {
         const double r0 = y[0], th0 = y[1], ph0 = S.ph, dr0 = S.k1[0], dth0 = S.k1[1];
-        if (NC == 5) S.ph += a5[NC - 1];
-        for (int i = 0; i < 4; ++i) S.y[i] += d[i];
+        if (NC == 5) S.ph += dph5;
+        S.y[0] = z[0]; S.y[1] = th0 + dth5; S.y[2] = z[2]; S.y[3] = z[3];
         for (int i = 0; i < 5; ++i) S.k1[i] = k7[i];
         S.sc = sc7;
         double hn = h * ((err2 > 1e-24) ? 0.9*pow_neg(err2, 0.1f) : 4.0);
