@@ -50,13 +50,16 @@ F_ROT = 31
 F_HERMITE = 370
 PEAK_VALU_TFLOPS = 78.6   # MI355X vector FP64; FP32 without packed math issues at the same rate
 
-# Adjoint reverse pass (kt_adjoint_backward_kernel, KS chart), per recorded accepted step:
+# Adjoint reverse pass (KS chart), per accepted step of one sweep:
 #   segment replay: 6 stages x (right-hand side + sin/cos rotation) + combinations (stage sums
 #   120, 5th-order sum 36, update 8, stage inputs 18 = 182);  the step's stage points again
 #   (same count);  6 reverse-mode vector-Jacobian products (F_VJP, counted by
 #   tests/hostcheck/flopcount.cpp) + the adjoint stage combinations (168 + 24 + 30 = 222).
 F_VJP = {"bl": 367, "ks": 348}
 F_ADJ_STEP = 2 * (6 * (F_RHS["ks"] + F_ROT) + 182) + 6 * F_VJP["ks"] + 222
+# The reverse pass runs the checkpoint segments in parallel (kt_adjoint_segments_kernel): every
+# step of a disk pixel is swept four times, once per basis direction of its segment's end adjoint.
+F_SEG_STEP = 4 * F_ADJ_STEP
 # Jacobi bundles (kt_bundle_kernel, BL, RK4 doubling), per attempt: the full step (3 plain
 # right-hand sides F_RHS["bl"] + rotations + sums: 3 x (F_RHS + 31 + 3) + 54), the two half steps (3 stages
 # each of one dual-number right-hand side F_DUAL_BL (counted by flopcount.cpp) + 4 for its sin/cos
@@ -140,15 +143,17 @@ def c5_run(ctx, kt, reps):
     wall = (time.perf_counter() - t0) / reps
     st = outs[-1]["stats"]
     bw_ms = float(np.mean([o["stats"]["backward_ms"] for o in outs]))
-    fl = st["steps"] * F_ADJ_STEP
+    fl = st["steps_differentiated"] * F_SEG_STEP
     ach = fl / (bw_ms * 1e-3) / 1e12
     return {"workload": "256x256 KN a=0.7 Q=0.2 KS theta=60 DOPRI5, dI/d(a,Q,Lambda,theta,r_obs) per pixel",
             "ms_per_call": 1e3 * wall, "forward_ms": st["forward_ms"], "record_ms": st["record_ms"],
             "backward_ms": bw_ms, "rays_differentiated": st["rays_differentiated"],
             "steps_recorded": st["steps"], "checkpoints": st["checkpoints"], "record_bytes": st["record_bytes"],
+            "steps_differentiated": st["steps_differentiated"], "reverse_entries": st["reverse_entries"],
             "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                         "frac": ach / PEAK_VALU_TFLOPS, "kernel": "kt_adjoint_backward_kernel<KS>",
-                         "flops_per_launch": fl, "flops_per_step": F_ADJ_STEP}}
+                         "frac": ach / PEAK_VALU_TFLOPS,
+                         "kernel": "reverse pass (kt_adjoint_segments_kernel<KS> + table + composition)",
+                         "flops_per_launch": fl, "flops_per_step": F_SEG_STEP}}
 
 
 def c3_run(ctx, kt, reps):

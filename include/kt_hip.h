@@ -280,6 +280,10 @@ typedef struct kt_adjoint_stats {
     int64_t record_bytes;     /* device bytes of the step sizes and checkpoints                */
     int64_t rays_differentiated;
     int64_t seed_failures;    /* disk pixels whose last step gave no root on replay            */
+    int64_t steps_differentiated; /* accepted steps of the disk pixels, swept by the reverse pass */
+    int64_t reverse_entries;  /* reverse-pass entries: checkpoint segments (four sweeps each,
+                                 one per basis direction of the adjoint at the segment end)
+                                 and one event entry per disk pixel                            */
 } kt_adjoint_stats;
 
 /* Hand-written adjoint of the adaptive integrator + disk shading.  Gradients of the per-pixel
@@ -288,7 +292,9 @@ typedef struct kt_adjoint_stats {
  * pixel (the context keeps these buffers between calls; a ray longer than the stride makes the
  * stride grow and the pass run once more), and the reverse pass replays each segment from its
  * checkpoint and propagates the adjoint back through the DOPRI5 stages (accepted step sizes
- * held fixed).
+ * held fixed).  The reverse pass runs the segments in parallel: each segment is swept once
+ * per basis direction of the adjoint at its end, and each pixel's chain is then composed from
+ * these small linear maps, last segment first.
  *   weights:       W*H values w (dL/dI), NULL for all ones
  *   intensity_out: W*H (may be NULL);  dI_dp_out: W*H*5, pixel-major (may be NULL)
  *   grad_out:      5 values, sum_p w_p dI_p/dp (summed on the device in a fixed order)      */

@@ -686,4 +686,125 @@ KT_HD bool adjoint_pixel(const FrameDev& F, int px, int py, int N, const double*
     return true;
 }
 
+// ---------------------------------------------------------------------------
+//  Reverse pass in parallel over checkpoint segments (the GPU's reverse pass).
+//
+//  The adjoint of a segment is linear in the adjoint at its end, so every segment's map is
+//  formed on its own, once per basis direction of that end adjoint: four independent sweeps
+//  (one lane each) give the transposed 4x4 state map and the 4x5 parameter rows of the
+//  segment.  The last step of a ray, which holds the disk event, is the pixel's "event entry":
+//  the seed propagated through that one step.  A pixel's gradient is then the event entry
+//  carried back through its segments' small matrices, last to first (adjoint_compose): the
+//  same quantity as adjoint_pixel's single sweep, summed in another order.
+// ---------------------------------------------------------------------------
+constexpr int kSegOut = 9;     // per sweep: adjoint at the segment start (4), p_t, p_phi (2), a, Q, Lambda (3)
+
+// checkpoint segments of a ray of N >= 1 accepted steps: steps [16k, min(16k + 16, N - 1))
+KT_HD int adjoint_segments(int N) { return (N - 1 + kCkptEvery - 1) / kCkptEvery; }
+
+// conserved momenta of the ray at pixel (px, py), by the forward pass's own camera path
+template <int CHART>
+KT_HD void ray_momenta(const FrameDev& F, int px, int py, double& pt, double& pp) {
+    double st[7];
+    camera_ray(F, (double)px, (double)py, st);
+    if (CHART == CHART_KS) bl_to_ks_covector(F, st);
+    pt = st[5];
+    pp = st[6];
+}
+
+// replay of the accepted steps n0 .. n1 - 1 from the checkpoint before step n0 (ck: that
+// checkpoint's 6 values); segy/segsc[i] = the state before step n0 + i, i = 0 .. n1 - n0
+template <int CHART>
+KT_HD void replay_steps(const Knds<double>& P, const double* hs, const double* ck, int n0, int n1, double pt,
+                        double pp, double segy[kCkptEvery + 1][4], SinCos<double> segsc[kCkptEvery + 1]) {
+    double y[4] = {ck[0], ck[1], ck[2], ck[3]};
+    SinCos<double> sc;
+    sc.s = ck[4];
+    sc.c = ck[5];
+    double k1[5];
+    rhs<CHART>(P, y[0], sc, y[2], y[3], pt, pp, k1);
+    for (int i = 0; i < 4; ++i) segy[0][i] = y[i];
+    segsc[0] = sc;
+    for (int n = n0; n < n1; ++n) {
+        dp5_replay<CHART>(P, y, sc, k1, hs[n], pt, pp);
+        for (int i = 0; i < 4; ++i) segy[n - n0 + 1][i] = y[i];
+        segsc[n - n0 + 1] = sc;
+    }
+}
+
+// Segment k (steps [16k, min(16k + 16, N - 1))) of a ray of N accepted steps, swept back from
+// the unit adjoint e_d at its end.  ck: the ray's checkpoints (6 per segment).
+template <int CHART>
+KT_HD void adjoint_segment(const FrameDev& F, double pt, double pp, int N, const double* hs, const double* ck,
+                           int k, int d, double out[kSegOut]) {
+    const double par[3] = {F.a, F.Q, F.Lambda};
+    const Knds<double>& P = F.P;
+    const int n0 = k*kCkptEvery;
+    const int n1 = (n0 + kCkptEvery < N - 1) ? n0 + kCkptEvery : N - 1;
+    double segy[kCkptEvery + 1][4];
+    SinCos<double> segsc[kCkptEvery + 1];
+    replay_steps<CHART>(P, hs, ck + 6*k, n0, n1, pt, pp, segy, segsc);
+    double lam[4] = {0.0, 0.0, 0.0, 0.0}, gc[2] = {0.0, 0.0}, gP[3] = {0.0, 0.0, 0.0};
+    lam[d] = 1.0;
+    for (int n = n1 - 1; n >= n0; --n)
+        dp5_step_adjoint<CHART>(F.M, par, P, segy[n - n0], segsc[n - n0], hs[n], pt, pp, lam, gc, gP);
+    for (int i = 0; i < 4; ++i) out[i] = lam[i];
+    out[4] = gc[0]; out[5] = gc[1];
+    out[6] = gP[0]; out[7] = gP[1]; out[8] = gP[2];
+}
+
+// The event entry of a ray of N >= 1 accepted steps: the disk seed on step N - 1 carried back
+// through that step.  out: the adjoint at the start of step N - 1 and the parameter sums.
+// Returns false when the replayed last step has no disk root.
+template <int CHART>
+KT_HD bool adjoint_event(const FrameDev& F, double pt, double pp, int N, const double* hs, const double* ck,
+                         double out[kSegOut]) {
+    const double par[3] = {F.a, F.Q, F.Lambda};
+    const Knds<double>& P = F.P;
+    const int k = (N - 1) / kCkptEvery, n0 = k*kCkptEvery;
+    double segy[kCkptEvery + 1][4];
+    SinCos<double> segsc[kCkptEvery + 1];
+    replay_steps<CHART>(P, hs, ck + 6*k, n0, N, pt, pp, segy, segsc);
+    const int n = N - 1;
+    double lam0[4], lam1[4], gc[2] = {0.0, 0.0}, gP[3] = {0.0, 0.0, 0.0}, inten = 0.0;
+    if (!event_seed<CHART>(F, segy[n - n0], segy[n - n0 + 1], hs[n], pt, pp, lam0, lam1, gc, gP, &inten))
+        return false;
+    dp5_step_adjoint<CHART>(F.M, par, P, segy[n - n0], segsc[n - n0], hs[n], pt, pp, lam1, gc, gP);
+    for (int i = 0; i < 4; ++i) out[i] = lam1[i] + lam0[i];
+    out[4] = gc[0]; out[5] = gc[1];
+    out[6] = gP[0]; out[7] = gP[1]; out[8] = gP[2];
+    return true;
+}
+
+// A pixel's gradient from its event entry ev and its nseg segments' sweeps seg (segment k,
+// direction d at seg + (4k + d) kSegOut): the end adjoint carried back segment by segment, then
+// through the camera map.
+template <int CHART>
+KT_HD void adjoint_compose(const FrameDev& F, int px, int py, int nseg, const double* ev, const double* seg,
+                           double grad[AP_N]) {
+    double al, be;
+    pixel_angles(F, (double)px, (double)py, al, be);
+    double y0[4], cc[2], Jy[4][AP_N], Jc[2][AP_N];
+    init_jacobian<CHART>(F, al, be, y0, cc, Jy, Jc);
+    double lam[4] = {ev[0], ev[1], ev[2], ev[3]}, gc[2] = {ev[4], ev[5]}, gP[3] = {ev[6], ev[7], ev[8]};
+    for (int k = nseg - 1; k >= 0; --k) {
+        const double* S = seg + 4*kSegOut*k;
+        double nl[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int d = 0; d < 4; ++d) {
+            const double* o = S + kSegOut*d;
+            for (int i = 0; i < 4; ++i) nl[i] = fma(lam[d], o[i], nl[i]);
+            gc[0] = fma(lam[d], o[4], gc[0]);
+            gc[1] = fma(lam[d], o[5], gc[1]);
+            for (int q = 0; q < 3; ++q) gP[q] = fma(lam[d], o[6 + q], gP[q]);
+        }
+        for (int i = 0; i < 4; ++i) lam[i] = nl[i];
+    }
+    for (int q = 0; q < AP_N; ++q) {
+        double g = gc[0]*Jc[0][q] + gc[1]*Jc[1][q];
+        for (int i = 0; i < 4; ++i) g += lam[i]*Jy[i][q];
+        if (q < 3) g += gP[q];
+        grad[q] = g;
+    }
+}
+
 }  // namespace kt

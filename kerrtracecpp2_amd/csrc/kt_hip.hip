@@ -650,28 +650,97 @@ struct AdjBackArgs {
     const double* hs;
     const double* ckpt;
     double* grad;                  // AP_N per pixel: dI/dp
-    unsigned long long* words;     // [0] rays differentiated, [1] seeds without a disk root
+    unsigned long long* words;     // [0] rays differentiated, [1] seeds without a disk root,
+                                   // [2] reverse-pass entries (segments + event entries),
+                                   // [3] accepted steps of the pixels with entries
+    int64_t* base;                 // per pixel: its first entry, -1 when it has none
+    int32_t* tab;                  // per entry: pixel, segment (-1: the event entry)
+    int64_t max_entries;           // capacity of tab (host bound from the step counts)
+    double* out;                   // per entry and basis direction: kSegOut values
 };
 
-// Reverse pass: one lane per pixel (adjoint_pixel: segments replayed from their
-// checkpoints, last first, into a local buffer, then walked backwards).
+// Reverse pass, 1 of 3: the entries of every disk pixel, nseg segments then its event entry, in
+// one contiguous range per pixel (the ranges' order depends on timing, their contents do not).
+__global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
+    unsigned long long steps = 0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < B.n_pix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int N = B.n_acc[p];
+        if (!(B.inten[p] > 0.0) || N < 1 || N > B.cap) { B.base[p] = -1; continue; }
+        const int nseg = adjoint_segments(N);
+        const int64_t b = (int64_t)atomicAdd(&B.words[2], (unsigned long long)(nseg + 1));
+        steps += (unsigned long long)N;
+        if (b + nseg + 1 > B.max_entries) { B.base[p] = -1; continue; }    // not reached: the host bound holds
+        B.base[p] = b;
+        for (int k = 0; k <= nseg; ++k) {
+            B.tab[2*(b + k)] = (int32_t)p;
+            B.tab[2*(b + k) + 1] = k < nseg ? k : -1;
+        }
+    }
+    steps = wave_sum(steps);
+    if ((threadIdx.x & 63) == 0 && steps) atomicAdd(&B.words[3], steps);
+}
+
+// Reverse pass, 2 of 3: four lanes per entry, side by side in the wave (lane d sweeps the basis
+// direction d of a segment; all four run an event entry, lane 0 stores it, lane 1 its status).
 template <int CHART>
-__global__ __launch_bounds__(256) void kt_adjoint_backward_kernel(AdjBackArgs B) {
+__global__ __launch_bounds__(256) void kt_adjoint_segments_kernel(AdjBackArgs B) {
     const FrameDev& F = *B.frame;
+    const int64_t used = (int64_t)B.words[2];
+    const int64_t lanes = 4 * (used < B.max_entries ? used : B.max_entries);
+    for (int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; L < lanes; L += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = L >> 2;
+        const int d = (int)(L & 3);
+        const int64_t p = B.tab[2*j];
+        const int k = B.tab[2*j + 1];
+        if (p < 0 || p >= B.n_pix) continue;
+        const int N = B.n_acc[p];
+        if (N < 1 || N > B.cap || k >= adjoint_segments(N)) continue;
+        double pt, pp;
+        ray_momenta<CHART>(F, (int)(p % F.W), (int)(p / F.W), pt, pp);
+        const double* hs = B.hs + p*B.cap;
+        const double* ck = B.ckpt + 6*p*(B.cap / kCkptEvery);
+        double o[kSegOut];
+        double* dst = B.out + (4*j + d)*kSegOut;
+        if (k >= 0) {
+            adjoint_segment<CHART>(F, pt, pp, N, hs, ck, k, d, o);
+            for (int i = 0; i < kSegOut; ++i) dst[i] = o[i];
+        } else {
+            const bool ok = adjoint_event<CHART>(F, pt, pp, N, hs, ck, o);
+            if (d == 0) for (int i = 0; i < kSegOut; ++i) dst[i] = o[i];
+            if (d == 1) dst[0] = ok ? 1.0 : 0.0;
+        }
+    }
+}
+
+// Reverse pass, 3 of 3: one lane per pixel, the event entry carried back through the segments.
+template <int CHART>
+__global__ __launch_bounds__(256) void kt_adjoint_compose_kernel(AdjBackArgs B) {
+    const FrameDev& F = *B.frame;
+    unsigned long long n_ok = 0, n_fail = 0;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < B.n_pix;
          p += (int64_t)gridDim.x * blockDim.x) {
         double* gout = B.grad + AP_N*p;
         for (int q = 0; q < AP_N; ++q) gout[q] = 0.0;
-        const int N = B.n_acc[p];
-        if (!(B.inten[p] > 0.0) || N < 1) continue;
-        double g[AP_N];
-        if (!adjoint_pixel<CHART>(F, (int)(p % F.W), (int)(p / F.W), N, B.hs + p*B.cap,
-                                  B.ckpt + 6*p*(B.cap / kCkptEvery), g)) {
-            atomicAdd(&B.words[1], 1ull);
+        const int64_t b = B.base[p];
+        if (b < 0) continue;
+        const int nseg = adjoint_segments(B.n_acc[p]);
+        const double* ev = B.out + 4*(b + nseg)*kSegOut;
+        if (!(ev[kSegOut] == 1.0)) {                 // lane 1 of the event entry: no disk root
+            ++n_fail;
             continue;
         }
+        double g[AP_N];
+        adjoint_compose<CHART>(F, (int)(p % F.W), (int)(p / F.W), nseg, ev, B.out + 4*b*kSegOut, g);
         for (int q = 0; q < AP_N; ++q) gout[q] = g[q];
-        atomicAdd(&B.words[0], 1ull);
+        ++n_ok;
+    }
+    // one atomic per wave: 40 000 per-lane atomics on one word serialise for about half a millisecond
+    n_ok = wave_sum(n_ok);
+    n_fail = wave_sum(n_fail);
+    if ((threadIdx.x & 63) == 0) {
+        if (n_ok) atomicAdd(&B.words[0], n_ok);
+        if (n_fail) atomicAdd(&B.words[1], n_fail);
     }
 }
 
@@ -892,6 +961,10 @@ struct kt_hip_ctx {
     double* d_adj_hs = nullptr; double* d_adj_ckpt = nullptr;
     double* d_adj_w = nullptr; double* d_adj_sum = nullptr;       // pixel weights, weighted sum
     int64_t adj_pix = 0; int32_t adj_cap = 0; int32_t adj_next_cap = 256;
+    // the reverse pass's entry table and per-entry sweeps (adj_entries entries), and each pixel's
+    // first entry (adj_base_pix pixels)
+    int64_t* d_adj_base = nullptr; int32_t* d_adj_tab = nullptr; double* d_adj_out = nullptr;
+    int64_t adj_entries = 0; int64_t adj_base_pix = 0;
 };
 
 static int fail(kt_hip_ctx* c, int code, const char* fmt, ...) {
@@ -1050,6 +1123,7 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     for (auto& kv : c->orders) (void)hipFree(kv.second);
     (void)hipFree(c->d_adj_nacc); (void)hipFree(c->d_adj_inten); (void)hipFree(c->d_adj_grad);
     (void)hipFree(c->d_adj_hs); (void)hipFree(c->d_adj_ckpt); (void)hipFree(c->d_adj_w); (void)hipFree(c->d_adj_sum);
+    (void)hipFree(c->d_adj_base); (void)hipFree(c->d_adj_tab); (void)hipFree(c->d_adj_out);
     for (int i = 0; i < 4; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2286,6 +2360,29 @@ int adjoint_buffers(kt_hip_ctx* c, int64_t pix, int32_t cap) {
     return KT_OK;
 }
 
+// The reverse pass's buffers for pix pixels and up to entries entries (segments + event
+// entries), kept in the context and grown as calls need.
+int adjoint_rev_buffers(kt_hip_ctx* c, int64_t pix, int64_t entries) {
+    if (pix > c->adj_base_pix) {
+        (void)hipFree(c->d_adj_base);
+        c->d_adj_base = nullptr;
+        c->adj_base_pix = 0;
+        if (hipMalloc(&c->d_adj_base, pix * 8) != hipSuccess)
+            return fail(c, KT_E_ALLOC, "adjoint entry bases for %lld pixels", (long long)pix);
+        c->adj_base_pix = pix;
+    }
+    if (entries > c->adj_entries) {
+        (void)hipFree(c->d_adj_tab); (void)hipFree(c->d_adj_out);
+        c->d_adj_tab = nullptr; c->d_adj_out = nullptr;
+        c->adj_entries = 0;
+        const size_t out_bytes = (size_t)entries * 4 * kSegOut * 8;
+        if (hipMalloc(&c->d_adj_tab, entries * 8) != hipSuccess || hipMalloc(&c->d_adj_out, out_bytes) != hipSuccess)
+            return fail(c, KT_E_ALLOC, "adjoint reverse-pass entries: %lld (%zu bytes)", (long long)entries, out_bytes);
+        c->adj_entries = entries;
+    }
+    return KT_OK;
+}
+
 typedef void (*AdjFwdFn)(TraceArgs);
 
 // One forward pass of the adjoint (counting when rec.hoff is null); times it with HIP events.
@@ -2406,7 +2503,12 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     S.checkpoints = nc;
     S.record_bytes = nh * 8 + nc * 48;
     float ms = 0.f;
-    // 3. reverse pass
+    // 2. reverse pass in parallel over checkpoint segments: entry table, four sweeps per entry,
+    //    then one composition per pixel.  The table's capacity bounds every pixel as a disk hit.
+    int64_t entries = 0;
+    for (int64_t i = 0; i < pix; ++i)
+        if (cnt[i] >= 1) entries += adjoint_segments(cnt[i]) + 1;
+    if ((rc = adjoint_rev_buffers(c, pix, entries > 0 ? entries : 1))) return rc;
     AdjBackArgs B;
     B.frame = c->table.frames();
     B.n_pix = pix;
@@ -2417,13 +2519,22 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     B.ckpt = c->d_adj_ckpt;
     B.grad = c->d_adj_grad;
     B.words = c->d_words;
+    B.base = c->d_adj_base;
+    B.tab = c->d_adj_tab;
+    B.max_entries = c->adj_entries;
+    B.out = c->d_adj_out;
     hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
     KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    const int64_t blocks = std::min<int64_t>((pix + 255) / 256, (int64_t)c->n_cu * 8);
-    if (chart == KT_CHART_KS)
-        hipLaunchKernelGGL(kt_adjoint_backward_kernel<CHART_KS>, dim3((unsigned)blocks), dim3(256), 0, c->stream, B);
-    else
-        hipLaunchKernelGGL(kt_adjoint_backward_kernel<CHART_BL>, dim3((unsigned)blocks), dim3(256), 0, c->stream, B);
+    const int64_t pblocks = std::min<int64_t>((pix + 255) / 256, (int64_t)c->n_cu * 8);
+    const int64_t sblocks = std::max<int64_t>(1, (4 * entries + 255) / 256);
+    hipLaunchKernelGGL(kt_adjoint_table_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+    if (chart == KT_CHART_KS) {
+        hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_KS>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
+        hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_KS>, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+    } else {
+        hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_BL>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
+        hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_BL>, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+    }
     KT_HIP(c, hipGetLastError());
     KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
     KT_HIP(c, hipStreamSynchronize(c->stream));
@@ -2433,6 +2544,8 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     KT_HIP(c, hipMemcpy(words, c->d_words, sizeof(words), hipMemcpyDeviceToHost));
     S.rays_differentiated = (int64_t)words[0];
     S.seed_failures = (int64_t)words[1];
+    S.reverse_entries = (int64_t)words[2];
+    S.steps_differentiated = (int64_t)words[3];
     if (grad_out) {
         if (w) KT_HIP(c, hipMemcpyAsync(c->d_adj_w, w, pix * 8, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(kt_adjoint_sum_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_grad,

@@ -47,6 +47,7 @@ def lib():
         L.ktc_trace_bundle_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
         L.ktc_sincos.argtypes = [P, ctypes.c_int64, P, P]
         L.ktc_adjoint_pixels.argtypes = [P, P, P, P, ctypes.c_int64, P, P, P, P]
+        L.ktc_adjoint_pixels_seg.argtypes = [P, P, P, P, ctypes.c_int64, P, P, P, P]
         L.ktc_camera_pair.argtypes = [P, P, P, P, ctypes.c_int64, P, P]
         D = ctypes.c_double
         L.ktc_rhs_jacobian.argtypes = [ctypes.c_int, D, P, P, D, D, P]
@@ -112,9 +113,10 @@ def sincos(x):
     return s, c
 
 
-def adjoint_pixels(frame, opts, px, py, want_grad=True):
+def adjoint_pixels(frame, opts, px, py, want_grad=True, segmented=False):
     """Intensity, dI/d(a, Q, Lambda, theta_deg, r_obs), accepted steps and (r_h, r_isco, r_esc,
-    chart, ks_ref_init) of the host build of the adjoint."""
+    chart, ks_ref_init) of the host build of the adjoint (segmented: through the GPU's
+    segment-parallel decomposition of the reverse pass)."""
     px = np.ascontiguousarray(px, np.int32)
     py = np.ascontiguousarray(py, np.int32)
     n = px.size
@@ -122,8 +124,8 @@ def adjoint_pixels(frame, opts, px, py, want_grad=True):
     grad = np.zeros((n, 5)) if want_grad else None
     nacc = np.zeros(n, np.int32)
     info = np.zeros(5)
-    lib().ktc_adjoint_pixels(ctypes.byref(frame), ctypes.byref(opts), _p(px), _p(py), n, _p(inten), _p(grad),
-                             _p(nacc), _p(info))
+    fn = lib().ktc_adjoint_pixels_seg if segmented else lib().ktc_adjoint_pixels
+    fn(ctypes.byref(frame), ctypes.byref(opts), _p(px), _p(py), n, _p(inten), _p(grad), _p(nacc), _p(info))
     return inten, grad, nacc, info
 
 

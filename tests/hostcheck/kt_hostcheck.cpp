@@ -171,7 +171,24 @@ extern "C" int ktc_sincos(const double* x, int64_t n, double* s, double* c) {
 // ---------------------------------------------------------------------------
 //  adjoint on the host: forward with records, then the kernels' reverse pass
 // ---------------------------------------------------------------------------
+// the GPU's segment-parallel reverse pass, run serially: four sweeps per checkpoint segment,
+// the event entry, then the composition
 template <int CHART>
+static bool adjoint_pixel_segmented(const FrameDev& F, int px, int py, int N, const double* hs, const double* ck,
+                                    double g[AP_N]) {
+    double pt, pp;
+    ray_momenta<CHART>(F, px, py, pt, pp);
+    const int nseg = adjoint_segments(N);
+    std::vector<double> seg((size_t)4 * kSegOut * (nseg > 0 ? nseg : 1));
+    for (int k = 0; k < nseg; ++k)
+        for (int d = 0; d < 4; ++d) adjoint_segment<CHART>(F, pt, pp, N, hs, ck, k, d, &seg[(size_t)(4*k + d)*kSegOut]);
+    double ev[kSegOut];
+    if (!adjoint_event<CHART>(F, pt, pp, N, hs, ck, ev)) return false;
+    adjoint_compose<CHART>(F, px, py, nseg, ev, seg.data(), g);
+    return true;
+}
+
+template <int CHART, bool SEG>
 static void adjoint_run(const FrameDev& F, const RunOpts& O, const int32_t* px, const int32_t* py, int64_t n,
                         double* inten, double* grad, int32_t* nacc) {
     const Knds<float> Pf = knds_cast<float>(F.P);
@@ -197,16 +214,18 @@ static void adjoint_run(const FrameDev& F, const RunOpts& O, const int32_t* px, 
         if (inten) inten[i] = I;
         if (nacc) nacc[i] = S.n_acc;
         double g[AP_N] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (grad && I > 0.0 && S.n_acc >= 1)
-            if (!adjoint_pixel<CHART>(F, px[i], py[i], S.n_acc, hs.data(), ck.data(), g))
-                for (int q = 0; q < AP_N; ++q) g[q] = std::nan("");
+        if (grad && I > 0.0 && S.n_acc >= 1) {
+            const bool ok = SEG ? adjoint_pixel_segmented<CHART>(F, px[i], py[i], S.n_acc, hs.data(), ck.data(), g)
+                                : adjoint_pixel<CHART>(F, px[i], py[i], S.n_acc, hs.data(), ck.data(), g);
+            if (!ok) for (int q = 0; q < AP_N; ++q) g[q] = std::nan("");
+        }
         if (grad) for (int q = 0; q < AP_N; ++q) grad[i*AP_N + q] = g[q];
     }
 }
 
 // intensity (and dI/dp when grad != NULL) of n pixels, binary64 DOPRI5
-extern "C" int ktc_adjoint_pixels(const kt_frame* f, const kt_opts* o, const int32_t* px, const int32_t* py, int64_t n,
-                                  double* inten, double* grad, int32_t* nacc, double* info5) {
+static int adjoint_pixels_impl(const kt_frame* f, const kt_opts* o, const int32_t* px, const int32_t* py, int64_t n,
+                               double* inten, double* grad, int32_t* nacc, double* info5, bool seg) {
     FrameDev F;
     fill_frame(*f, *o, F);
     RunOpts O;
@@ -217,9 +236,26 @@ extern "C" int ktc_adjoint_pixels(const kt_frame* f, const kt_opts* o, const int
     O.isl_noise = kMixedNoiseDefault;
     O.isl_r = kMixedRIslandDefault;
     if (info5) { info5[0] = F.r_h; info5[1] = F.r_isco; info5[2] = F.r_esc; info5[3] = F.chart; info5[4] = F.ks_ref_init; }
-    if (F.chart == CHART_KS) adjoint_run<CHART_KS>(F, O, px, py, n, inten, grad, nacc);
-    else adjoint_run<CHART_BL>(F, O, px, py, n, inten, grad, nacc);
+    if (F.chart == CHART_KS) {
+        if (seg) adjoint_run<CHART_KS, true>(F, O, px, py, n, inten, grad, nacc);
+        else adjoint_run<CHART_KS, false>(F, O, px, py, n, inten, grad, nacc);
+    } else {
+        if (seg) adjoint_run<CHART_BL, true>(F, O, px, py, n, inten, grad, nacc);
+        else adjoint_run<CHART_BL, false>(F, O, px, py, n, inten, grad, nacc);
+    }
     return 0;
+}
+
+// intensity (and dI/dp when grad != NULL) of n pixels: one sequential reverse sweep per pixel
+extern "C" int ktc_adjoint_pixels(const kt_frame* f, const kt_opts* o, const int32_t* px, const int32_t* py, int64_t n,
+                                  double* inten, double* grad, int32_t* nacc, double* info5) {
+    return adjoint_pixels_impl(f, o, px, py, n, inten, grad, nacc, info5, false);
+}
+
+// the same through the GPU's decomposition: segment sweeps per basis direction, composed
+extern "C" int ktc_adjoint_pixels_seg(const kt_frame* f, const kt_opts* o, const int32_t* px, const int32_t* py,
+                                      int64_t n, double* inten, double* grad, int32_t* nacc, double* info5) {
+    return adjoint_pixels_impl(f, o, px, py, n, inten, grad, nacc, info5, true);
 }
 
 // templated camera map (T = double) against the kernels' camera_ray + covector: 7 values per pixel each

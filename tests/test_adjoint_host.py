@@ -135,3 +135,33 @@ def test_rhs_vjp_equals_forward_jacobian(chart):
         g = hc.rhs_vjp(chart, 1.0, [a, Q, L], y, pt, pp, mu)
         scale = np.abs(J.T) @ np.abs(mu)
         assert np.all(np.abs(g - J.T @ mu) <= 1e-10 * scale + 1e-300)
+
+
+@pytest.mark.parametrize("chart", [0, 1])
+def test_segmented_reverse_pass_equals_the_sequential_sweep(chart):
+    """The GPU's reverse pass (each checkpoint segment swept once per basis direction of the
+    adjoint at its end, the last step with the disk seed as an entry of its own, then the
+    pixel's chain composed from these linear maps) against one sequential sweep per pixel: the
+    same derivative summed in another order."""
+    rng = np.random.default_rng(7 + chart)
+    o = _opts(chart)
+    cases = []
+    # the frame of the differentiable render, and a camera at 14 M whose disk rays end in a few steps
+    for kw in (dict(a=0.7, Q=0.2, theta=60), dict(a=0.7, Q=0.2, theta=75, r_obs=14.0, fov=60)):
+        fr = kt.make_frame(256, 256, **kw)
+        px, py = rng.integers(0, 256, 500), rng.integers(0, 256, 500)
+        I, G, N, _ = hc.adjoint_pixels(fr, o, px, py)
+        I2, G2, N2, _ = hc.adjoint_pixels(fr, o, px, py, segmented=True)
+        np.testing.assert_array_equal(N, N2)
+        np.testing.assert_array_equal(I, I2)
+        m = I > 0
+        assert m.sum() > 100
+        assert np.isfinite(G[m]).all() and np.isfinite(G2[m]).all()
+        scale = np.abs(G[m]).max(axis=0)
+        assert np.all(np.abs(G2[m] - G[m]) <= 1e-9 * np.abs(G[m]) + 1e-12 * scale)
+        cases.append(N[m])
+    n = np.concatenate(cases)
+    r = (n - 1) % 16
+    # the split's edge cases are among them: the event in the first segment (no segment before it),
+    # a last segment of 16 steps with a one-step event entry, and a 16-step event replay
+    assert (n <= 16).any() and (n > 16).any() and (r == 0).any() and (r == 15).any()
