@@ -712,42 +712,60 @@ KT_HD void ray_momenta(const FrameDev& F, int px, int py, double& pt, double& pp
     pp = st[6];
 }
 
+// The replayed states of one segment.  LocalStates keeps them in the lane (the host build);
+// the GPU's segment kernel passes a store in LDS that the four sweeps of a segment share (its
+// first lane writes, all four read after sync()), which keeps them out of scratch.
+struct LocalStates {
+    double y[kCkptEvery + 1][4];
+    SinCos<double> sc[kCkptEvery + 1];
+    KT_HD void put(int i, const double v[4], const SinCos<double>& s) {
+        for (int k = 0; k < 4; ++k) y[i][k] = v[k];
+        sc[i] = s;
+    }
+    KT_HD void get(int i, double v[4], SinCos<double>& s) const {
+        for (int k = 0; k < 4; ++k) v[k] = y[i][k];
+        s = sc[i];
+    }
+    KT_HD void sync() {}
+};
+
 // replay of the accepted steps n0 .. n1 - 1 from the checkpoint before step n0 (ck: that
-// checkpoint's 6 values); segy/segsc[i] = the state before step n0 + i, i = 0 .. n1 - n0
-template <int CHART>
+// checkpoint's 6 values); state i of the store = the state before step n0 + i, i = 0 .. n1 - n0
+template <int CHART, class Store>
 KT_HD void replay_steps(const Knds<double>& P, const double* hs, const double* ck, int n0, int n1, double pt,
-                        double pp, double segy[kCkptEvery + 1][4], SinCos<double> segsc[kCkptEvery + 1]) {
+                        double pp, Store& st) {
     double y[4] = {ck[0], ck[1], ck[2], ck[3]};
     SinCos<double> sc;
     sc.s = ck[4];
     sc.c = ck[5];
     double k1[5];
     rhs<CHART>(P, y[0], sc, y[2], y[3], pt, pp, k1);
-    for (int i = 0; i < 4; ++i) segy[0][i] = y[i];
-    segsc[0] = sc;
+    st.put(0, y, sc);
     for (int n = n0; n < n1; ++n) {
         dp5_replay<CHART>(P, y, sc, k1, hs[n], pt, pp);
-        for (int i = 0; i < 4; ++i) segy[n - n0 + 1][i] = y[i];
-        segsc[n - n0 + 1] = sc;
+        st.put(n - n0 + 1, y, sc);
     }
+    st.sync();
 }
 
 // Segment k (steps [16k, min(16k + 16, N - 1))) of a ray of N accepted steps, swept back from
 // the unit adjoint e_d at its end.  ck: the ray's checkpoints (6 per segment).
-template <int CHART>
+template <int CHART, class Store>
 KT_HD void adjoint_segment(const FrameDev& F, double pt, double pp, int N, const double* hs, const double* ck,
-                           int k, int d, double out[kSegOut]) {
+                           int k, int d, Store& st, double out[kSegOut]) {
     const double par[3] = {F.a, F.Q, F.Lambda};
     const Knds<double>& P = F.P;
     const int n0 = k*kCkptEvery;
     const int n1 = (n0 + kCkptEvery < N - 1) ? n0 + kCkptEvery : N - 1;
-    double segy[kCkptEvery + 1][4];
-    SinCos<double> segsc[kCkptEvery + 1];
-    replay_steps<CHART>(P, hs, ck + 6*k, n0, n1, pt, pp, segy, segsc);
+    replay_steps<CHART>(P, hs, ck + 6*k, n0, n1, pt, pp, st);
     double lam[4] = {0.0, 0.0, 0.0, 0.0}, gc[2] = {0.0, 0.0}, gP[3] = {0.0, 0.0, 0.0};
     lam[d] = 1.0;
-    for (int n = n1 - 1; n >= n0; --n)
-        dp5_step_adjoint<CHART>(F.M, par, P, segy[n - n0], segsc[n - n0], hs[n], pt, pp, lam, gc, gP);
+    for (int n = n1 - 1; n >= n0; --n) {
+        double y[4];
+        SinCos<double> sc;
+        st.get(n - n0, y, sc);
+        dp5_step_adjoint<CHART>(F.M, par, P, y, sc, hs[n], pt, pp, lam, gc, gP);
+    }
     for (int i = 0; i < 4; ++i) out[i] = lam[i];
     out[4] = gc[0]; out[5] = gc[1];
     out[6] = gP[0]; out[7] = gP[1]; out[8] = gP[2];
@@ -756,20 +774,21 @@ KT_HD void adjoint_segment(const FrameDev& F, double pt, double pp, int N, const
 // The event entry of a ray of N >= 1 accepted steps: the disk seed on step N - 1 carried back
 // through that step.  out: the adjoint at the start of step N - 1 and the parameter sums.
 // Returns false when the replayed last step has no disk root.
-template <int CHART>
+template <int CHART, class Store>
 KT_HD bool adjoint_event(const FrameDev& F, double pt, double pp, int N, const double* hs, const double* ck,
-                         double out[kSegOut]) {
+                         Store& st, double out[kSegOut]) {
     const double par[3] = {F.a, F.Q, F.Lambda};
     const Knds<double>& P = F.P;
     const int k = (N - 1) / kCkptEvery, n0 = k*kCkptEvery;
-    double segy[kCkptEvery + 1][4];
-    SinCos<double> segsc[kCkptEvery + 1];
-    replay_steps<CHART>(P, hs, ck + 6*k, n0, N, pt, pp, segy, segsc);
+    replay_steps<CHART>(P, hs, ck + 6*k, n0, N, pt, pp, st);
     const int n = N - 1;
+    double y0[4], y1[4];
+    SinCos<double> sc0, sc1;
+    st.get(n - n0, y0, sc0);
+    st.get(n - n0 + 1, y1, sc1);
     double lam0[4], lam1[4], gc[2] = {0.0, 0.0}, gP[3] = {0.0, 0.0, 0.0}, inten = 0.0;
-    if (!event_seed<CHART>(F, segy[n - n0], segy[n - n0 + 1], hs[n], pt, pp, lam0, lam1, gc, gP, &inten))
-        return false;
-    dp5_step_adjoint<CHART>(F.M, par, P, segy[n - n0], segsc[n - n0], hs[n], pt, pp, lam1, gc, gP);
+    if (!event_seed<CHART>(F, y0, y1, hs[n], pt, pp, lam0, lam1, gc, gP, &inten)) return false;
+    dp5_step_adjoint<CHART>(F.M, par, P, y0, sc0, hs[n], pt, pp, lam1, gc, gP);
     for (int i = 0; i < 4; ++i) out[i] = lam1[i] + lam0[i];
     out[4] = gc[0]; out[5] = gc[1];
     out[6] = gP[0]; out[7] = gP[1]; out[8] = gP[2];

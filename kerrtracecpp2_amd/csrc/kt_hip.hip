@@ -683,9 +683,38 @@ __global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
 
 // Reverse pass, 2 of 3: four lanes per entry, side by side in the wave (lane d sweeps the basis
 // direction d of a segment; all four run an event entry, lane 0 stores it, lane 1 its status).
+// A segment's replayed states in LDS, shared by its four sweeps (lanes 4s .. 4s + 3 of the
+// block): lane 4s writes, and the four read after a wavefront-scope fence (the four lanes are
+// in one wave, whose LDS operations complete in order).
+struct LdsStates {
+    double* slot;            // 6 (kCkptEvery + 1) doubles
+    bool writer;
+    __device__ void put(int i, const double v[4], const SinCos<double>& s) {
+        if (!writer) return;
+        double* q = slot + 6*i;
+        q[0] = v[0]; q[1] = v[1]; q[2] = v[2]; q[3] = v[3]; q[4] = s.s; q[5] = s.c;
+    }
+    __device__ void get(int i, double v[4], SinCos<double>& s) const {
+        const double* q = slot + 6*i;
+        v[0] = q[0]; v[1] = q[1]; v[2] = q[2]; v[3] = q[3]; s.s = q[4]; s.c = q[5];
+    }
+    __device__ void sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+// Two waves per SIMD (256 VGPRs; the replay's stage arrays spill a little): 1.88 ms against
+// 2.26 ms at the compiler's own budget of 367 VGPRs, one wave per SIMD (C5 reverse pass).
 template <int CHART>
-__global__ __launch_bounds__(256) void kt_adjoint_segments_kernel(AdjBackArgs B) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void kt_adjoint_segments_kernel(AdjBackArgs B) {
     const FrameDev& F = *B.frame;
+    __shared__ double states[64][6 * (kCkptEvery + 1)];
+    LdsStates st;
+    st.slot = states[threadIdx.x >> 2];
+    st.writer = (threadIdx.x & 3) == 0;
     const int64_t used = (int64_t)B.words[2];
     const int64_t lanes = 4 * (used < B.max_entries ? used : B.max_entries);
     for (int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; L < lanes; L += (int64_t)gridDim.x * blockDim.x) {
@@ -703,10 +732,10 @@ __global__ __launch_bounds__(256) void kt_adjoint_segments_kernel(AdjBackArgs B)
         double o[kSegOut];
         double* dst = B.out + (4*j + d)*kSegOut;
         if (k >= 0) {
-            adjoint_segment<CHART>(F, pt, pp, N, hs, ck, k, d, o);
+            adjoint_segment<CHART>(F, pt, pp, N, hs, ck, k, d, st, o);
             for (int i = 0; i < kSegOut; ++i) dst[i] = o[i];
         } else {
-            const bool ok = adjoint_event<CHART>(F, pt, pp, N, hs, ck, o);
+            const bool ok = adjoint_event<CHART>(F, pt, pp, N, hs, ck, st, o);
             if (d == 0) for (int i = 0; i < kSegOut; ++i) dst[i] = o[i];
             if (d == 1) dst[0] = ok ? 1.0 : 0.0;
         }

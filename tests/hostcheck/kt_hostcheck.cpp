@@ -180,10 +180,12 @@ static bool adjoint_pixel_segmented(const FrameDev& F, int px, int py, int N, co
     ray_momenta<CHART>(F, px, py, pt, pp);
     const int nseg = adjoint_segments(N);
     std::vector<double> seg((size_t)4 * kSegOut * (nseg > 0 ? nseg : 1));
+    LocalStates st;
     for (int k = 0; k < nseg; ++k)
-        for (int d = 0; d < 4; ++d) adjoint_segment<CHART>(F, pt, pp, N, hs, ck, k, d, &seg[(size_t)(4*k + d)*kSegOut]);
+        for (int d = 0; d < 4; ++d)
+            adjoint_segment<CHART>(F, pt, pp, N, hs, ck, k, d, st, &seg[(size_t)(4*k + d)*kSegOut]);
     double ev[kSegOut];
-    if (!adjoint_event<CHART>(F, pt, pp, N, hs, ck, ev)) return false;
+    if (!adjoint_event<CHART>(F, pt, pp, N, hs, ck, st, ev)) return false;
     adjoint_compose<CHART>(F, px, py, nseg, ev, seg.data(), g);
     return true;
 }
