@@ -681,6 +681,33 @@ __global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
     if ((threadIdx.x & 63) == 0 && steps) atomicAdd(&B.words[3], steps);
 }
 
+// After the forward pass: the longest ray (does the record stride hold it?), the number of
+// reverse-pass entries if every ray ended on the disk (the entry table's capacity), and the
+// steps and checkpoints recorded, so that the host reads 32 bytes instead of every pixel's
+// count.  One block of 1024 threads.
+__global__ __launch_bounds__(1024) void kt_adjoint_count_kernel(const int32_t* n_acc, int64_t n, int64_t* info) {
+    __shared__ int64_t acc[4][1024];
+    const int t = threadIdx.x;
+    int64_t m = 0, e = 0, st = 0, ck = 0;
+    for (int64_t i = t; i < n; i += 1024) {
+        const int N = n_acc[i];
+        if (N > m) m = N;
+        if (N >= 1) e += adjoint_segments(N) + 1;
+        st += N > 0 ? N : 0;
+        ck += N > 0 ? (N + kCkptEvery - 1) / kCkptEvery : 1;
+    }
+    acc[0][t] = m; acc[1][t] = e; acc[2][t] = st; acc[3][t] = ck;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            if (acc[0][t + s] > acc[0][t]) acc[0][t] = acc[0][t + s];
+            for (int k = 1; k < 4; ++k) acc[k][t] += acc[k][t + s];
+        }
+        __syncthreads();
+    }
+    if (t < 4) info[t] = acc[t][0];
+}
+
 // Reverse pass, 2 of 3: four lanes per entry, side by side in the wave (lane d sweeps the basis
 // direction d of a segment; all four run an event entry, lane 0 stores it, lane 1 its status).
 // A segment's replayed states in LDS, shared by its four sweeps (lanes 4s .. 4s + 3 of the
@@ -774,27 +801,38 @@ __global__ __launch_bounds__(256) void kt_adjoint_compose_kernel(AdjBackArgs B) 
 }
 
 // sum over pixels of w_p dI_p/dp (w = nullptr: all ones; pixels with w_p = 0 skipped) in a fixed
-// order: thread t takes pixels t, t + 1024, ... in turn, then a fixed pairwise tree in LDS, so the
-// result does not depend on the launch or on timing
-__global__ __launch_bounds__(1024) void kt_adjoint_sum_kernel(const double* grad, const double* w, int64_t n,
-                                                               double* out) {
-    __shared__ double part[AP_N][1024];
+// order, independent of timing: stage 1, kSumBlocks blocks of 256 threads, block b takes the
+// pixel range [b n / kSumBlocks, (b + 1) n / kSumBlocks), thread t every 256th pixel of it, then a
+// fixed pairwise tree in LDS; stage 2, one block adds the partial sums in block order.
+constexpr int kSumBlocks = 64;
+__global__ __launch_bounds__(256) void kt_adjoint_sum_kernel(const double* grad, const double* w, int64_t n,
+                                                             double* partial) {
+    __shared__ double part[AP_N][256];
     const int t = threadIdx.x;
+    const int64_t lo = n * blockIdx.x / kSumBlocks, hi = n * (blockIdx.x + 1) / kSumBlocks;
     double acc[AP_N] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int64_t i = t; i < n; i += 1024) {
+    for (int64_t i = lo + t; i < hi; i += 256) {
         const double wi = w ? w[i] : 1.0;
         if (wi == 0.0) continue;
         for (int q = 0; q < AP_N; ++q) acc[q] = fma(wi, grad[i*AP_N + q], acc[q]);
     }
     for (int q = 0; q < AP_N; ++q) part[q][t] = acc[q];
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
         if (t < s)
             for (int q = 0; q < AP_N; ++q) part[q][t] += part[q][t + s];
         __syncthreads();
     }
     if (t == 0)
-        for (int q = 0; q < AP_N; ++q) out[q] = part[q][0];
+        for (int q = 0; q < AP_N; ++q) partial[blockIdx.x*AP_N + q] = part[q][0];
+}
+
+__global__ __launch_bounds__(64) void kt_adjoint_sum_final_kernel(const double* partial, double* out) {
+    if (threadIdx.x < AP_N) {
+        double s = 0.0;
+        for (int b = 0; b < kSumBlocks; ++b) s += partial[b*AP_N + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -993,6 +1031,8 @@ struct kt_hip_ctx {
     // the reverse pass's entry table and per-entry sweeps (adj_entries entries), and each pixel's
     // first entry (adj_base_pix pixels)
     int64_t* d_adj_base = nullptr; int32_t* d_adj_tab = nullptr; double* d_adj_out = nullptr;
+    double* d_adj_part = nullptr;           // the weighted sum's per-block partials
+    int64_t* d_adj_info = nullptr;          // longest ray, entry-table capacity (after a forward pass)
     int64_t adj_entries = 0; int64_t adj_base_pix = 0;
 };
 
@@ -1153,6 +1193,7 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     (void)hipFree(c->d_adj_nacc); (void)hipFree(c->d_adj_inten); (void)hipFree(c->d_adj_grad);
     (void)hipFree(c->d_adj_hs); (void)hipFree(c->d_adj_ckpt); (void)hipFree(c->d_adj_w); (void)hipFree(c->d_adj_sum);
     (void)hipFree(c->d_adj_base); (void)hipFree(c->d_adj_tab); (void)hipFree(c->d_adj_out);
+    (void)hipFree(c->d_adj_part); (void)hipFree(c->d_adj_info);
     for (int i = 0; i < 4; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2414,8 +2455,9 @@ int adjoint_rev_buffers(kt_hip_ctx* c, int64_t pix, int64_t entries) {
 
 typedef void (*AdjFwdFn)(TraceArgs);
 
-// One forward pass of the adjoint (counting when rec.hoff is null); times it with HIP events.
-int adjoint_forward(kt_hip_ctx* c, const kt_opts* o, int chart, int64_t rays, const AdjRecord& rec, float* ms) {
+// One forward pass of the adjoint, enqueued between the events ev[0] and ev[1] (the caller
+// reads their elapsed time after its next synchronisation).
+int adjoint_forward(kt_hip_ctx* c, const kt_opts* o, int chart, int64_t rays, const AdjRecord& rec) {
     Launch L = launch_config(c, o, chart);
     int64_t grid = (int64_t)c->n_cu * (L.bpc > 0 ? L.bpc : kDefaultBlocksPerCu);
     const int64_t max_useful = (rays + kBlock - 1) / kBlock;
@@ -2437,8 +2479,6 @@ int adjoint_forward(kt_hip_ctx* c, const kt_opts* o, int chart, int64_t rays, co
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(kBlock), 0, c->stream, A);
     KT_HIP(c, hipGetLastError());
     KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    KT_HIP(c, hipStreamSynchronize(c->stream));
-    (void)hipEventElapsedTime(ms, c->ev[0], c->ev[1]);
     return KT_OK;
 }
 
@@ -2479,9 +2519,11 @@ int kt_hip_render_intensity(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin
     std::memset(&rec, 0, sizeof(rec));
     rec.n_acc = n_acc.as<int32_t>();
     rec.inten = inten.as<double>();
-    float ms = 0.f;
-    rc = adjoint_forward(c, &o, chart, rays, rec, &ms);
+    rc = adjoint_forward(c, &o, chart, rays, rec);
     if (rc) return rc;
+    KT_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     rc = read_stats(c, c->d_words, ms, ms, st);
     if (rc) return rc;
     KT_HIP(c, hipMemcpy(I_out, rec.inten, pix * 8, hipMemcpyDeviceToHost));
@@ -2501,7 +2543,9 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     // 1. one forward pass: counts, intensities and the records at the context's current stride;
     //    a ray longer than the stride makes the host enlarge it and run the pass again
     int32_t cap = c->adj_next_cap;
-    std::vector<int32_t> cnt((size_t)pix);
+    if (!c->d_adj_info && hipMalloc(&c->d_adj_info, 4 * sizeof(int64_t)) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "adjoint counters");
+    int64_t info[4] = {0, 0, 0, 0};
     for (int pass = 0; pass < 2; ++pass) {
         if ((rc = adjoint_buffers(c, pix, cap))) return rc;
         AdjRecord rec;
@@ -2510,33 +2554,32 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         rec.cap = cap;
         rec.hs = c->d_adj_hs;
         rec.ckpt = c->d_adj_ckpt;
-        float ms = 0.f;
-        rc = adjoint_forward(c, &o, chart, rays, rec, &ms);
+        rc = adjoint_forward(c, &o, chart, rays, rec);
         if (rc) return rc;
+        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc, pix,
+                           c->d_adj_info);
+        KT_HIP(c, hipGetLastError());
+        KT_HIP(c, hipMemcpyAsync(info, c->d_adj_info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
+        KT_HIP(c, hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
         if (pass == 0) S.forward_ms = ms; else S.record_ms = ms;
-        KT_HIP(c, hipMemcpy(cnt.data(), c->d_adj_nacc, pix * 4, hipMemcpyDeviceToHost));
-        int32_t nmax = 0;
-        for (int64_t i = 0; i < pix; ++i) nmax = std::max(nmax, cnt[i]);
+        const int64_t nmax = info[0];
         if (nmax <= cap) break;
-        if (pass == 1) return fail(c, KT_E_DEVICE, "adjoint records: %d steps after enlarging to %d", nmax, cap);
-        cap = (nmax + kCkptEvery - 1) / kCkptEvery * kCkptEvery;
+        if (pass == 1)
+            return fail(c, KT_E_DEVICE, "adjoint records: %lld steps after enlarging to %d", (long long)nmax, cap);
+        if (nmax > (int64_t)1 << 30) return fail(c, KT_E_DEVICE, "adjoint records: %lld steps", (long long)nmax);
+        cap = (int32_t)((nmax + kCkptEvery - 1) / kCkptEvery * kCkptEvery);
         c->adj_next_cap = cap;
     }
-    int64_t nh = 0, nc = 0;
-    for (int64_t i = 0; i < pix; ++i) {
-        const int64_t n = cnt[i] > 0 ? cnt[i] : 0;
-        nh += n;
-        nc += n > 0 ? (n + kCkptEvery - 1) / kCkptEvery : 1;
-    }
+    const int64_t nh = info[2], nc = info[3];
     S.steps = nh;
     S.checkpoints = nc;
     S.record_bytes = nh * 8 + nc * 48;
     float ms = 0.f;
     // 2. reverse pass in parallel over checkpoint segments: entry table, four sweeps per entry,
     //    then one composition per pixel.  The table's capacity bounds every pixel as a disk hit.
-    int64_t entries = 0;
-    for (int64_t i = 0; i < pix; ++i)
-        if (cnt[i] >= 1) entries += adjoint_segments(cnt[i]) + 1;
+    const int64_t entries = info[1];
     if ((rc = adjoint_rev_buffers(c, pix, entries > 0 ? entries : 1))) return rc;
     AdjBackArgs B;
     B.frame = c->table.frames();
@@ -2577,8 +2620,11 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     S.steps_differentiated = (int64_t)words[3];
     if (grad_out) {
         if (w) KT_HIP(c, hipMemcpyAsync(c->d_adj_w, w, pix * 8, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(kt_adjoint_sum_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_grad,
-                           w ? c->d_adj_w : nullptr, pix, c->d_adj_sum);
+        if (!c->d_adj_part && hipMalloc(&c->d_adj_part, kSumBlocks * AP_N * 8) != hipSuccess)
+            return fail(c, KT_E_ALLOC, "adjoint partial sums");
+        hipLaunchKernelGGL(kt_adjoint_sum_kernel, dim3(kSumBlocks), dim3(256), 0, c->stream, c->d_adj_grad,
+                           w ? c->d_adj_w : nullptr, pix, c->d_adj_part);
+        hipLaunchKernelGGL(kt_adjoint_sum_final_kernel, dim3(1), dim3(64), 0, c->stream, c->d_adj_part, c->d_adj_sum);
         KT_HIP(c, hipGetLastError());
         // the context stream does not synchronise with the null stream: copy on it and wait
         KT_HIP(c, hipMemcpyAsync(grad_out, c->d_adj_sum, AP_N * 8, hipMemcpyDeviceToHost, c->stream));
