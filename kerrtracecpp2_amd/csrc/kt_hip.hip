@@ -412,6 +412,7 @@ struct GeodesicPolicy {
     __device__ static const RayState& geo(const State& S) { return S; }
 };
 
+template <bool SCI>
 struct BundlePolicy {
     typedef BundleState State;
     static constexpr bool kUnconditional = false;
@@ -422,7 +423,7 @@ struct BundlePolicy {
     template <bool SINGLE>
     __device__ static bool advance(const TraceArgs& A, const FrameDev& F, const Knds<double>&, const Knds<float>&,
                                    State& B, RayOut& o, double& magnif, bool) {
-        return bundle_step(F, A.O, B, o, magnif) == STEP_DONE;
+        return bundle_step<SCI>(F, A.O, B, o, magnif) == STEP_DONE;
     }
     __device__ static const RayState& geo(const State& B) { return B.g; }
 };
@@ -607,10 +608,10 @@ void kt_trace_inline_kernel(TraceArgs A) {
 // attempt keeps the geodesic, the 4x2 deviation matrix, its derivative and the old state
 // for a reject live across dual-number right-hand sides of ~110 registers each: about
 // 390 registers, so OCC = 1 (512 per lane) holds it without scratch, OCC = 2 spills.
-template <bool SINGLE, int OCC>
+template <bool SINGLE, int OCC, bool SCI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
 void kt_bundle_kernel(TraceArgs A) {
-    trace_loop<BundlePolicy, SINGLE>(A);
+    trace_loop<BundlePolicy<SCI>, SINGLE>(A);
 }
 
 // Initial states of every pixel (camera ray, covector of the chart, sin/cos, first RHS).
@@ -1452,10 +1453,14 @@ static RunOpts run_opts(const kt_opts* o) {
 // autotuner's result for this option class, else the built-in defaults
 struct Launch { int bpc, chunk, occ; };
 
-// bundle kernels: occupancy 1 (default) or 2
-static TraceKernelFn pick_bundle_kernel(int occ, bool single) {
-    if (occ == 2) return single ? kt_bundle_kernel<true, 2> : kt_bundle_kernel<false, 2>;
-    return single ? kt_bundle_kernel<true, 1> : kt_bundle_kernel<false, 1>;
+// bundle kernels: occupancy 1 (default) or 2; integrated stage sin/cos unless the root-skip quirk is set
+static TraceKernelFn pick_bundle_kernel(int occ, bool single, uint32_t quirks) {
+    if (quirks & KT_QUIRK_BUNDLE_ROOT_SKIP) {
+        if (occ == 2) return single ? kt_bundle_kernel<true, 2, false> : kt_bundle_kernel<false, 2, false>;
+        return single ? kt_bundle_kernel<true, 1, false> : kt_bundle_kernel<false, 1, false>;
+    }
+    if (occ == 2) return single ? kt_bundle_kernel<true, 2, true> : kt_bundle_kernel<false, 2, true>;
+    return single ? kt_bundle_kernel<true, 1, true> : kt_bundle_kernel<false, 1, true>;
 }
 
 static Launch launch_config(const kt_hip_ctx* c, const kt_opts* o, int chart) {
@@ -1522,7 +1527,7 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
     const bool phi = !(o->integrator == KT_INTEGRATOR_DOPRI5 && (o->quirks & KT_QUIRK_DOPRI5_PHI_FROZEN));
     // no init records (the allocation-free path): the multi-frame kernel that runs the camera itself
     const bool inline_init = !o->bundles && B.init == nullptr;
-    TraceKernelFn fn = o->bundles ? pick_bundle_kernel(L.occ, single)
+    TraceKernelFn fn = o->bundles ? pick_bundle_kernel(L.occ, single, o->quirks)
                        : inline_init ? pick_inline_kernel(chart, o->integrator, o->precision, phi)
                                      : pick_kernel(chart, o->integrator, o->precision, phi, L.occ, single);
     if (!fn) return fail(c, KT_E_UNSUPPORTED, "no kernel for this option set");

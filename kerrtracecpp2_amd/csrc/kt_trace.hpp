@@ -880,17 +880,40 @@ KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>
 
 // Classic RK4 over [0, h] from y0 with f(y0) = k0 (geodesic.hpp:169-194): returns
 // the increment of (r, theta, p_r, p_theta, phi).
-template <int CHART, int PREC>
+// SCI = false: each stage rotates the base sin/cos to its theta (the trace kernels' RK4, whose
+//   step sequence then follows the reference's closely enough that escape angles, interpolated
+//   inside the escaping step, match its images);
+// SCI = true: as in the DOPRI5 attempt, stages 2-4 take (sin theta, cos theta) as two more
+//   integrated components, d sin/dl = cos theta', d cos/dl = -sin theta', with the RK4 stage
+//   offsets (a 4th-order method for the augmented system) instead of a rotation series per
+//   stage (the bundle kernel).  The accepted point's sin/cos is rotated exactly by the caller.
+template <int CHART, int PREC, bool SCI = false>
 KT_HD void rk4_incr(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y0[4],
                     const SinCos<double>& sc0, const double k0[5], double h, double pt, double pp,
                     double inc[5]) {
     double k2[5], k3[5], k4[5], d[4];
-    for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k0[i];
-    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k2);
-    for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k2[i];
-    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k3);
-    for (int i = 0; i < 4; ++i) d[i] = h*k3[i];
-    stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k4);
+    if (SCI) {
+        const double hh = 0.5*h;
+        SinCos<double> sc, sp;
+        for (int i = 0; i < 4; ++i) d[i] = hh*k0[i];
+        sc.s = fma(hh, sc0.c*k0[1], sc0.s); sc.c = fma(-hh, sc0.s*k0[1], sc0.c);
+        stage_given<CHART, PREC, 5>(P, Pf, use64, y0, d, sc, pt, pp, k2);
+        sp = sc;
+        for (int i = 0; i < 4; ++i) d[i] = hh*k2[i];
+        sc.s = fma(hh, sp.c*k2[1], sc0.s); sc.c = fma(-hh, sp.s*k2[1], sc0.c);
+        stage_given<CHART, PREC, 5>(P, Pf, use64, y0, d, sc, pt, pp, k3);
+        sp = sc;
+        for (int i = 0; i < 4; ++i) d[i] = h*k3[i];
+        sc.s = fma(h, sp.c*k3[1], sc0.s); sc.c = fma(-h, sp.s*k3[1], sc0.c);
+        stage_given<CHART, PREC, 5>(P, Pf, use64, y0, d, sc, pt, pp, k4);
+    } else {
+        for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k0[i];
+        stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k2);
+        for (int i = 0; i < 4; ++i) d[i] = 0.5*h*k2[i];
+        stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k3);
+        for (int i = 0; i < 4; ++i) d[i] = h*k3[i];
+        stage<CHART, PREC, 5>(P, Pf, use64, y0, d, sc0, pt, pp, k4);
+    }
     for (int i = 0; i < 5; ++i) inc[i] = h/6.0*(k0[i] + 2.0*k2[i] + 2.0*k3[i] + k4[i]);
 }
 

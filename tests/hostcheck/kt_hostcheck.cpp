@@ -153,7 +153,10 @@ extern "C" int ktc_trace_bundle_pixels(const kt_frame* f, const kt_opts* o, cons
         RayOut ro;
         double mag = 1.0;
         bundle_init(F, O, px[i], py[i], B);
-        while (bundle_step(F, O, B, ro, mag) != STEP_DONE) {}
+        if (O.quirks & Q_BUNDLE_ROOT_SKIP)
+            while (bundle_step<false>(F, O, B, ro, mag) != STEP_DONE) {}
+        else
+            while (bundle_step<true>(F, O, B, ro, mag) != STEP_DONE) {}
         uint32_t w[7];
         pack_geopixel_m(ro, mag, w);
         std::memcpy(&out[i], w, 28);
