@@ -51,24 +51,34 @@ F_HERMITE = 370
 PEAK_VALU_TFLOPS = 78.6   # MI355X vector FP64; FP32 without packed math issues at the same rate
 
 # Adjoint reverse pass (KS chart), per accepted step of one sweep:
-#   segment replay: 6 stages x (right-hand side + sin/cos rotation) + combinations (stage sums
-#   120, 5th-order sum 36, update 8, stage inputs 18 = 182);  the step's stage points again
-#   (same count);  6 reverse-mode vector-Jacobian products (F_VJP, counted by
-#   tests/hostcheck/flopcount.cpp) + the adjoint stage combinations (168 + 24 + 30 = 222).
+#   segment replay (the forward pass's own attempt, dp5_attempt_h, without its error sum): five
+#   h-scaled right-hand sides (F_RHS + 1) and the FSAL one, integrated stage sin/cos F_SCI, the
+#   stage and 5th-order combinations (134), the FSAL rotation F_ROT;
+#   the step adjoint (dp5_step_adjoint): the stage points again (first right-hand side and its
+#   h-scaling 82; per stage st = 1..5 the combinations 12 st and an h-scaled right-hand side),
+#   then per stage st = 5..0 a reverse-mode vector-Jacobian product (F_VJP, counted by
+#   tests/hostcheck/flopcount.cpp) with its combinations (26 + 10 (5 - st)), and the sin/cos
+#   adjoint folded into theta's (4).
 F_VJP = {"bl": 367, "ks": 348}
-F_ADJ_STEP = 2 * (6 * (F_RHS["ks"] + F_ROT) + 182) + 6 * F_VJP["ks"] + 222
+F_REPLAY_STEP = 5 * (F_RHS["ks"] + 1) + F_RHS["ks"] + F_SCI + 134 + F_ROT
+F_STEP_ADJ = (F_RHS["ks"] + 4) + sum(12 * st + F_RHS["ks"] + 1 for st in range(1, 6)) \
+    + sum(F_VJP["ks"] + 26 + 10 * (5 - st) for st in range(6)) + 4
+F_ADJ_STEP = F_REPLAY_STEP + F_STEP_ADJ
 # The reverse pass runs the checkpoint segments in parallel (kt_adjoint_segments_kernel): every
 # step of a disk pixel is swept four times, once per basis direction of its segment's end adjoint.
 F_SEG_STEP = 4 * F_ADJ_STEP
-# Jacobi bundles (kt_bundle_kernel, BL, RK4 doubling), per attempt: the full step (3 plain
-# right-hand sides F_RHS["bl"] + rotations + sums: 3 x (F_RHS + 31 + 3) + 54), the two half steps (3 stages
-# each of one dual-number right-hand side F_DUAL_BL (counted by flopcount.cpp) + 4 for its sin/cos
-# tangents + rotation + state/W updates 24 + stage sums 26, then 14 for the increments), the
-# derivative at the midpoint (F_DUAL_BL + 4 + 31 + 12) and the error norm (22); an accepted
-# attempt adds the derivative at the new point (F_DUAL_BL + 4 + 31 + 21).
+# Jacobi bundles (kt_bundle_kernel, BL, RK4 doubling, root-skip quirk off), per attempt: the full
+# step (3 plain right-hand sides F_RHS["bl"] + integrated stage sin/cos + sums: 3 x (F_RHS + 6 + 3)
+# + 54), the two half steps (3 stages each of one dual-number right-hand side F_DUAL_BL (counted by
+# flopcount.cpp) + 4 for its sin/cos tangents + integrated sin/cos 6 + state/W updates 24 + stage
+# sums 26, then 14 for the increments), the derivative at the midpoint (F_DUAL_BL + 4 + rotation 31
+# + 12) and the error norm (22); an accepted attempt adds the derivative at the new point
+# (F_DUAL_BL + 4 + 31 + 21).  Integrated stage sin/cos: s' = s0 + c h (c_prev k_theta),
+# c' = c0 - c h (s_prev k_theta), 3 flops each.
 F_DUAL_BL = 392
-F_BUNDLE_ATTEMPT = 3 * (F_RHS["bl"] + F_ROT + 3) + 54 + 2 * (3 * (F_DUAL_BL + 4 + F_ROT + 24 + 26) + 14) \
-    + (F_DUAL_BL + 4 + F_ROT + 12) + 22
+F_SCI_STAGE = 6
+F_BUNDLE_ATTEMPT = 3 * (F_RHS["bl"] + F_SCI_STAGE + 3) + 54 \
+    + 2 * (3 * (F_DUAL_BL + 4 + F_SCI_STAGE + 24 + 26) + 14) + (F_DUAL_BL + 4 + F_ROT + 12) + 22
 F_BUNDLE_ACCEPT = F_DUAL_BL + 4 + F_ROT + 21
 
 

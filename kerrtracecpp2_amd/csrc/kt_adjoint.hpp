@@ -261,9 +261,11 @@ KT_HD void rhs_jacobian(double M, const double par[3], const double y[4], double
 // intermediates (1/Sigma, 1/Delta_r and 1/(Delta_theta sin^2) taken as such); the reverse
 // sweep visits them in the opposite order.
 //   P: the frame's constants (knds_t of a, Q, Lambda); sc: sin, cos of y[1].
+//   gsc (optional): the adjoints of sin and cos taken as independent inputs (g[1] is their
+//   combination along theta, gsc[0] cos - gsc[1] sin).
 template <int CHART, class T = double>
 KT_HD void rhs_vjp_sc(const Knds<T>& P, T Q, T Lam, const T y[4], const SinCos<T>& sc, T pt, T pp, const T mu[4],
-                      T g[9]) {
+                      T g[9], T* gsc = nullptr) {
     const T a = P.a;
     const T r = y[0], pr = y[2], pth = y[3];
     const T s = sc.s, c = sc.c;
@@ -425,6 +427,7 @@ KT_HD void rhs_vjp_sc(const Knds<T>& P, T Q, T Lam, const T y[4], const SinCos<T
     b_r += 2.0*r*b_r2;
     b_s += b_sc2*c + 2.0*s*b_s2;
     b_c += b_sc2*s + 2.0*c*b_c2;
+    if (gsc) { gsc[0] = b_s; gsc[1] = b_c; }
     // (s, c) = (sin, cos) theta
     const T b_th = b_s*c - b_c*s;
     // constants: a2 = a^2, Q2 = Q^2, lam3 = L/3, lam3a2 = L a^2/3, xi = 1 + L a^2/3, xi2 = xi^2
@@ -473,53 +476,42 @@ KT_HD void init_jacobian(const FrameDev& F, double al, double be, double y0[4], 
 }
 
 // ---------------------------------------------------------------------------
-//  replay of an accepted DOPRI5 step (the binary64 arithmetic of dp5_step, phi
-//  not carried): y, sc at the step start and the FSAL derivative k1 = f(y) ->
-//  the step end, its sin/cos and its derivative
+//  replay of an accepted DOPRI5 step: the forward pass's own attempt (dp5_attempt_h,
+//  binary64, phi not carried), so the replay retraces it bit for bit: y, sc at the step
+//  start and the FSAL derivative k1 = f(y) -> the step end, its sin/cos and its derivative
 // ---------------------------------------------------------------------------
 template <int CHART>
-KT_HD void dp5_replay(const Knds<double>& P, double y[4], SinCos<double>& sc, double k1[4], double h, double pt,
+KT_HD void dp5_replay(const Knds<double>& P, double y[4], SinCos<double>& sc, double k1[5], double h, double pt,
                       double pp) {
-    typedef DP5 C;
-    double k2[4], k3[4], k4[4], d[4], a5[4];
-    SinCos<double> scx;
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
-    stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k2, scx);
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
-    stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k3, scx);
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
-    stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k4, scx);
-    {
-        double k5[4];
-        for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
-        stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k5, scx);
-        for (int i = 0; i < 4; ++i)
-            d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
-        for (int i = 0; i < 4; ++i) a5[i] = C::b1*k1[i] + C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i];
-    }
-    {
-        double k6[4];
-        stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k6, scx);
-        for (int i = 0; i < 4; ++i) a5[i] += C::b6*k6[i];
-    }
-    for (int i = 0; i < 4; ++i) d[i] = h*a5[i];
-    double k7[5];
+    double z[4], ae[4], k7[5], dth5, dph5;
     SinCos<double> sc7;
-    stage_f64<CHART, 5>(P, y, d, sc, pt, pp, k7, sc7);
-    for (int i = 0; i < 4; ++i) { y[i] += d[i]; k1[i] = k7[i]; }
+    dp5_attempt_h<CHART, PREC_FP64, 4>(P, knds_cast<float>(P), true, y, k1, sc, h, pt, pp, z, dth5, dph5, ae, k7,
+                                       sc7);
+    y[0] = z[0]; y[1] = y[1] + dth5; y[2] = z[2]; y[3] = z[3];
+    for (int i = 0; i < 5; ++i) k1[i] = k7[i];
     sc = sc7;
 }
 
-// Adjoint of one accepted step: lam (adjoint of the step end) -> adjoint of the step start;
-// accumulates the parameter adjoints gc (p_t, p_phi) and gP (a, Q, Lambda).
+// Adjoint of one accepted step of dp5_attempt_h: lam (adjoint of the step end) -> adjoint of
+// the step start; accumulates the parameter adjoints gc (p_t, p_phi) and gP (a, Q, Lambda).
+// The step, with F_i = h f(z_i, s_i, c_i) the scaled stages:
+//   z_i = y + sum_j a_ij F_j          (r, p_r, p_theta)
+//   s_i = s + sum_j a_ij u_j,  c_i = c + sum_j a_ij v_j,   u_j = c_j F_j[theta], v_j = -s_j F_j[theta]
+//   y+  = y + sum_i b_i F_i           (r, theta, p_r, p_theta)
+// with (s, c) = (sin, cos) of theta at the step start (the forward pass's FSAL rotation makes
+// them so).  Reverse, i = 6 .. 1, with bars for adjoints:
+//   ubar_i = sum_m a_mi sbar_m,  vbar_i = sum_m a_mi cbar_m
+//   Fbar_i = b_i lam + sum_m a_mi zbar_m   (r, p_r, p_theta),
+//   Fbar_i[theta] = b_i lam[theta] + ubar_i c_i - vbar_i s_i
+//   (zbar_i, sbar_i, cbar_i, p_t, p_phi, a, Q, Lambda) += h J_f^T Fbar_i   (rhs_vjp_sc)
+//   sbar_i -= vbar_i F_i[theta],  cbar_i += ubar_i F_i[theta]
+// then lam[theta] <- lam[theta] + cos theta sum_i sbar_i - sin theta sum_i cbar_i: the exact
+// adjoint of the integrated stage sin/cos, not of a rotation per stage.
 template <int CHART>
 KT_HD void dp5_step_adjoint(double M, const double par[3], const Knds<double>& P, const double y[4],
                             const SinCos<double>& sc, double h, double pt, double pp, double lam[4], double gc[2],
                             double gP[3]) {
     typedef DP5 C;
-    // stage points Y_1..Y_6 and their sin/cos theta (forward arithmetic)
-    double Y[6][4], k[6][4], d[4];
-    SinCos<double> scs[6];
     const double A[6][6] = {{0, 0, 0, 0, 0, 0},
                             {C::a21, 0, 0, 0, 0, 0},
                             {C::a31, C::a32, 0, 0, 0, 0},
@@ -527,40 +519,62 @@ KT_HD void dp5_step_adjoint(double M, const double par[3], const Knds<double>& P
                             {C::a51, C::a52, C::a53, C::a54, 0, 0},
                             {C::a61, C::a62, C::a63, C::a64, C::a65, 0}};
     const double B[6] = {C::b1, 0.0, C::b3, C::b4, C::b5, C::b6};
-    const double z[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = 0; i < 4; ++i) Y[0][i] = y[i];
-    stage_f64<CHART, 4>(P, y, z, sc, pt, pp, k[0], scs[0]);
-    for (int s = 1; s < 6; ++s) {
-        for (int i = 0; i < 4; ++i) {
-            double acc = 0.0;
-            for (int j = 0; j < s; ++j) acc += A[s][j]*k[j][i];
-            d[i] = h*acc;
-            Y[s][i] = y[i] + d[i];
+    // stage points (r, p_r, p_theta: theta enters f only through sin/cos), sin/cos and
+    // scaled stages (forward arithmetic of the attempt)
+    const int ix[3] = {0, 2, 3};
+    double Z[6][3], Fs[6][4];
+    SinCos<double> scs[6];
+    for (int i = 0; i < 3; ++i) Z[0][i] = y[ix[i]];
+    scs[0] = sc;
+    {
+        double k[5];
+        rhs<CHART>(P, y[0], sc, y[2], y[3], pt, pp, k);
+        for (int i = 0; i < 4; ++i) Fs[0][i] = h*k[i];
+    }
+    for (int st = 1; st < 6; ++st) {
+        double zs = sc.s, zc = sc.c;
+        for (int i = 0; i < 3; ++i) Z[st][i] = y[ix[i]];
+        for (int j = 0; j < st; ++j) {
+            for (int i = 0; i < 3; ++i) Z[st][i] = fma(A[st][j], Fs[j][ix[i]], Z[st][i]);
+            zs = fma(A[st][j], scs[j].c*Fs[j][1], zs);
+            zc = fma(-A[st][j], scs[j].s*Fs[j][1], zc);
         }
-        stage_f64<CHART, 4>(P, y, d, sc, pt, pp, k[s], scs[s]);
+        scs[st].s = zs;
+        scs[st].c = zc;
+        double k[5];
+        rhs_h<CHART>(P, Z[st][0], scs[st], Z[st][1], Z[st][2], pt, pp, h, k);
+        for (int i = 0; i < 4; ++i) Fs[st][i] = k[i];
     }
     // reverse over the stages
-    double nu[6][4];
+    double zb[6][3], sb[6], cb[6];
     double lam_out[4] = {lam[0], lam[1], lam[2], lam[3]};
-    for (int s = 5; s >= 0; --s) {
-        double mu[4];
-        for (int i = 0; i < 4; ++i) {
-            double acc = B[s]*lam[i];
-            for (int m = s + 1; m < 6; ++m) acc += A[m][s]*nu[m][i];
-            mu[i] = h*acc;
-        }
-        double gv[9];                       // J^T mu over (y, p_t, p_phi, a, Q, Lambda)
-        rhs_vjp_sc<CHART>(P, par[1], par[2], Y[s], scs[s], pt, pp, mu, gv);
-        for (int k2 = 0; k2 < 4; ++k2) {
-            nu[s][k2] = gv[k2];
-            lam_out[k2] += gv[k2];
-        }
+    double sb_sum = 0.0, cb_sum = 0.0;
+    for (int st = 5; st >= 0; --st) {
+        double ub = 0.0, vb = 0.0, Fb[4];
+        for (int m = st + 1; m < 6; ++m) { ub = fma(A[m][st], sb[m], ub); vb = fma(A[m][st], cb[m], vb); }
+        for (int i = 0; i < 4; ++i) Fb[i] = B[st]*lam[i];
+        for (int i = 0; i < 3; ++i)
+            for (int m = st + 1; m < 6; ++m) Fb[ix[i]] = fma(A[m][st], zb[m][i], Fb[ix[i]]);
+        Fb[1] = fma(ub, scs[st].c, fma(-vb, scs[st].s, Fb[1]));
+        double mu[4], gv[9], gsc[2];
+        for (int i = 0; i < 4; ++i) mu[i] = h*Fb[i];
+        const double zs[4] = {Z[st][0], 0.0, Z[st][1], Z[st][2]};    // theta is not an input of f
+        rhs_vjp_sc<CHART>(P, par[1], par[2], zs, scs[st], pt, pp, mu, gv, gsc);
+        for (int i = 0; i < 3; ++i) zb[st][i] = gv[ix[i]];
+        sb[st] = fma(-vb, Fs[st][1], gsc[0]);
+        cb[st] = fma(ub, Fs[st][1], gsc[1]);
+        sb_sum += sb[st];
+        cb_sum += cb[st];
+        lam_out[0] += gv[0];
+        lam_out[2] += gv[2];
+        lam_out[3] += gv[3];
         gc[0] += gv[4];
         gc[1] += gv[5];
         gP[0] += gv[6];
         gP[1] += gv[7];
         gP[2] += gv[8];
     }
+    lam_out[1] = fma(sb_sum, sc.c, fma(-cb_sum, sc.s, lam_out[1]));
     for (int i = 0; i < 4; ++i) lam[i] = lam_out[i];
 }
 

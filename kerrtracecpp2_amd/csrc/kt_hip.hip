@@ -451,7 +451,7 @@ struct AdjointForwardPolicy {
         const double h = S.g.h;
         const int n0 = S.g.n_acc;
         magnif = 1.0;
-        const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false, false>(F, P, Pf, A.O, true, S.g, o);
+        const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false>(F, P, Pf, A.O, true, S.g, o);
         const AdjRecord& R = A.rec;
         if (R.cap && S.g.n_acc > n0 && n0 < R.cap) {
             R.hs[S.pix * R.cap + n0] = h;
@@ -733,10 +733,12 @@ struct LdsStates {
     }
 };
 
-// Two waves per SIMD (256 VGPRs; the replay's stage arrays spill a little): 1.88 ms against
-// 2.26 ms at the compiler's own budget of 367 VGPRs, one wave per SIMD (C5 reverse pass).
+// One wave per SIMD, at the compiler's own register budget (no spills): 1.75 ms for the C5
+// reverse pass against 2.25 ms at two waves per SIMD, where the step adjoint's stage arrays spill
+// about 250 registers.  (The earlier adjoint of a per-stage rotation had fewer live values and
+// went the other way: 1.88 ms at two waves per SIMD, 2.26 ms at one.)
 template <int CHART>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8)))
 void kt_adjoint_segments_kernel(AdjBackArgs B) {
     const FrameDev& F = *B.frame;
     __shared__ double states[64][6 * (kCkptEvery + 1)];

@@ -605,131 +605,6 @@ KT_HD int after_reject(const RunOpts& O, RayState& S) {
     return STEP_REJECTED;
 }
 
-// ---------------------------------------------------------------------------
-//  DOPRI5 attempt (geodesic.hpp:312-414), ordered for a small live set (register
-//  economy); NC = 5 also advances phi with the same weights.
-// ---------------------------------------------------------------------------
-//
-// SCI (integrated stage sin/cos): the metric depends on theta only through sin and cos,
-// so stages 2-6 take (sin, cos) as two more integrated components,
-//   d(sin theta)/dl = cos theta * theta',   d(cos theta)/dl = -sin theta * theta',
-// combined with the same tableau as the other components (a consistent 5th-order method
-// for the augmented system), instead of a Taylor rotation per stage.  The FSAL stage at the
-// accepted point keeps the exact rotation, so every step starts from sin/cos of theta.
-// SCI = false (the differentiable path) rotates at every stage, as its replay does.
-template <int CHART, int PREC, int NC, bool SCI>
-KT_HD int dp5_step(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
-                   RayState& S, RayOut& o) {
-    typedef DP5 C;
-    const double h = S.h;
-    const double* y = S.y;
-    const double* k1 = S.k1;
-    // Live set kept small: k2..k4 stay raw until k5 exists (b2 = e2 = 0), the 5th-order
-    // and error combinations are formed after k5, so at most three stage vectors besides
-    // k1 are live while a stage is evaluated.
-    double k2[NC], k3[NC], k4[NC];
-    double d[4], a5[NC], ae[4];
-    // SCI: stage sin/cos, and their derivatives u = cos*theta', v = -sin*theta' per stage
-    const double s1 = S.sc.s, c1 = S.sc.c;
-    const double u1 = c1*k1[1], v1 = -s1*k1[1];
-    double u2 = 0.0, v2 = 0.0, u3 = 0.0, v3 = 0.0, u4 = 0.0, v4 = 0.0;
-    SinCos<double> sc;
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a21*k1[i]);
-    if (SCI) {
-        sc.s = s1 + h*(C::a21*u1); sc.c = c1 + h*(C::a21*v1);
-        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k2);
-        u2 = sc.c*k2[1]; v2 = -sc.s*k2[1];
-    } else {
-        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k2);
-    }
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a31*k1[i] + C::a32*k2[i]);
-    if (SCI) {
-        sc.s = s1 + h*(C::a31*u1 + C::a32*u2); sc.c = c1 + h*(C::a31*v1 + C::a32*v2);
-        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k3);
-        u3 = sc.c*k3[1]; v3 = -sc.s*k3[1];
-    } else {
-        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k3);
-    }
-    for (int i = 0; i < 4; ++i) d[i] = h*(C::a41*k1[i] + C::a42*k2[i] + C::a43*k3[i]);
-    if (SCI) {
-        sc.s = s1 + h*(C::a41*u1 + C::a42*u2 + C::a43*u3); sc.c = c1 + h*(C::a41*v1 + C::a42*v2 + C::a43*v3);
-        stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k4);
-        u4 = sc.c*k4[1]; v4 = -sc.s*k4[1];
-    } else {
-        stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k4);
-    }
-    {
-        double k5[NC];
-        for (int i = 0; i < 4; ++i) d[i] = h*(C::a51*k1[i] + C::a52*k2[i] + C::a53*k3[i] + C::a54*k4[i]);
-        double u5 = 0.0, v5 = 0.0;
-        if (SCI) {
-            sc.s = s1 + h*(C::a51*u1 + C::a52*u2 + C::a53*u3 + C::a54*u4);
-            sc.c = c1 + h*(C::a51*v1 + C::a52*v2 + C::a53*v3 + C::a54*v4);
-            stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k5);
-            u5 = sc.c*k5[1]; v5 = -sc.s*k5[1];
-            sc.s = s1 + h*(C::a61*u1 + C::a62*u2 + C::a63*u3 + C::a64*u4 + C::a65*u5);
-            sc.c = c1 + h*(C::a61*v1 + C::a62*v2 + C::a63*v3 + C::a64*v4 + C::a65*v5);
-        } else {
-            stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k5);
-        }
-        for (int i = 0; i < 4; ++i)
-            d[i] = h*(C::a61*k1[i] + C::a62*k2[i] + C::a63*k3[i] + C::a64*k4[i] + C::a65*k5[i]);
-        for (int i = 0; i < NC; ++i) a5[i] = C::b1*k1[i] + C::b3*k3[i] + C::b4*k4[i] + C::b5*k5[i];
-        for (int i = 0; i < 4; ++i) ae[i] = C::e1*k1[i] + C::e3*k3[i] + C::e4*k4[i] + C::e5*k5[i];
-    }
-    {
-        double k6[NC];
-        if (SCI) stage_given<CHART, PREC, NC>(P, Pf, use64, y, d, sc, S.pt, S.pp, k6);
-        else stage<CHART, PREC, NC>(P, Pf, use64, y, d, S.sc, S.pt, S.pp, k6);
-        for (int i = 0; i < NC; ++i) a5[i] += C::b6*k6[i];
-        for (int i = 0; i < 4; ++i) ae[i] += C::e6*k6[i];
-    }
-    for (int i = 0; i < 4; ++i) d[i] = h*a5[i];
-    // FSAL stage at y5 in the stage precision; its sin/cos becomes the next base
-    double k7[5];
-    SinCos<double> sc7;
-    if (PREC == PREC_FP64 || use64) {
-        stage_f64<CHART, 5>(P, y, d, S.sc, S.pt, S.pp, k7, sc7);
-    } else {
-        stage_f32<CHART, 5>(Pf, y, d, S.sc, S.pt, S.pp, k7);
-        sc7 = sc_at(S.sc, y[1], d[1]);
-    }
-    count_rhs<PREC>(S, use64, 6);      // stages 2-6 and the FSAL stage
-    // Scaled RMS of the embedded error, err^2 = mean_i (dy_i / (atol + rtol |y_i|))^2, in
-    // binary64 in both precisions: each quotient through the refined reciprocal, and the
-    // accept test err <= 1 taken on err^2 (no square root; sqrt is monotone, so the test is
-    // the same).
-    double e2 = 0.0;
-    for (int i = 0; i < 4; ++i) {
-        const double q = (h*(ae[i] + C::e7*k7[i])) * kt_rcp(O.atol + O.rtol*fabs(y[i]));
-        e2 = fma(q, q, e2);
-    }
-    const double err2 = e2 * 0.25;
-    if (!isfinite(err2)) {
-        const double hn = (isfinite(h) && h > 1e-10) ? h*0.5 : 1e-6;
-        S.h = hn < 1e-10 ? 1e-10 : hn;
-        mixed_retry<PREC>(O, use64, h, S);
-        return after_reject(O, S);
-    }
-    if (err2 <= 1.0 || h < 1e-10) {
-        const double r0 = y[0], th0 = y[1], ph0 = S.ph, dr0 = k1[0], dth0 = k1[1];
-        if (NC == 5) S.ph += h*a5[NC - 1];
-        for (int i = 0; i < 4; ++i) S.y[i] += d[i];
-        for (int i = 0; i < 5; ++i) S.k1[i] = k7[i];
-        S.sc = sc7;
-        // err > 1e-12 <=> err^2 > 1e-24; err^(-1/5) = (err^2)^(-1/10)
-        double hn = h * ((err2 > 1e-24) ? 0.9*pow_neg(err2, 0.1f) : 4.0);
-        if (!isfinite(hn)) hn = h;
-        S.h = hn > 100.0 ? 100.0 : (hn < 1e-10 ? 1e-10 : hn);
-        return after_accept<CHART, PREC>(F, P, O, S, r0, th0, ph0, dr0, dth0, h, o, use64);
-    }
-    double hn = h * 0.9 * pow_neg(err2, 0.125f);
-    if (!isfinite(hn)) hn = h*0.5;
-    S.h = hn < 1e-10 ? 1e-10 : hn;
-    mixed_retry<PREC>(O, use64, h, S);
-    return after_reject(O, S);
-}
-
 // stage derivative times h at the stage point z (r, -, p_r, p_theta; z[1] is not read), the
 // sin/cos of theta given (integrated stage values)
 template <int CHART, int PREC, int NC>
@@ -770,26 +645,37 @@ KT_HD void fsal_stage(const Knds<double>& P, const Knds<float>& Pf, bool use64, 
     }
 }
 
-// DOPRI5 attempt of the trace kernels: the dp5_step arithmetic with integrated stage sin/cos,
-// stages 2-6 carried as h k_i (h folded into each right-hand side's final scaling) and the
-// k1 terms taking their tableau coefficient times h.  Each stage point, and the 5th-order
-// solution, is accumulated onto y by fused multiply-adds (one per tableau term, no separate
-// sum and add); theta enters the stages only through its sin/cos, so its stage values are not
-// formed, and its 5th-order increment feeds the FSAL rotation.  k1 and the FSAL stage stay
-// unscaled: they are the event derivatives and the next attempt's (or a retry's) k1.
+// ---------------------------------------------------------------------------
+//  DOPRI5 attempt (geodesic.hpp:312-414) with integrated stage sin/cos.
+//
+//  The metric depends on theta only through sin and cos, so stages 2-6 take (sin, cos) as
+//  two more integrated components,
+//    d(sin theta)/dl = cos theta * theta',   d(cos theta)/dl = -sin theta * theta',
+//  combined with the same tableau as the other components (a consistent 5th-order method
+//  for the augmented system), instead of a Taylor rotation per stage.  The FSAL stage at
+//  the accepted point keeps the exact rotation, so every step starts from sin/cos of theta.
+//  Stages 2-6 are carried as h k_i (h folded into each right-hand side's final scaling) and
+//  the k1 terms take their tableau coefficient times h.  Each stage point, and the 5th-order
+//  solution, is accumulated onto y by fused multiply-adds (one per tableau term, no separate
+//  sum and add); theta enters the stages only through its sin/cos, so its stage values are
+//  not formed, and its 5th-order increment feeds the FSAL rotation.  k1 and the FSAL stage
+//  stay unscaled: they are the event derivatives and the next attempt's (or a retry's) k1.
+//  The adjoint's replay runs this same function, so it retraces the forward pass bit for bit.
+// ---------------------------------------------------------------------------
+// One attempt from (y, k1 = f(y), sc = sin/cos of y[1]) over h: the 5th-order solution
+// (z[0], z[2], z[3]; z[1] is not formed), its theta and phi increments, the error sum ae
+// (without the FSAL term), the FSAL derivative k7 = f(solution) and its sin/cos.
 template <int CHART, int PREC, int NC>
-KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
-                     RayState& S, RayOut& o) {
+KT_HD void dp5_attempt_h(const Knds<double>& P, const Knds<float>& Pf, bool use64, const double y[4],
+                         const double k1[5], const SinCos<double>& sc0, double h, double pt, double pp,
+                         double z[4], double& dth5, double& dph5, double ae[4], double k7[5],
+                         SinCos<double>& sc7) {
     typedef DP5 C;
-    const double h = S.h;
-    const double* y = S.y;
-    const double* k1 = S.k1;                    // f(y), kept for a rejected attempt
     double k2[NC], k3[NC], k4[NC];              // h k_i
-    double z[4], ae[4];                         // stage point / 5th-order solution; error sum
-    double dth5, dph5 = 0.0;                    // theta (and phi) increments of the 5th-order solution
+    dph5 = 0.0;
     z[1] = 0.0;
     // stage sin/cos and their h-scaled derivatives u = cos h theta', v = -sin h theta'
-    const double s1 = S.sc.s, c1 = S.sc.c;
+    const double s1 = sc0.s, c1 = sc0.c;
     const double hth1 = h*k1[1];
     const double u1 = c1*hth1, v1 = -s1*hth1;
     double u2, v2, u3, v3, u4, v4;
@@ -797,18 +683,18 @@ KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>
     double ch = C::a21*h;
     for (int i = 0; i < 4; ++i) if (i != 1) z[i] = fma(ch, k1[i], y[i]);
     sc.s = fma(C::a21, u1, s1); sc.c = fma(C::a21, v1, c1);
-    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k2);
+    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, pt, pp, h, k2);
     u2 = sc.c*k2[1]; v2 = -sc.s*k2[1];
     ch = C::a31*h;
     for (int i = 0; i < 4; ++i) if (i != 1) z[i] = fma(C::a32, k2[i], fma(ch, k1[i], y[i]));
     sc.s = fma(C::a32, u2, fma(C::a31, u1, s1)); sc.c = fma(C::a32, v2, fma(C::a31, v1, c1));
-    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k3);
+    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, pt, pp, h, k3);
     u3 = sc.c*k3[1]; v3 = -sc.s*k3[1];
     ch = C::a41*h;
     for (int i = 0; i < 4; ++i) if (i != 1) z[i] = fma(C::a43, k3[i], fma(C::a42, k2[i], fma(ch, k1[i], y[i])));
     sc.s = fma(C::a43, u3, fma(C::a42, u2, fma(C::a41, u1, s1)));
     sc.c = fma(C::a43, v3, fma(C::a42, v2, fma(C::a41, v1, c1)));
-    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k4);
+    stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, pt, pp, h, k4);
     u4 = sc.c*k4[1]; v4 = -sc.s*k4[1];
     {
         double k5[NC];
@@ -817,7 +703,7 @@ KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>
             if (i != 1) z[i] = fma(C::a54, k4[i], fma(C::a53, k3[i], fma(C::a52, k2[i], fma(ch, k1[i], y[i]))));
         sc.s = fma(C::a54, u4, fma(C::a53, u3, fma(C::a52, u2, fma(C::a51, u1, s1))));
         sc.c = fma(C::a54, v4, fma(C::a53, v3, fma(C::a52, v2, fma(C::a51, v1, c1))));
-        stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k5);
+        stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, pt, pp, h, k5);
         const double u5 = sc.c*k5[1], v5 = -sc.s*k5[1];
         sc.s = fma(C::a65, u5, fma(C::a64, u4, fma(C::a63, u3, fma(C::a62, u2, fma(C::a61, u1, s1)))));
         sc.c = fma(C::a65, v5, fma(C::a64, v4, fma(C::a63, v3, fma(C::a62, v2, fma(C::a61, v1, c1)))));
@@ -835,18 +721,30 @@ KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>
         ch = C::e1*h;
         for (int i = 0; i < 4; ++i) ae[i] = fma(ch, k1[i], C::e3*k3[i] + C::e4*k4[i] + C::e5*k5[i]);
         double k6[NC];
-        stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, S.pt, S.pp, h, k6);
+        stage_at_h<CHART, PREC, NC>(P, Pf, use64, z, sc, pt, pp, h, k6);
         for (int i = 0; i < 4; ++i) if (i != 1) z[i] = fma(C::b6, k6[i], a5[i]);
         dth5 = fma(C::b6, k6[1], dth5);
         if (NC == 5) dph5 = fma(C::b6, k6[NC - 1], dph5);
         for (int i = 0; i < 4; ++i) ae[i] = fma(C::e6, k6[i], ae[i]);
     }
     // FSAL stage at the 5th-order point, unscaled; its sin/cos becomes the next base
-    double k7[5];
+    fsal_stage<CHART, PREC>(P, Pf, use64, z, y[1], dth5, sc0, pt, pp, k7, sc7);
+}
+
+template <int CHART, int PREC, int NC>
+KT_HD int dp5_step_h(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O, bool use64,
+                     RayState& S, RayOut& o) {
+    typedef DP5 C;
+    const double h = S.h;
+    const double* y = S.y;
+    double z[4], ae[4], k7[5], dth5, dph5;
     SinCos<double> sc7;
-    fsal_stage<CHART, PREC>(P, Pf, use64, z, y[1], dth5, S.sc, S.pt, S.pp, k7, sc7);
+    dp5_attempt_h<CHART, PREC, NC>(P, Pf, use64, S.y, S.k1, S.sc, h, S.pt, S.pp, z, dth5, dph5, ae, k7, sc7);
     count_rhs<PREC>(S, use64, 6);      // stages 2-6 and the FSAL stage
-    // err^2 = mean_i (dy_i / (atol + rtol |y_i|))^2 in binary64 (as dp5_step)
+    // Scaled RMS of the embedded error, err^2 = mean_i (dy_i / (atol + rtol |y_i|))^2, in
+    // binary64 in both precisions: each quotient through the refined reciprocal, and the
+    // accept test err <= 1 taken on err^2 (no square root; sqrt is monotone, so the test is
+    // the same).
     const double e7h = C::e7*h;
     double e2 = 0.0;
     for (int i = 0; i < 4; ++i) {
@@ -1051,15 +949,11 @@ KT_HD void pack_geopixel_m(const RayOut& o, double magnif, uint32_t w[7]) {
 
 // One attempt + bookkeeping for an active ray; returns true when the ray is done.
 // PHI: DOPRI5 advances phi (false reproduces the reference's frozen phi).
-// SCI: DOPRI5 integrates the stage sin/cos (dp5_step); false rotates at every stage.
-template <int CHART, int METHOD, int PREC, bool PHI, bool SCI = true>
+template <int CHART, int METHOD, int PREC, bool PHI>
 KT_HD bool ray_advance(const FrameDev& F, const Knds<double>& P, const Knds<float>& Pf, const RunOpts& O,
                        bool use64, RayState& S, RayOut& o) {
     int rc;
-    if (METHOD == METHOD_DP5) {
-        if constexpr (SCI) rc = dp5_step_h<CHART, PREC, PHI ? 5 : 4>(F, P, Pf, O, use64, S, o);
-        else rc = dp5_step<CHART, PREC, PHI ? 5 : 4, false>(F, P, Pf, O, use64, S, o);
-    }
+    if (METHOD == METHOD_DP5) rc = dp5_step_h<CHART, PREC, PHI ? 5 : 4>(F, P, Pf, O, use64, S, o);
     else rc = rk4_step<CHART, PREC>(F, P, Pf, O, use64, S, o);
     if (rc != STEP_DONE) return false;
     if (S.n_rej > 64 || !isfinite(S.h)) stopped_result<CHART>(S, O.quirks, o);   // reject cap

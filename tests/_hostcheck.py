@@ -53,6 +53,10 @@ def lib():
         L.ktc_rhs_jacobian.argtypes = [ctypes.c_int, D, P, P, D, D, P]
         L.ktc_rhs.argtypes = [ctypes.c_int, D, P, P, D, D, P]
         L.ktc_rhs_vjp.argtypes = [ctypes.c_int, D, P, P, D, D, P, P]
+        L.ktc_rhs_sc.argtypes = [ctypes.c_int, D, P, P, D, D, D, D, P]
+        L.ktc_rhs_vjp_sc.argtypes = [ctypes.c_int, D, P, P, D, D, D, D, P, P]
+        L.ktc_dp5_step.argtypes = [ctypes.c_int, D, P, P, D, D, D, P]
+        L.ktc_dp5_step_adjoint.argtypes = [ctypes.c_int, D, P, P, D, D, D, P, P]
         L.ktc_replay_intensity.argtypes = [P, P, P, P, D, D, P, ctypes.c_int, P, P]
         L.ktc_ray_steps.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P, P]
         L.ktc_event_filters.argtypes = [P, ctypes.c_int64, P]
@@ -154,6 +158,36 @@ def rhs(chart, M, par, y, pt, pp):
     f = np.zeros(4)
     lib().ktc_rhs(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), pt, pp, _p(f))
     return f
+
+
+def rhs_sc(chart, M, par, y, s, c, pt, pp):
+    """The right-hand side with sin and cos of theta given as independent inputs (y[1] unused)."""
+    f = np.zeros(4)
+    lib().ktc_rhs_sc(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), s, c, pt, pp, _p(f))
+    return f
+
+
+def rhs_vjp_sc(chart, M, par, y, s, c, pt, pp, mu):
+    """rhs_vjp_sc's adjoints of (sin, cos) taken as independent inputs."""
+    g = np.zeros(2)
+    lib().ktc_rhs_vjp_sc(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), s, c, pt, pp,
+                         _p(np.asarray(mu, np.float64)), _p(g))
+    return g
+
+
+def dp5_step(chart, M, par, y, pt, pp, h):
+    """One step of the forward pass's DOPRI5 arithmetic from y (sin/cos of y[1])."""
+    out = np.zeros(4)
+    lib().ktc_dp5_step(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), pt, pp, h, _p(out))
+    return out
+
+
+def dp5_step_adjoint(chart, M, par, y, pt, pp, h, lam):
+    """Adjoint of dp5_step: lam at the step end -> adjoints of (y, p_t, p_phi, a, Q, Lambda)."""
+    out = np.zeros(9)
+    lib().ktc_dp5_step_adjoint(chart, M, _p(np.asarray(par, np.float64)), _p(np.asarray(y, np.float64)), pt, pp,
+                               h, _p(np.asarray(lam, np.float64)), _p(out))
+    return out
 
 
 def ray_steps(frame, opts, px, py, cap=200000):

@@ -208,7 +208,7 @@ static void adjoint_run(const FrameDev& F, const RunOpts& O, const int32_t* px, 
         for (;;) {
             const double h = S.h;
             const int n0 = S.n_acc;
-            const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false, false>(F, F.P, Pf, O, true, S, o);
+            const bool done = ray_advance<CHART, METHOD_DP5, PREC_FP64, false>(F, F.P, Pf, O, true, S, o);
             if (S.n_acc > n0) {
                 hs.push_back(h);
                 if (!done && S.n_acc % kCkptEvery == 0) push_ck();
@@ -315,6 +315,63 @@ extern "C" int ktc_rhs(int chart, double M, const double* par3, const double* y4
     return 0;
 }
 
+// the right-hand side with (sin, cos) given as independent inputs, and the reverse-mode
+// adjoints of (sin, cos) from rhs_vjp_sc (finite differences in s and c separately pin them)
+extern "C" int ktc_rhs_sc(int chart, double M, const double* par3, const double* y4, double s, double c, double pt,
+                          double pp, double* f4) {
+    const Knds<double> P = knds_t<double>(M, par3[0], par3[1], par3[2]);
+    const SinCos<double> sc = {s, c};
+    double o5[5];
+    if (chart == CHART_KS) rhs<CHART_KS>(P, y4[0], sc, y4[2], y4[3], pt, pp, o5);
+    else rhs<CHART_BL>(P, y4[0], sc, y4[2], y4[3], pt, pp, o5);
+    for (int i = 0; i < 4; ++i) f4[i] = o5[i];
+    return 0;
+}
+
+extern "C" int ktc_rhs_vjp_sc(int chart, double M, const double* par3, const double* y4, double s, double c,
+                              double pt, double pp, const double* mu4, double* g2) {
+    const Knds<double> P = knds_t<double>(M, par3[0], par3[1], par3[2]);
+    const SinCos<double> sc = {s, c};
+    double g9[9];
+    if (chart == CHART_KS) rhs_vjp_sc<CHART_KS>(P, par3[1], par3[2], y4, sc, pt, pp, mu4, g9, g2);
+    else rhs_vjp_sc<CHART_BL>(P, par3[1], par3[2], y4, sc, pt, pp, mu4, g9, g2);
+    return 0;
+}
+
+// One DOPRI5 step of the forward pass's arithmetic (dp5_replay) from y with sin/cos of y[1],
+// and its adjoint (dp5_step_adjoint): lam4 at the step end -> out9 = adjoints of y (4),
+// p_t, p_phi, a, Q, Lambda.
+extern "C" int ktc_dp5_step(int chart, double M, const double* par3, const double* y4, double pt, double pp,
+                            double h, double* y_out) {
+    const Knds<double> P = knds_t<double>(M, par3[0], par3[1], par3[2]);
+    double y[4] = {y4[0], y4[1], y4[2], y4[3]}, k1[5];
+    SinCos<double> sc;
+    kt_sincos(y[1], &sc.s, &sc.c);
+    if (chart == CHART_KS) {
+        rhs<CHART_KS>(P, y[0], sc, y[2], y[3], pt, pp, k1);
+        dp5_replay<CHART_KS>(P, y, sc, k1, h, pt, pp);
+    } else {
+        rhs<CHART_BL>(P, y[0], sc, y[2], y[3], pt, pp, k1);
+        dp5_replay<CHART_BL>(P, y, sc, k1, h, pt, pp);
+    }
+    for (int i = 0; i < 4; ++i) y_out[i] = y[i];
+    return 0;
+}
+
+extern "C" int ktc_dp5_step_adjoint(int chart, double M, const double* par3, const double* y4, double pt, double pp,
+                                    double h, const double* lam4, double* out9) {
+    const Knds<double> P = knds_t<double>(M, par3[0], par3[1], par3[2]);
+    SinCos<double> sc;
+    kt_sincos(y4[1], &sc.s, &sc.c);
+    double lam[4] = {lam4[0], lam4[1], lam4[2], lam4[3]}, gc[2] = {0.0, 0.0}, gP[3] = {0.0, 0.0, 0.0};
+    if (chart == CHART_KS) dp5_step_adjoint<CHART_KS>(M, par3, P, y4, sc, h, pt, pp, lam, gc, gP);
+    else dp5_step_adjoint<CHART_BL>(M, par3, P, y4, sc, h, pt, pp, lam, gc, gP);
+    for (int i = 0; i < 4; ++i) out9[i] = lam[i];
+    out9[4] = gc[0]; out9[5] = gc[1];
+    out9[6] = gP[0]; out9[7] = gP[1]; out9[8] = gP[2];
+    return 0;
+}
+
 // Fixed-step pieces of the adjoint (no step-size control): replay N steps of sizes hs from
 // (y0, p_t, p_phi) with parameters par = (a, Q, Lambda) and return the intensity of the disk
 // event in the last step; with lam_out != NULL also the reverse pass: adjoints of y0 (4),
@@ -380,7 +437,7 @@ extern "C" int ktc_ray_steps(const kt_frame* f, const kt_opts* o, int px, int py
         for (;;) {
             const double h = S.h;
             const int n0 = S.n_acc;
-            const bool done = ray_advance<CH, METHOD_DP5, PREC_FP64, false, false>(F, F.P, Pf, O, true, S, out);
+            const bool done = ray_advance<CH, METHOD_DP5, PREC_FP64, false>(F, F.P, Pf, O, true, S, out);
             if (S.n_acc > n0 && n < cap) hs[n++] = h;
             if (done) break;
         }

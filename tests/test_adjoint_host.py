@@ -165,3 +165,62 @@ def test_segmented_reverse_pass_equals_the_sequential_sweep(chart):
     # the split's edge cases are among them: the event in the first segment (no segment before it),
     # a last segment of 16 steps with a one-step event entry, and a 16-step event replay
     assert (n <= 16).any() and (n > 16).any() and (r == 0).any() and (r == 15).any()
+
+
+def _richardson(f, x0, k, e):
+    """Central difference in input k with one Richardson step (error O(e^4))."""
+    xs = [x0.copy() for _ in range(4)]
+    xs[0][k] += e
+    xs[1][k] -= e
+    xs[2][k] += 2 * e
+    xs[3][k] -= 2 * e
+    d1 = (f(xs[0]) - f(xs[1])) / (2 * e)
+    d2 = (f(xs[2]) - f(xs[3])) / (4 * e)
+    return (4 * d1 - d2) / 3
+
+
+def _random_point(rng):
+    a = rng.uniform(-0.95, 0.95)
+    par = np.array([a, rng.uniform(0, 0.4) * np.sqrt(1 - a * a), rng.choice([0.0, rng.uniform(0, 1e-5)])])
+    y = np.array([rng.uniform(4, 30), rng.uniform(0.3, np.pi - 0.3), 0.5 * rng.normal(), 2 * rng.normal()])
+    return par, y, -1.0, 3 * rng.normal()
+
+
+@pytest.mark.parametrize("chart", [0, 1])
+def test_rhs_vjp_sin_cos_inputs(chart):
+    """rhs_vjp_sc's adjoints of sin and cos taken as independent inputs (the integrated stage
+    sin/cos are not on the unit circle) against central differences of the right-hand side in s
+    and in c separately, at points off the circle."""
+    rng = np.random.default_rng(31 + chart)
+    for _ in range(200):
+        par, y, pt, pp = _random_point(rng)
+        s, c, mu = rng.uniform(0.2, 1.0), rng.uniform(-1.0, 1.0), rng.normal(size=4)
+        g = hc.rhs_vjp_sc(chart, 1.0, par, y, s, c, pt, pp, mu)
+        f = lambda x: mu @ hc.rhs_sc(chart, 1.0, par, y, x[0], x[1], pt, pp)
+        x0 = np.array([s, c])
+        fd = np.array([_richardson(f, x0, k, 1e-4) for k in range(2)])
+        nat = np.abs(fd).max() + np.abs(mu) @ np.abs(hc.rhs_sc(chart, 1.0, par, y, s, c, pt, pp))
+        assert np.all(np.abs(g - fd) <= 1e-9 * nat)
+
+
+@pytest.mark.parametrize("chart", [0, 1])
+def test_dp5_step_adjoint_vs_central_differences(chart):
+    """The adjoint of one DOPRI5 step (dp5_step_adjoint: the integrated stage sin/cos carried as
+    adjoint components) against central differences of the forward pass's own step arithmetic
+    (dp5_replay) in all nine inputs: state, p_t, p_phi, a, Q, Lambda.  Steps of 0.3 % to 30 % of
+    r; the errors are measured on each input's natural scale (r, 1, 1, 1, 1, 1, 0.1, 0.1, 1e-5)
+    against the largest such sensitivity.  An adjoint of a per-stage rotation instead of the
+    integrated sin/cos differs from the step's derivative at the level of its truncation error
+    and fails the 1e-9 bar."""
+    rng = np.random.default_rng(41 + chart)
+    for n in range(150):
+        par, y, pt, pp = _random_point(rng)
+        h = (0.003, 0.03, 0.3)[n % 3] * y[0] * rng.uniform(0.5, 1.0)
+        lam = rng.normal(size=4)
+        g = hc.dp5_step_adjoint(chart, 1.0, par, y, pt, pp, h, lam)
+        x0 = np.concatenate([y, [pt, pp], par])
+        scale = np.array([y[0], 1, 1, 1, 1, 1, 0.1, 0.1, 1e-5])
+        f = lambda x: lam @ hc.dp5_step(chart, 1.0, x[6:], x[:4], x[4], x[5], h)
+        fd = np.array([_richardson(f, x0, k, 1e-4 * scale[k]) for k in range(9)])
+        nat = np.max(np.abs(fd) * scale)
+        assert np.all(np.abs(g - fd) * scale <= 1e-9 * nat), (n, h, (g - fd) * scale / nat)

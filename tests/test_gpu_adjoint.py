@@ -6,7 +6,7 @@ theta_obs=60 deg, DOPRI5 (rtol 1e-9, atol 1e-12).
   per-pixel dI/dp vs central differences (h = 1e-6) of the GPU binary64 forward
   loss gradient: L = sum over interior disk pixels of (I(p) - I_target)^2, target rendered from
       seeded perturbed parameters (da, dQ ~ U(+-0.02), dtheta ~ U(+-2 deg)); adjoint vs central
-      differences, 1e-3 relative
+      differences, 1e-3 relative, over the pixels whose intensity is smooth across the stencil
   torch: KerrIntensity.apply(p).backward() == the adjoint's gradient
 
 "Interior disk pixels": I > 0 at the pixel and its 8 neighbours in the base image, so the fixed
@@ -150,24 +150,42 @@ def _loss_setup(gpu_ctx):
 
 
 def test_c5_loss_gradient_vs_central_differences(gpu_ctx):
+    """The loss gradient against central differences of the binary64 forward, h = 1e-6.
+
+    The forward's adaptive step control makes each I(p) piecewise smooth: where a step is accepted
+    on one side of p and rejected on the other, the step sequence changes, and with it the step in
+    which the ray crosses the disk and the Hermite interpolation of the hit radius inside it.  I
+    then jumps, by up to ~2e-5 I at this configuration, which over 2h = 2e-6 is a slope of order
+    10 I: one such pixel moved the loss's difference quotient by 3.6e-3 in r_obs.  A pixel whose
+    difference quotients at h and 2h disagree has a jump inside the stencil; those pixels (a few
+    in 30 000) leave both sides of the comparison: the difference quotient sums the others, and
+    the adjoint is weighted over the same pixels."""
     target, mask = _loss_setup(gpu_ctx)
-
-    def loss(**kw):
-        I, _ = gpu_ctx.render_intensity(_frame(**kw), _opts())
-        return float((((I - target) ** 2) * mask).sum()), I
-
-    L0, I0 = loss()
+    I0, _ = gpu_ctx.render_intensity(_frame(), _opts())
     w = 2.0 * (I0 - target) * mask
-    g = gpu_ctx.render_adjoint(_frame(), _opts(), weights=w)["grad"]
-    rel = {}
+    out = gpu_ctx.render_adjoint(_frame(), _opts(), weights=w, want_pixel_grads=True)
+    G = out["dI_dp"]
+    rel, dropped = {}, {}
     for q, k in enumerate(NAMES):
         if k == "Lambda":
             continue
         h = 1e-6
-        fd = (loss(**{k: C5[k] + h})[0] - loss(**{k: C5[k] - h})[0]) / (2 * h)
-        rel[k] = abs(g[q] - fd) / abs(fd)
-    print("C5 loss", L0, "grad", g, "rel err vs FD", rel, flush=True)
+        I = {m: gpu_ctx.render_intensity(_frame(**{k: C5[k] + m * h}), _opts())[0] for m in (-2, -1, 1, 2)}
+        d1 = (I[1] - I[-1]) / (2 * h)
+        d2 = (I[2] - I[-2]) / (4 * h)
+        jump = mask & (np.abs(d1 - d2) > 1e-3 * np.abs(d2) + 1e-6 * np.abs(d2[mask]).max())
+        keep = mask & ~jump
+        fd = float(((I[1] - target) ** 2 - (I[-1] - target) ** 2)[keep].sum()) / (2 * h)
+        g = float((w * G[..., q])[keep].sum())
+        rel[k] = abs(g - fd) / abs(fd)
+        dropped[k] = int(jump.sum())
+    print("C5 loss gradient: rel err vs FD", rel, "pixels with a jump in the stencil", dropped,
+          "of", int(mask.sum()), flush=True)
     assert max(rel.values()) <= 1e-3
+    assert max(dropped.values()) <= 1e-3 * mask.sum()
+    # the full adjoint gradient is the weighted sum over every pixel
+    np.testing.assert_allclose(out["grad"], (w[..., None] * G).sum(axis=(0, 1)), rtol=1e-10,
+                               atol=1e-12 * np.abs(w[..., None] * G).sum(axis=(0, 1)).max())
 
 
 def test_torch_autograd_function(gpu_ctx):
