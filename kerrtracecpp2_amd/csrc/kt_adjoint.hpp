@@ -716,6 +716,19 @@ constexpr int kSegOut = 9;     // per sweep: adjoint at the segment start (4), p
 // checkpoint segments of a ray of N >= 1 accepted steps: steps [16k, min(16k + 16, N - 1))
 KT_HD int adjoint_segments(int N) { return (N - 1 + kCkptEvery - 1) / kCkptEvery; }
 
+// Cost classes of the reverse pass's entries, so that the entries of one class run side by side:
+// class 0 the full segments (16 steps swept), classes 1..16 a ray's last segment of 16 .. 1 steps,
+// classes 17..32 the event entry, whose replay from the last checkpoint takes 16 .. 1 steps.
+constexpr int kAdjClasses = 2 * kCkptEvery + 1;
+KT_HD int adjoint_last_class(int N) {                // the last segment's class (N >= 2)
+    const int m = (N - 1) - kCkptEvery * (adjoint_segments(N) - 1);
+    return 1 + kCkptEvery - m;
+}
+KT_HD int adjoint_event_class(int N) {
+    const int r = N - kCkptEvery * ((N - 1) / kCkptEvery);
+    return 1 + 2 * kCkptEvery - r;
+}
+
 // conserved momenta of the ray at pixel (px, py), by the forward pass's own camera path
 template <int CHART>
 KT_HD void ray_momenta(const FrameDev& F, int px, int py, double& pt, double& pp) {
@@ -809,19 +822,19 @@ KT_HD bool adjoint_event(const FrameDev& F, double pt, double pp, int N, const d
     return true;
 }
 
-// A pixel's gradient from its event entry ev and its nseg segments' sweeps seg (segment k,
-// direction d at seg + (4k + d) kSegOut): the end adjoint carried back segment by segment, then
-// through the camera map.
+// A pixel's gradient from its event entry ev and its nseg segments' sweeps (segment k < nseg - 1,
+// direction d at seg + (4k + d) kSegOut; the last segment's at last + d kSegOut): the end adjoint
+// carried back segment by segment, then through the camera map.
 template <int CHART>
 KT_HD void adjoint_compose(const FrameDev& F, int px, int py, int nseg, const double* ev, const double* seg,
-                           double grad[AP_N]) {
+                           const double* last, double grad[AP_N]) {
     double al, be;
     pixel_angles(F, (double)px, (double)py, al, be);
     double y0[4], cc[2], Jy[4][AP_N], Jc[2][AP_N];
     init_jacobian<CHART>(F, al, be, y0, cc, Jy, Jc);
     double lam[4] = {ev[0], ev[1], ev[2], ev[3]}, gc[2] = {ev[4], ev[5]}, gP[3] = {ev[6], ev[7], ev[8]};
     for (int k = nseg - 1; k >= 0; --k) {
-        const double* S = seg + 4*kSegOut*k;
+        const double* S = (k == nseg - 1) ? last : seg + 4*kSegOut*k;
         double nl[4] = {0.0, 0.0, 0.0, 0.0};
         for (int d = 0; d < 4; ++d) {
             const double* o = S + kSegOut*d;

@@ -654,41 +654,99 @@ struct AdjBackArgs {
     unsigned long long* words;     // [0] rays differentiated, [1] seeds without a disk root,
                                    // [2] reverse-pass entries (segments + event entries),
                                    // [3] accepted steps of the pixels with entries
-    int64_t* base;                 // per pixel: its first entry, -1 when it has none
+    int64_t* base;                 // per pixel, 3 values: its first full segment's entry, its last
+                                   // segment's entry, its event entry (-1 when it has none)
+    unsigned long long* cls;       // kAdjClasses class offsets, then kAdjClasses cursors (count kernel)
     int32_t* tab;                  // per entry: pixel, segment (-1: the event entry)
     int64_t max_entries;           // capacity of tab (host bound from the step counts)
     double* out;                   // per entry and basis direction: kSegOut values
 };
 
-// Reverse pass, 1 of 3: the entries of every disk pixel, nseg segments then its event entry, in
-// one contiguous range per pixel (the ranges' order depends on timing, their contents do not).
-__global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
-    unsigned long long steps = 0;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < B.n_pix;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        const int N = B.n_acc[p];
-        if (!(B.inten[p] > 0.0) || N < 1 || N > B.cap) { B.base[p] = -1; continue; }
-        const int nseg = adjoint_segments(N);
-        const int64_t b = (int64_t)atomicAdd(&B.words[2], (unsigned long long)(nseg + 1));
-        steps += (unsigned long long)N;
-        if (b + nseg + 1 > B.max_entries) { B.base[p] = -1; continue; }    // not reached: the host bound holds
-        B.base[p] = b;
-        for (int k = 0; k <= nseg; ++k) {
-            B.tab[2*(b + k)] = (int32_t)p;
-            B.tab[2*(b + k) + 1] = k < nseg ? k : -1;
+// Slots in a class-ordered table, one atomic per class present in the wave: lane i takes want_i
+// slots of class c_i; lanes of one class get consecutive ranges (in lane order) after the class's
+// offset off[c] (cur: the classes' cursors).
+__device__ int64_t wave_alloc(const unsigned long long* off, unsigned long long* cur, int c, int want) {
+    const int lane = threadIdx.x & 63;
+    int64_t slot = -1;
+    bool pending = want > 0;
+    unsigned long long todo = __ballot(pending);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int cl = __shfl(c, leader, 64);
+        const bool mine = pending && c == cl;
+        // inclusive prefix of want over the lanes of this class
+        unsigned long long v = mine ? (unsigned long long)want : 0ull, incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
         }
+        const unsigned long long total = __shfl(incl, 63, 64);
+        unsigned long long b = 0;
+        if (lane == leader) b = atomicAdd(&cur[cl], total);
+        b = __shfl(b, leader, 64);
+        if (mine) { slot = (int64_t)(off[cl] + b + incl - v); pending = false; }
+        todo = __ballot(pending);
+    }
+    return slot;
+}
+
+// Reverse pass, 1 of 3: the entries of every disk pixel, grouped by cost class (all full
+// segments, then the last segments by length, then the event entries by replay length), so
+// that the sweeps side by side in a wave take the same number of steps and the same code path.
+// A pixel's full segments are consecutive; within a class the order depends on timing, the
+// contents do not.
+__global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
+    unsigned long long steps = 0, entries = 0;
+    const unsigned long long* off = B.cls;
+    unsigned long long* cur = B.cls + kAdjClasses;
+    // every lane of a wave takes the same number of trips, so the wave-wide allocation below
+    // sees all 64 lanes
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t trips = (B.n_pix + stride - 1) / stride;
+    for (int64_t t = 0; t < trips; ++t) {
+        const int64_t p = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int N = p < B.n_pix ? B.n_acc[p] : 0;
+        const bool on = p < B.n_pix && B.inten[p] > 0.0 && N >= 1 && N <= B.cap;
+        const int nseg = on ? adjoint_segments(N) : 0;
+        const int64_t jf = wave_alloc(off, cur, 0, nseg >= 2 ? nseg - 1 : 0);
+        const int64_t jl = wave_alloc(off, cur, nseg >= 1 ? adjoint_last_class(N) : 0, nseg >= 1 ? 1 : 0);
+        const int64_t je = wave_alloc(off, cur, on ? adjoint_event_class(N) : 0, on ? 1 : 0);
+        if (p >= B.n_pix) continue;
+        int64_t* bp = B.base + 3*p;
+        // not reached: the class offsets come from the same pixels, within the host's bound
+        const bool fits = on && je < B.max_entries && jl < B.max_entries && jf + nseg - 1 <= B.max_entries;
+        if (!fits) { bp[0] = -1; bp[1] = -1; bp[2] = -1; continue; }
+        bp[0] = jf; bp[1] = jl; bp[2] = je;
+        for (int k = 0; k + 1 < nseg; ++k) {
+            B.tab[2*(jf + k)] = (int32_t)p;
+            B.tab[2*(jf + k) + 1] = k;
+        }
+        if (nseg >= 1) { B.tab[2*jl] = (int32_t)p; B.tab[2*jl + 1] = nseg - 1; }
+        B.tab[2*je] = (int32_t)p;
+        B.tab[2*je + 1] = -1;
+        steps += (unsigned long long)N;
+        entries += (unsigned long long)(nseg + 1);
     }
     steps = wave_sum(steps);
-    if ((threadIdx.x & 63) == 0 && steps) atomicAdd(&B.words[3], steps);
+    entries = wave_sum(entries);
+    if ((threadIdx.x & 63) == 0) {
+        if (steps) atomicAdd(&B.words[3], steps);
+        if (entries) atomicAdd(&B.words[2], entries);
+    }
 }
 
 // After the forward pass: the longest ray (does the record stride hold it?), the number of
 // reverse-pass entries if every ray ended on the disk (the entry table's capacity), and the
 // steps and checkpoints recorded, so that the host reads 32 bytes instead of every pixel's
-// count.  One block of 1024 threads.
-__global__ __launch_bounds__(1024) void kt_adjoint_count_kernel(const int32_t* n_acc, int64_t n, int64_t* info) {
+// count; and the entry table's class offsets over the disk pixels, with zeroed cursors for the
+// table kernel (cls).  One block of 1024 threads.
+__global__ __launch_bounds__(1024) void kt_adjoint_count_kernel(const int32_t* n_acc, const double* inten, int64_t n,
+                                                                int32_t cap, int64_t* info, unsigned long long* cls) {
     __shared__ int64_t acc[4][1024];
+    __shared__ unsigned long long hist[kAdjClasses];
     const int t = threadIdx.x;
+    if (t < kAdjClasses) hist[t] = 0ull;
+    __syncthreads();
     int64_t m = 0, e = 0, st = 0, ck = 0;
     for (int64_t i = t; i < n; i += 1024) {
         const int N = n_acc[i];
@@ -696,6 +754,12 @@ __global__ __launch_bounds__(1024) void kt_adjoint_count_kernel(const int32_t* n
         if (N >= 1) e += adjoint_segments(N) + 1;
         st += N > 0 ? N : 0;
         ck += N > 0 ? (N + kCkptEvery - 1) / kCkptEvery : 1;
+        if (inten[i] > 0.0 && N >= 1 && N <= cap) {
+            const int nseg = adjoint_segments(N);
+            if (nseg >= 2) atomicAdd(&hist[0], (unsigned long long)(nseg - 1));
+            if (nseg >= 1) atomicAdd(&hist[adjoint_last_class(N)], 1ull);
+            atomicAdd(&hist[adjoint_event_class(N)], 1ull);
+        }
     }
     acc[0][t] = m; acc[1][t] = e; acc[2][t] = st; acc[3][t] = ck;
     __syncthreads();
@@ -707,6 +771,10 @@ __global__ __launch_bounds__(1024) void kt_adjoint_count_kernel(const int32_t* n
         __syncthreads();
     }
     if (t < 4) info[t] = acc[t][0];
+    if (t == 0) {
+        unsigned long long o = 0;
+        for (int c = 0; c < kAdjClasses; ++c) { cls[c] = o; o += hist[c]; cls[kAdjClasses + c] = 0ull; }
+    }
 }
 
 // Reverse pass, 2 of 3: four lanes per entry, side by side in the wave (lane d sweeps the basis
@@ -781,16 +849,20 @@ __global__ __launch_bounds__(256) void kt_adjoint_compose_kernel(AdjBackArgs B) 
          p += (int64_t)gridDim.x * blockDim.x) {
         double* gout = B.grad + AP_N*p;
         for (int q = 0; q < AP_N; ++q) gout[q] = 0.0;
-        const int64_t b = B.base[p];
-        if (b < 0) continue;
+        const int64_t* bp = B.base + 3*p;
+        if (bp[2] < 0) continue;
         const int nseg = adjoint_segments(B.n_acc[p]);
-        const double* ev = B.out + 4*(b + nseg)*kSegOut;
+        const double* ev = B.out + 4*bp[2]*kSegOut;
         if (!(ev[kSegOut] == 1.0)) {                 // lane 1 of the event entry: no disk root
             ++n_fail;
             continue;
         }
         double g[AP_N];
-        adjoint_compose<CHART>(F, (int)(p % F.W), (int)(p / F.W), nseg, ev, B.out + 4*b*kSegOut, g);
+        // a pixel with fewer than two segments has no full segment (bp[0] = -1), with none no last
+        // one (bp[1] = -1): adjoint_compose then reads nothing there
+        const double* seg = B.out + 4*(bp[0] >= 0 ? bp[0] : 0)*kSegOut;
+        const double* last = B.out + 4*(bp[1] >= 0 ? bp[1] : 0)*kSegOut;
+        adjoint_compose<CHART>(F, (int)(p % F.W), (int)(p / F.W), nseg, ev, seg, last, g);
         for (int q = 0; q < AP_N; ++q) gout[q] = g[q];
         ++n_ok;
     }
@@ -2444,7 +2516,7 @@ int adjoint_rev_buffers(kt_hip_ctx* c, int64_t pix, int64_t entries) {
         (void)hipFree(c->d_adj_base);
         c->d_adj_base = nullptr;
         c->adj_base_pix = 0;
-        if (hipMalloc(&c->d_adj_base, pix * 8) != hipSuccess)
+        if (hipMalloc(&c->d_adj_base, pix * 24) != hipSuccess)
             return fail(c, KT_E_ALLOC, "adjoint entry bases for %lld pixels", (long long)pix);
         c->adj_base_pix = pix;
     }
@@ -2550,7 +2622,8 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     // 1. one forward pass: counts, intensities and the records at the context's current stride;
     //    a ray longer than the stride makes the host enlarge it and run the pass again
     int32_t cap = c->adj_next_cap;
-    if (!c->d_adj_info && hipMalloc(&c->d_adj_info, 4 * sizeof(int64_t)) != hipSuccess)
+    // info: 4 counts, then the entry table's class offsets and cursors
+    if (!c->d_adj_info && hipMalloc(&c->d_adj_info, (4 + 2 * kAdjClasses) * sizeof(int64_t)) != hipSuccess)
         return fail(c, KT_E_ALLOC, "adjoint counters");
     int64_t info[4] = {0, 0, 0, 0};
     for (int pass = 0; pass < 2; ++pass) {
@@ -2563,8 +2636,9 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         rec.ckpt = c->d_adj_ckpt;
         rc = adjoint_forward(c, &o, chart, rays, rec);
         if (rc) return rc;
-        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc, pix,
-                           c->d_adj_info);
+        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc,
+                           c->d_adj_inten, pix, cap, c->d_adj_info,
+                           reinterpret_cast<unsigned long long*>(c->d_adj_info + 4));
         KT_HIP(c, hipGetLastError());
         KT_HIP(c, hipMemcpyAsync(info, c->d_adj_info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
         KT_HIP(c, hipStreamSynchronize(c->stream));
@@ -2599,6 +2673,7 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     B.grad = c->d_adj_grad;
     B.words = c->d_words;
     B.base = c->d_adj_base;
+    B.cls = reinterpret_cast<unsigned long long*>(c->d_adj_info + 4);
     B.tab = c->d_adj_tab;
     B.max_entries = c->adj_entries;
     B.out = c->d_adj_out;

@@ -189,7 +189,7 @@ static bool adjoint_pixel_segmented(const FrameDev& F, int px, int py, int N, co
             adjoint_segment<CHART>(F, pt, pp, N, hs, ck, k, d, st, &seg[(size_t)(4*k + d)*kSegOut]);
     double ev[kSegOut];
     if (!adjoint_event<CHART>(F, pt, pp, N, hs, ck, st, ev)) return false;
-    adjoint_compose<CHART>(F, px, py, nseg, ev, seg.data(), g);
+    adjoint_compose<CHART>(F, px, py, nseg, ev, seg.data(), seg.data() + 4*kSegOut*(nseg > 0 ? nseg - 1 : 0), g);
     return true;
 }
 
