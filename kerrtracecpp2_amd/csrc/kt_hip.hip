@@ -656,64 +656,78 @@ struct AdjBackArgs {
                                    // [3] accepted steps of the pixels with entries
     int64_t* base;                 // per pixel, 3 values: its first full segment's entry, its last
                                    // segment's entry, its event entry (-1 when it has none)
-    unsigned long long* cls;       // kAdjClasses class offsets, then kAdjClasses cursors (count kernel)
+    unsigned long long* cls;       // kAdjClasses class counts (classes kernel), then kAdjClasses cursors
     int32_t* tab;                  // per entry: pixel, segment (-1: the event entry)
     int64_t max_entries;           // capacity of tab (host bound from the step counts)
     double* out;                   // per entry and basis direction: kSegOut values
 };
 
-// Slots in a class-ordered table, one atomic per class present in the wave: lane i takes want_i
-// slots of class c_i; lanes of one class get consecutive ranges (in lane order) after the class's
-// offset off[c] (cur: the classes' cursors).
-__device__ int64_t wave_alloc(const unsigned long long* off, unsigned long long* cur, int c, int want) {
-    const int lane = threadIdx.x & 63;
-    int64_t slot = -1;
-    bool pending = want > 0;
-    unsigned long long todo = __ballot(pending);
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int cl = __shfl(c, leader, 64);
-        const bool mine = pending && c == cl;
-        // inclusive prefix of want over the lanes of this class
-        unsigned long long v = mine ? (unsigned long long)want : 0ull, incl = v;
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned long long u = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += u;
-        }
-        const unsigned long long total = __shfl(incl, 63, 64);
-        unsigned long long b = 0;
-        if (lane == leader) b = atomicAdd(&cur[cl], total);
-        b = __shfl(b, leader, 64);
-        if (mine) { slot = (int64_t)(off[cl] + b + incl - v); pending = false; }
-        todo = __ballot(pending);
+// The disk pixels' entries per cost class: a block histogram in LDS, then one global atomic per
+// class and block into cls[0 .. kAdjClasses) (zeroed by the count kernel).
+__device__ __forceinline__ bool adjoint_pixel_on(const AdjBackArgs& B, int64_t p, int& N) {
+    N = B.n_acc[p];
+    return B.inten[p] > 0.0 && N >= 1 && N <= B.cap;
+}
+
+__global__ __launch_bounds__(256) void kt_adjoint_classes_kernel(AdjBackArgs B) {
+    __shared__ unsigned int hist[kAdjClasses];
+    const int t = threadIdx.x;
+    if (t < kAdjClasses) hist[t] = 0u;
+    __syncthreads();
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + t; p < B.n_pix; p += (int64_t)gridDim.x * blockDim.x) {
+        int N;
+        if (!adjoint_pixel_on(B, p, N)) continue;
+        const int nseg = adjoint_segments(N);
+        if (nseg >= 2) atomicAdd(&hist[0], (unsigned int)(nseg - 1));
+        if (nseg >= 1) atomicAdd(&hist[adjoint_last_class(N)], 1u);
+        atomicAdd(&hist[adjoint_event_class(N)], 1u);
     }
-    return slot;
+    __syncthreads();
+    if (t < kAdjClasses && hist[t]) atomicAdd(&B.cls[t], (unsigned long long)hist[t]);
 }
 
 // Reverse pass, 1 of 3: the entries of every disk pixel, grouped by cost class (all full
 // segments, then the last segments by length, then the event entries by replay length), so
 // that the sweeps side by side in a wave take the same number of steps and the same code path.
-// A pixel's full segments are consecutive; within a class the order depends on timing, the
-// contents do not.
+// Per trip a block ranks its pixels' entries within each class in LDS and reserves each class's
+// range with one global atomic; a pixel's full segments are consecutive.  Within a class the
+// order depends on timing, the contents do not.
 __global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
-    unsigned long long steps = 0, entries = 0;
-    const unsigned long long* off = B.cls;
+    __shared__ unsigned long long off[kAdjClasses], base[kAdjClasses];
+    __shared__ unsigned int cnt[kAdjClasses];
+    const int t = threadIdx.x;
+    const unsigned long long* hist = B.cls;
     unsigned long long* cur = B.cls + kAdjClasses;
-    // every lane of a wave takes the same number of trips, so the wave-wide allocation below
-    // sees all 64 lanes
+    if (t == 0) {
+        unsigned long long o = 0;
+        for (int c = 0; c < kAdjClasses; ++c) { off[c] = o; o += hist[c]; }
+    }
+    unsigned long long steps = 0, entries = 0;
+    // the same number of trips in every block: the barriers below are reached by all threads
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t trips = (B.n_pix + stride - 1) / stride;
-    for (int64_t t = 0; t < trips; ++t) {
-        const int64_t p = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        const int N = p < B.n_pix ? B.n_acc[p] : 0;
-        const bool on = p < B.n_pix && B.inten[p] > 0.0 && N >= 1 && N <= B.cap;
+    for (int64_t trip = 0; trip < trips; ++trip) {
+        const int64_t p = trip * stride + (int64_t)blockIdx.x * blockDim.x + t;
+        int N = 0;
+        const bool on = p < B.n_pix && adjoint_pixel_on(B, p, N);
         const int nseg = on ? adjoint_segments(N) : 0;
-        const int64_t jf = wave_alloc(off, cur, 0, nseg >= 2 ? nseg - 1 : 0);
-        const int64_t jl = wave_alloc(off, cur, nseg >= 1 ? adjoint_last_class(N) : 0, nseg >= 1 ? 1 : 0);
-        const int64_t je = wave_alloc(off, cur, on ? adjoint_event_class(N) : 0, on ? 1 : 0);
+        const int cl = nseg >= 1 ? adjoint_last_class(N) : 0, ce = on ? adjoint_event_class(N) : 0;
+        __syncthreads();
+        if (t < kAdjClasses) cnt[t] = 0u;
+        __syncthreads();
+        unsigned int rf = 0, rl = 0, re = 0;           // ranks within the block's share of each class
+        if (nseg >= 2) rf = atomicAdd(&cnt[0], (unsigned int)(nseg - 1));
+        if (nseg >= 1) rl = atomicAdd(&cnt[cl], 1u);
+        if (on) re = atomicAdd(&cnt[ce], 1u);
+        __syncthreads();
+        if (t < kAdjClasses) base[t] = cnt[t] ? atomicAdd(&cur[t], (unsigned long long)cnt[t]) : 0ull;
+        __syncthreads();
         if (p >= B.n_pix) continue;
         int64_t* bp = B.base + 3*p;
-        // not reached: the class offsets come from the same pixels, within the host's bound
+        const int64_t jf = nseg >= 2 ? (int64_t)(off[0] + base[0] + rf) : -1;
+        const int64_t jl = nseg >= 1 ? (int64_t)(off[cl] + base[cl] + rl) : -1;
+        const int64_t je = on ? (int64_t)(off[ce] + base[ce] + re) : -1;
+        // not reached: the classes' counts come from the same pixels, within the host's bound
         const bool fits = on && je < B.max_entries && jl < B.max_entries && jf + nseg - 1 <= B.max_entries;
         if (!fits) { bp[0] = -1; bp[1] = -1; bp[2] = -1; continue; }
         bp[0] = jf; bp[1] = jl; bp[2] = je;
@@ -729,7 +743,7 @@ __global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
     }
     steps = wave_sum(steps);
     entries = wave_sum(entries);
-    if ((threadIdx.x & 63) == 0) {
+    if ((t & 63) == 0) {
         if (steps) atomicAdd(&B.words[3], steps);
         if (entries) atomicAdd(&B.words[2], entries);
     }
@@ -738,27 +752,32 @@ __global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
 // After the forward pass: the longest ray (does the record stride hold it?), the number of
 // reverse-pass entries if every ray ended on the disk (the entry table's capacity), and the
 // steps and checkpoints recorded, so that the host reads 32 bytes instead of every pixel's
-// count; and the entry table's class offsets over the disk pixels, with zeroed cursors for the
-// table kernel (cls).  One block of 1024 threads.
-__global__ __launch_bounds__(1024) void kt_adjoint_count_kernel(const int32_t* n_acc, const double* inten, int64_t n,
-                                                                int32_t cap, int64_t* info, unsigned long long* cls) {
+// count; it also zeroes the class counts and cursors of the entry table (cls).  One block of
+// 1024 threads.
+__global__ __launch_bounds__(1024) void kt_adjoint_count_kernel(const int32_t* n_acc, int64_t n, int64_t* info,
+                                                                unsigned long long* cls) {
     __shared__ int64_t acc[4][1024];
-    __shared__ unsigned long long hist[kAdjClasses];
     const int t = threadIdx.x;
-    if (t < kAdjClasses) hist[t] = 0ull;
-    __syncthreads();
+    if (t < 2 * kAdjClasses) cls[t] = 0ull;
     int64_t m = 0, e = 0, st = 0, ck = 0;
-    for (int64_t i = t; i < n; i += 1024) {
-        const int N = n_acc[i];
-        if (N > m) m = N;
-        if (N >= 1) e += adjoint_segments(N) + 1;
-        st += N > 0 ? N : 0;
-        ck += N > 0 ? (N + kCkptEvery - 1) / kCkptEvery : 1;
-        if (inten[i] > 0.0 && N >= 1 && N <= cap) {
-            const int nseg = adjoint_segments(N);
-            if (nseg >= 2) atomicAdd(&hist[0], (unsigned long long)(nseg - 1));
-            if (nseg >= 1) atomicAdd(&hist[adjoint_last_class(N)], 1ull);
-            atomicAdd(&hist[adjoint_event_class(N)], 1ull);
+    // eight independent loads in flight per trip: one block would otherwise wait out a global
+    // load latency per count
+    constexpr int kU = 8;
+    for (int64_t i0 = t; i0 < n; i0 += 1024 * kU) {
+        int v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int64_t i = i0 + (int64_t)u * 1024;
+            v[u] = i < n ? n_acc[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (i0 + (int64_t)u * 1024 >= n) continue;
+            const int N = v[u];
+            if (N > m) m = N;
+            if (N >= 1) e += adjoint_segments(N) + 1;
+            st += N > 0 ? N : 0;
+            ck += N > 0 ? (N + kCkptEvery - 1) / kCkptEvery : 1;
         }
     }
     acc[0][t] = m; acc[1][t] = e; acc[2][t] = st; acc[3][t] = ck;
@@ -771,10 +790,6 @@ __global__ __launch_bounds__(1024) void kt_adjoint_count_kernel(const int32_t* n
         __syncthreads();
     }
     if (t < 4) info[t] = acc[t][0];
-    if (t == 0) {
-        unsigned long long o = 0;
-        for (int c = 0; c < kAdjClasses; ++c) { cls[c] = o; o += hist[c]; cls[kAdjClasses + c] = 0ull; }
-    }
 }
 
 // Reverse pass, 2 of 3: four lanes per entry, side by side in the wave (lane d sweeps the basis
@@ -2636,9 +2651,8 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         rec.ckpt = c->d_adj_ckpt;
         rc = adjoint_forward(c, &o, chart, rays, rec);
         if (rc) return rc;
-        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc,
-                           c->d_adj_inten, pix, cap, c->d_adj_info,
-                           reinterpret_cast<unsigned long long*>(c->d_adj_info + 4));
+        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc, pix,
+                           c->d_adj_info, reinterpret_cast<unsigned long long*>(c->d_adj_info + 4));
         KT_HIP(c, hipGetLastError());
         KT_HIP(c, hipMemcpyAsync(info, c->d_adj_info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
         KT_HIP(c, hipStreamSynchronize(c->stream));
@@ -2681,6 +2695,7 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
     const int64_t pblocks = std::min<int64_t>((pix + 255) / 256, (int64_t)c->n_cu * 8);
     const int64_t sblocks = std::max<int64_t>(1, (4 * entries + 255) / 256);
+    hipLaunchKernelGGL(kt_adjoint_classes_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
     hipLaunchKernelGGL(kt_adjoint_table_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
     if (chart == KT_CHART_KS) {
         hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_KS>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
