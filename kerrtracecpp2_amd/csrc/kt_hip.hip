@@ -617,21 +617,37 @@ void kt_bundle_kernel(TraceArgs A) {
 // Initial states of every pixel (camera ray, covector of the chart, sin/cos, first RHS).
 template <int CHART>
 __global__ __launch_bounds__(256) void kt_init_kernel(const FrameDev* frames, const LaunchDesc* desc, RayInit* out) {
+    // the block's 256 records are staged in LDS and written out as consecutive 16-byte pieces, so
+    // each store instruction covers 1 KB of contiguous HBM instead of one 16-byte piece per record
+    __shared__ uint4 stage[256 * (sizeof(RayInit) / 16)];
+    constexpr int kPieces = (int)(sizeof(RayInit) / 16);
     const int n_frames = desc->n_frames;
     const int64_t n_pix = desc->n_pix;
     RunOpts O;
     O.step_init = 1.0;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n_pix; p += (int64_t)gridDim.x * blockDim.x) {
-        int lo = 0, hi = n_frames - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (frames[mid].pix_base <= p) lo = mid; else hi = mid - 1;
+    const int t = threadIdx.x;
+    // block-uniform trips: every thread reaches the barriers
+    for (int64_t p0 = (int64_t)blockIdx.x * 256; p0 < n_pix; p0 += (int64_t)gridDim.x * 256) {
+        const int64_t p = p0 + t;
+        if (p < n_pix) {
+            int lo = 0, hi = n_frames - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (frames[mid].pix_base <= p) lo = mid; else hi = mid - 1;
+            }
+            const FrameDev& F = frames[lo];
+            const int64_t loc = p - F.pix_base;
+            RayState S;
+            ray_init<CHART, PREC_FP64>(F, O, (int)(loc % F.W), (int)(loc / F.W), S);
+            RayInit R;
+            ray_store_init(S, R);
+            __builtin_memcpy(&stage[t*kPieces], &R, sizeof(RayInit));
         }
-        const FrameDev& F = frames[lo];
-        const int64_t loc = p - F.pix_base;
-        RayState S;
-        ray_init<CHART, PREC_FP64>(F, O, (int)(loc % F.W), (int)(loc / F.W), S);
-        ray_store_init(S, out[p]);
+        __syncthreads();
+        const int64_t n_here = (n_pix - p0 < 256) ? n_pix - p0 : 256;
+        uint4* dst = reinterpret_cast<uint4*>(out + p0);
+        for (int i = t; i < n_here * kPieces; i += 256) dst[i] = stage[i];
+        __syncthreads();
     }
 }
 
