@@ -230,3 +230,31 @@ def test_gpu_adjoint_other_spacetimes_equal_host_adjoint(gpu_ctx, chart, kw):
     gd = out["dI_dp"].reshape(-1, 5)[idx]
     scale = np.abs(G).max(axis=0) + 1e-30
     assert np.all(np.abs(gd - G) <= 1e-5 * scale + 1e-5 * np.abs(G))
+
+
+def test_gpu_reverse_pass_every_disk_pixel_of_a_close_frame(gpu_ctx):
+    """The GPU's reverse pass lays its entries out by cost class (full segments; last segments of
+    16 .. 1 steps; event entries replaying 16 .. 1 steps).  Every disk pixel of a 96x96 frame with
+    the camera at 14 M, where many rays end within a few steps, against the host build's
+    sequential sweep: intensity and dI/dp.  The pixels cover every class, rays with no full
+    segment, and step counts of 16 k + 1 and 16 k + 16."""
+    fr = kt.make_frame(96, 96, fov=60.0, disk_out=12.0, a=0.7, Q=0.2, Lambda=0.0, theta=75.0, r_obs=14.0)
+    o = _opts()
+    out = gpu_ctx.render_adjoint(fr, o, want_pixel_grads=True)
+    I_gpu = out["intensity"].ravel()
+    idx = np.flatnonzero(I_gpu > 0)
+    assert out["stats"]["seed_failures"] == 0 and idx.size > 1000
+    ho = _opts()
+    ho.quirks |= 1 << 30
+    px, py = (idx % 96).astype(np.int32), (idx // 96).astype(np.int32)
+    I, G, N, _ = hc.adjoint_pixels(fr, ho, px, py)
+    np.testing.assert_allclose(I_gpu[idx], I, rtol=1e-7)
+    gd = out["dI_dp"].reshape(-1, 5)[idx]
+    scale = np.abs(G).max(axis=0) + 1e-30
+    assert np.all(np.abs(gd - G) <= 1e-5 * scale + 1e-5 * np.abs(G))
+    nseg = (N - 1 + 15) // 16
+    last = (N - 1) - 16 * (nseg - 1)
+    replay = N - 16 * ((N - 1) // 16)
+    print("disk pixels", idx.size, "steps", N.min(), N.max(), flush=True)
+    assert (nseg == 1).any() and ((N % 16) == 1).any() and ((N % 16) == 0).any()
+    assert set(last[nseg >= 1]) == set(range(1, 17)) and set(replay) == set(range(1, 17))
