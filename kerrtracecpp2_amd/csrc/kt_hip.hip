@@ -1118,6 +1118,7 @@ struct kt_hip_ctx {
     RayInit* d_init = nullptr; size_t cap_init = 0;     // per-pixel initial states
     uint8_t* d_bg = nullptr; int bg_w = 0, bg_h = 0; size_t cap_bg = 0;
     unsigned long long* d_words = nullptr;
+    unsigned long long* h_words = nullptr;  // pinned: the statistics copied back on the stream
     uint8_t* d_rgb = nullptr; size_t cap_rgb = 0;       // camera-spp resolved RGB8
     double* d_tab = nullptr; double tab_gamma = -1.0;  // camera-spp linear table + thresholds
     // autotuner results per (chart, integrator, precision, bundles)
@@ -1272,7 +1273,8 @@ int kt_hip_create(int device, kt_hip_ctx** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return KT_E_DEVICE; }
     for (int i = 0; i < 4; ++i)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
-    if (hipMalloc(&c->d_words, W_COUNT * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&c->d_words, W_COUNT * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc(&c->h_words, W_COUNT * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         kt_hip_destroy(c);
         return KT_E_ALLOC;
     }
@@ -1293,6 +1295,7 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     (void)hipFree(c->d_init);
     (void)hipFree(c->d_bg);
     (void)hipFree(c->d_words);
+    if (c->h_words) (void)hipHostFree(c->h_words);
     (void)hipFree(c->d_rgb);
     (void)hipFree(c->d_tab);
     for (auto& kv : c->orders) (void)hipFree(kv.second);
@@ -1686,6 +1689,25 @@ static int enqueue_render(kt_hip_ctx* c, const RenderBufs& B, const kt_opts* o, 
     return KT_OK;
 }
 
+static void fill_stats(const unsigned long long* w, float t_trace, float t_total, kt_stats* st) {
+    st->trace_ms = t_trace;
+    st->total_ms = t_total;
+    st->rays = w[W_RAYS];
+    st->steps_accepted = w[W_ACC];
+    st->steps_rejected = w[W_REJ];
+    st->rhs_fp32 = w[W_RHS32];
+    st->rhs_fp64 = w[W_RHS64];
+    st->events = w[W_HERM];
+}
+
+// The statistics words onto the context's pinned copy, on the stream: the render's one
+// synchronisation then covers them (read them with fill_stats(c->h_words, ...) after it).
+static int enqueue_stats(kt_hip_ctx* c, const unsigned long long* d_words) {
+    KT_HIP(c, hipMemcpyAsync(c->h_words, d_words, W_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    return KT_OK;
+}
+
 static int read_stats(kt_hip_ctx* c, const unsigned long long* d_words, float t_trace, float t_total, kt_stats* st) {
     if (!st) return KT_OK;
     unsigned long long w[W_COUNT];
@@ -1716,12 +1738,12 @@ static int render_on_device(kt_hip_ctx* c, const kt_frame* frames, int n_frames,
     RenderBufs B{c->table.desc(), c->table.frames(), c->table.scratch, c->d_geo, c->d_rgba, c->d_words, c->d_init};
     rc = enqueue_render(c, B, &eff, chart, n_frames, rays, n_frames == 1, shade, L, c->ev);
     if (rc) return rc;
+    if (st && (rc = enqueue_stats(c, c->d_words))) return rc;
     KT_HIP(c, hipStreamSynchronize(c->stream));
     float t_trace = 0.f, t_total = 0.f;
     (void)hipEventElapsedTime(&t_trace, c->ev[1], c->ev[2]);
     (void)hipEventElapsedTime(&t_total, c->ev[0], c->ev[3]);
-    rc = read_stats(c, c->d_words, t_trace, t_total, st);
-    if (rc) return rc;
+    if (st) fill_stats(c->h_words, t_trace, t_total, st);
     *n_pix_out = pix;
     return KT_OK;
 }
