@@ -5,7 +5,7 @@ Tolerances (floating point, per pixel, outside the photon-ring band: pixels with
 8-neighbour of another fate in the oracle image -- photon ring, shadow and disk edges, where
 the map is discontinuous -- and disk pixels whose oracle g differs from a neighbour's by more
 than 0.05 -- near-critical rays on the disk's inner rim, ~1-3 % of disk pixels):
-  fate agreement >= 99.9 %; |dg| <= 1e-4 at tol 1e-9 and <= 5e-4 at the
+  fate agreement >= 99.9 %; |dg| <= 1e-4 at tol 1e-9 (the mixed-precision accuracy bar) and <= 5e-4 at the
   reference's default tol 1e-7, where the reference's own error against a tol 1e-11 run is
   already 1.6e-4; |dr_hit| <= 2e-3 (tol 1e-7) / 1e-3 (tol 1e-9).
 Frame scalars (horizon, ISCO) are bit-exact.
