@@ -323,12 +323,26 @@ __global__ void kt_setup_final_kernel(FrameDev* frames, const LaunchDesc* desc, 
 // per-pixel records of the adjoint's forward pass, at a fixed stride per pixel: pixel p's step
 // sizes are hs[p cap ..], its checkpoints ckpt[6 (p cap/16) ..]; a ray longer than cap steps
 // stops recording (its count tells the host, which enlarges cap and runs the pass again)
+// The adjoint's forward pass parks the rays still running after this many accepted steps (a
+// multiple of the checkpoint interval): at C5 that is the first part of the pass for most of the
+// frame, and the parked rays, a few percent, are its long tail.
+constexpr int32_t kAdjPause = 128;
+
 struct AdjRecord {
-    int32_t* n_acc;          // accepted steps of the ray (written when it ends)
+    int32_t* n_acc;          // accepted steps of the ray (written when it ends; -1 while parked)
     double* inten;           // disk intensity, 0 off the disk
     int32_t cap;             // step slots per pixel, a multiple of kCkptEvery (0: no records)
     double* hs;              // accepted step sizes
     double* ckpt;            // r, theta, p_r, p_theta, sin theta, cos theta per checkpoint
+    // Parking (pause > 0): a ray still running after pause accepted steps is parked -- its
+    // state saved, its pixel appended to list -- and finished by kt_adjoint_resume_kernel on a
+    // second stream while the reverse pass sweeps the pixels that have ended.
+    int32_t pause;           // accepted steps after which a running ray is parked (0: never)
+    int32_t end_phase;       // written to phase[] when a ray ends: 1 first pass, 2 resumed
+    RayState* park;          // per pixel: a parked ray's state
+    int8_t* phase;           // per pixel: the pass in which its ray ended (0: parked)
+    int32_t* list;           // the parked pixels
+    unsigned long long* n_list;
 };
 
 struct TraceArgs {
@@ -462,6 +476,14 @@ struct AdjointForwardPolicy {
         if (done) {
             R.n_acc[S.pix] = S.g.n_acc;
             R.inten[S.pix] = (o.fate == 1) ? disk_intensity_t<double>(o.r, o.g, F.r_isco) : 0.0;
+            if (R.phase) R.phase[S.pix] = (int8_t)R.end_phase;
+        } else if (R.pause > 0 && S.g.n_acc >= R.pause) {
+            R.park[S.pix] = S.g;
+            R.phase[S.pix] = 0;
+            R.n_acc[S.pix] = -1;
+            R.list[atomicAdd(R.n_list, 1ull)] = (int32_t)S.pix;
+            stopped_result<CHART>(S.g, A.O.quirks, o);     // a defined GeoPixel; the adjoint reads none
+            return true;
         }
         return done;
     }
@@ -658,6 +680,27 @@ void kt_adjoint_forward_kernel(TraceArgs A) {
     trace_loop<AdjointForwardPolicy<CHART>, true>(A);
 }
 
+// The parked rays of the adjoint's forward pass, one lane each, packed into as few waves as
+// they fill: each runs to its end with the forward pass's own step (records continue at the
+// ray's step count).  These waves set the end of the forward pass, so they take the highest
+// issue priority; the reverse pass meanwhile runs on the other SIMDs.  A.rec.pause = 0.
+template <int CHART>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void kt_adjoint_resume_kernel(TraceArgs A, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    __builtin_amdgcn_s_setprio(3);
+    const FrameDev F = A.frames[0];
+    const Knds<double> P = F.P;
+    const Knds<float> Pf = knds_cast<float>(P);
+    typename AdjointForwardPolicy<CHART>::State S;
+    S.pix = A.rec.list[i];
+    S.g = A.rec.park[S.pix];
+    RayOut o;
+    double magnif = 1.0;
+    while (!AdjointForwardPolicy<CHART>::template advance<true>(A, F, P, Pf, S, o, magnif, true)) {}
+}
+
 struct AdjBackArgs {
     const FrameDev* frame;
     int64_t n_pix;
@@ -673,6 +716,9 @@ struct AdjBackArgs {
     int64_t* base;                 // per pixel, 3 values: its first full segment's entry, its last
                                    // segment's entry, its event entry (-1 when it has none)
     unsigned long long* cls;       // kAdjClasses class counts (classes kernel), then kAdjClasses cursors
+    unsigned long long* n_entries; // this pass's entries (the table kernel counts them)
+    const int8_t* phase;           // per pixel: the forward pass in which its ray ended
+    int32_t want;                  // the pixels of this reverse pass: phase[p] == want
     int32_t* tab;                  // per entry: pixel, segment (-1: the event entry)
     int64_t max_entries;           // capacity of tab (host bound from the step counts)
     double* out;                   // per entry and basis direction: kSegOut values
@@ -682,7 +728,7 @@ struct AdjBackArgs {
 // class and block into cls[0 .. kAdjClasses) (zeroed by the count kernel).
 __device__ __forceinline__ bool adjoint_pixel_on(const AdjBackArgs& B, int64_t p, int& N) {
     N = B.n_acc[p];
-    return B.inten[p] > 0.0 && N >= 1 && N <= B.cap;
+    return B.phase[p] == B.want && B.inten[p] > 0.0 && N >= 1 && N <= B.cap;
 }
 
 __global__ __launch_bounds__(256) void kt_adjoint_classes_kernel(AdjBackArgs B) {
@@ -761,7 +807,7 @@ __global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
     entries = wave_sum(entries);
     if ((t & 63) == 0) {
         if (steps) atomicAdd(&B.words[3], steps);
-        if (entries) atomicAdd(&B.words[2], entries);
+        if (entries) atomicAdd(B.n_entries, entries);
     }
 }
 
@@ -844,7 +890,7 @@ void kt_adjoint_segments_kernel(AdjBackArgs B) {
     LdsStates st;
     st.slot = states[threadIdx.x >> 2];
     st.writer = (threadIdx.x & 3) == 0;
-    const int64_t used = (int64_t)B.words[2];
+    const int64_t used = (int64_t)*B.n_entries;
     const int64_t lanes = 4 * (used < B.max_entries ? used : B.max_entries);
     for (int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; L < lanes; L += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = L >> 2;
@@ -878,6 +924,7 @@ __global__ __launch_bounds__(256) void kt_adjoint_compose_kernel(AdjBackArgs B) 
     unsigned long long n_ok = 0, n_fail = 0;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < B.n_pix;
          p += (int64_t)gridDim.x * blockDim.x) {
+        if (B.phase[p] != B.want) continue;          // the other reverse pass's pixel
         double* gout = B.grad + AP_N*p;
         for (int q = 0; q < AP_N; ++q) gout[q] = 0.0;
         const int64_t* bp = B.base + 3*p;
@@ -1107,6 +1154,9 @@ struct kt_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the adjoint's second stream (default priority): parked rays beside the reverse pass
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev2[2] = {nullptr, nullptr};
     std::string err;
     char name[256] = {0};
     int n_cu = 0;
@@ -1135,6 +1185,8 @@ struct kt_hip_ctx {
     double* d_adj_hs = nullptr; double* d_adj_ckpt = nullptr;
     double* d_adj_w = nullptr; double* d_adj_sum = nullptr;       // pixel weights, weighted sum
     int64_t adj_pix = 0; int32_t adj_cap = 0; int32_t adj_next_cap = 256;
+    RayState* d_adj_park = nullptr; int8_t* d_adj_phase = nullptr; int32_t* d_adj_list = nullptr;
+    unsigned long long* d_adj_words2 = nullptr;        // the parked-ray count, the resumed pass's queue
     // the reverse pass's entry table and per-entry sweeps (adj_entries entries), and each pixel's
     // first entry (adj_base_pix pixels)
     int64_t* d_adj_base = nullptr; int32_t* d_adj_tab = nullptr; double* d_adj_out = nullptr;
@@ -1273,6 +1325,9 @@ int kt_hip_create(int device, kt_hip_ctx** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return KT_E_DEVICE; }
     for (int i = 0; i < 4; ++i)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
+    for (int i = 0; i < 2; ++i)
+        if (hipEventCreateWithFlags(&c->ev2[i], hipEventDisableTiming) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
     if (hipMalloc(&c->d_words, W_COUNT * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(&c->h_words, W_COUNT * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         kt_hip_destroy(c);
@@ -1287,6 +1342,7 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     (void)hipFree(c->table.dev);
     (void)hipFree(c->table.scratch);
     if (c->table.host) (void)hipHostFree(c->table.host);
@@ -1303,8 +1359,12 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     (void)hipFree(c->d_adj_hs); (void)hipFree(c->d_adj_ckpt); (void)hipFree(c->d_adj_w); (void)hipFree(c->d_adj_sum);
     (void)hipFree(c->d_adj_base); (void)hipFree(c->d_adj_tab); (void)hipFree(c->d_adj_out);
     (void)hipFree(c->d_adj_part); (void)hipFree(c->d_adj_info);
+    (void)hipFree(c->d_adj_park); (void)hipFree(c->d_adj_phase); (void)hipFree(c->d_adj_list);
+    (void)hipFree(c->d_adj_words2);
     for (int i = 0; i < 4; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 2; ++i) if (c->ev2[i]) (void)hipEventDestroy(c->ev2[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
 }
 
@@ -2547,14 +2607,17 @@ int adjoint_buffers(kt_hip_ctx* c, int64_t pix, int32_t cap) {
     const int32_t nc = std::max(cap, c->adj_cap);
     (void)hipFree(c->d_adj_nacc); (void)hipFree(c->d_adj_inten); (void)hipFree(c->d_adj_grad);
     (void)hipFree(c->d_adj_hs); (void)hipFree(c->d_adj_ckpt); (void)hipFree(c->d_adj_w); (void)hipFree(c->d_adj_sum);
+    (void)hipFree(c->d_adj_park); (void)hipFree(c->d_adj_phase); (void)hipFree(c->d_adj_list);
     c->d_adj_nacc = nullptr; c->d_adj_inten = nullptr; c->d_adj_grad = nullptr;
     c->d_adj_hs = nullptr; c->d_adj_ckpt = nullptr; c->d_adj_w = nullptr; c->d_adj_sum = nullptr;
+    c->d_adj_park = nullptr; c->d_adj_phase = nullptr; c->d_adj_list = nullptr;
     c->adj_pix = 0; c->adj_cap = 0;
     const size_t hs_bytes = (size_t)np * nc * 8, ck_bytes = (size_t)np * (nc / kCkptEvery) * 48;
     if (hipMalloc(&c->d_adj_nacc, np * 4) != hipSuccess || hipMalloc(&c->d_adj_inten, np * 8) != hipSuccess ||
         hipMalloc(&c->d_adj_grad, np * 8 * AP_N) != hipSuccess || hipMalloc(&c->d_adj_hs, hs_bytes) != hipSuccess ||
         hipMalloc(&c->d_adj_ckpt, ck_bytes) != hipSuccess || hipMalloc(&c->d_adj_w, np * 8) != hipSuccess ||
-        hipMalloc(&c->d_adj_sum, AP_N * 8) != hipSuccess)
+        hipMalloc(&c->d_adj_sum, AP_N * 8) != hipSuccess || hipMalloc(&c->d_adj_park, np * sizeof(RayState)) != hipSuccess ||
+        hipMalloc(&c->d_adj_phase, np) != hipSuccess || hipMalloc(&c->d_adj_list, np * 4) != hipSuccess)
         return fail(c, KT_E_ALLOC, "adjoint records for %lld pixels x %d steps (%zu bytes)", (long long)np, nc,
                     hs_bytes + ck_bytes);
     c->adj_pix = np;
@@ -2672,32 +2735,118 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     if (!grad_out && !dI_out) return fail(c, KT_E_ARG, "grad_out and dI_dp_out are both NULL");
     kt_adjoint_stats S;
     std::memset(&S, 0, sizeof(S));
-    // 1. one forward pass: counts, intensities and the records at the context's current stride;
-    //    a ray longer than the stride makes the host enlarge it and run the pass again
+    // The forward pass runs in two parts.  The first takes every ray up to kAdjPause accepted
+    // steps; a ray still running then is parked.  The parked rays (a few percent, the longest)
+    // are finished by the resume kernel on the second stream, while the reverse pass sweeps the
+    // pixels whose rays have ended; the parked pixels are swept after that.  A ray longer than
+    // the record stride makes the host enlarge it and run the call again.
     int32_t cap = c->adj_next_cap;
-    // info: 4 counts, then the entry table's class offsets and cursors
-    if (!c->d_adj_info && hipMalloc(&c->d_adj_info, (4 + 2 * kAdjClasses) * sizeof(int64_t)) != hipSuccess)
+    // info: 4 counts and the entry table's class counters for each of the two reverse passes
+    constexpr int kInfo = 4 + 2 * kAdjClasses;
+    if (!c->d_adj_info && hipMalloc(&c->d_adj_info, 2 * kInfo * sizeof(int64_t)) != hipSuccess)
         return fail(c, KT_E_ALLOC, "adjoint counters");
-    int64_t info[4] = {0, 0, 0, 0};
+    if (!c->d_adj_words2 && hipMalloc(&c->d_adj_words2, W_COUNT * sizeof(unsigned long long)) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "adjoint counters");
+    if (grad_out && !c->d_adj_part && hipMalloc(&c->d_adj_part, kSumBlocks * AP_N * 8) != hipSuccess)
+        return fail(c, KT_E_ALLOC, "adjoint partial sums");
+    int64_t* infoA = c->d_adj_info;
+    int64_t* infoB = c->d_adj_info + kInfo;
+    auto cls_of = [](int64_t* info) { return reinterpret_cast<unsigned long long*>(info + 4); };
+    int64_t hA[5] = {0, 0, 0, 0, 0}, hB[4] = {0, 0, 0, 0};
+    AdjBackArgs B;
     for (int pass = 0; pass < 2; ++pass) {
         if ((rc = adjoint_buffers(c, pix, cap))) return rc;
+        const int32_t pause = kAdjPause < cap ? kAdjPause : 0;
         AdjRecord rec;
         rec.n_acc = c->d_adj_nacc;
         rec.inten = c->d_adj_inten;
         rec.cap = cap;
         rec.hs = c->d_adj_hs;
         rec.ckpt = c->d_adj_ckpt;
+        rec.pause = pause;
+        rec.end_phase = 1;
+        rec.park = c->d_adj_park;
+        rec.phase = c->d_adj_phase;
+        rec.list = c->d_adj_list;
+        rec.n_list = c->d_adj_words2;
+        // 1. the first part of the forward pass, to the pause
+        hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_adj_words2, (int)W_COUNT);
         rc = adjoint_forward(c, &o, chart, rays, rec);
         if (rc) return rc;
-        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc, pix,
-                           c->d_adj_info, reinterpret_cast<unsigned long long*>(c->d_adj_info + 4));
+        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc, pix, infoA,
+                           cls_of(infoA));
         KT_HIP(c, hipGetLastError());
-        KT_HIP(c, hipMemcpyAsync(info, c->d_adj_info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
+        KT_HIP(c, hipMemcpyAsync(hA, infoA, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+        KT_HIP(c, hipMemcpyAsync(&hA[4], c->d_adj_words2, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
         KT_HIP(c, hipStreamSynchronize(c->stream));
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
         if (pass == 0) S.forward_ms = ms; else S.record_ms = ms;
-        const int64_t nmax = info[0];
+        const int64_t parked = hA[4];
+        // reverse-pass capacity: the ended pixels' entries, and at most cap/16 + 1 per parked pixel
+        const int64_t entries = std::max<int64_t>({hA[1], parked * (cap / kCkptEvery + 1), 1});
+        if ((rc = adjoint_rev_buffers(c, pix, entries))) return rc;
+        // 2. the parked rays on the second stream, then their counts
+        KT_HIP(c, hipEventRecord(c->ev2[0], c->stream));
+        KT_HIP(c, hipStreamWaitEvent(c->stream2, c->ev2[0], 0));
+        if (parked > 0) {
+            TraceArgs A;
+            std::memset(&A, 0, sizeof(A));
+            A.frames = c->table.frames();
+            A.desc = c->table.desc();
+            A.rec = rec;
+            A.rec.pause = 0;
+            A.rec.end_phase = 2;
+            A.O = run_opts(&o);
+            const unsigned rblocks = (unsigned)((parked + kBlock - 1) / kBlock);
+            if (chart == KT_CHART_KS)
+                hipLaunchKernelGGL(kt_adjoint_resume_kernel<CHART_KS>, dim3(rblocks), dim3(kBlock), 0, c->stream2, A, parked);
+            else
+                hipLaunchKernelGGL(kt_adjoint_resume_kernel<CHART_BL>, dim3(rblocks), dim3(kBlock), 0, c->stream2, A, parked);
+        }
+        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream2, c->d_adj_nacc, pix, infoB,
+                           cls_of(infoB));
+        KT_HIP(c, hipGetLastError());
+        KT_HIP(c, hipEventRecord(c->ev2[1], c->stream2));
+        // 3. the reverse pass over the pixels that ended in the first part, beside step 2
+        B.frame = c->table.frames();
+        B.n_pix = pix;
+        B.n_acc = c->d_adj_nacc;
+        B.inten = c->d_adj_inten;
+        B.cap = cap;
+        B.hs = c->d_adj_hs;
+        B.ckpt = c->d_adj_ckpt;
+        B.grad = c->d_adj_grad;
+        B.words = c->d_words;
+        B.base = c->d_adj_base;
+        B.phase = c->d_adj_phase;
+        B.tab = c->d_adj_tab;
+        B.max_entries = c->adj_entries;
+        B.out = c->d_adj_out;
+        hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
+        KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        const int64_t pblocks = std::min<int64_t>((pix + 255) / 256, (int64_t)c->n_cu * 8);
+        const int64_t sblocks = std::max<int64_t>(1, (4 * entries + 255) / 256);
+        for (int part = 1; part <= 2; ++part) {
+            if (part == 2) KT_HIP(c, hipStreamWaitEvent(c->stream, c->ev2[1], 0));   // the parked rays have ended
+            B.want = part;
+            B.cls = cls_of(part == 1 ? infoA : infoB);
+            B.n_entries = c->d_words + (part == 1 ? 2 : 5);
+            hipLaunchKernelGGL(kt_adjoint_classes_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+            hipLaunchKernelGGL(kt_adjoint_table_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+            if (chart == KT_CHART_KS) {
+                hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_KS>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
+                hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_KS>, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+            } else {
+                hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_BL>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
+                hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_BL>, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+            }
+            KT_HIP(c, hipGetLastError());
+        }
+        KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        KT_HIP(c, hipMemcpyAsync(hB, infoB, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+        KT_HIP(c, hipStreamSynchronize(c->stream));
+        const int64_t nmax = hB[0];
         if (nmax <= cap) break;
         if (pass == 1)
             return fail(c, KT_E_DEVICE, "adjoint records: %lld steps after enlarging to %d", (long long)nmax, cap);
@@ -2705,58 +2854,21 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         cap = (int32_t)((nmax + kCkptEvery - 1) / kCkptEvery * kCkptEvery);
         c->adj_next_cap = cap;
     }
-    const int64_t nh = info[2], nc = info[3];
+    const int64_t nh = hB[2], nc = hB[3];
     S.steps = nh;
     S.checkpoints = nc;
     S.record_bytes = nh * 8 + nc * 48;
     float ms = 0.f;
-    // 2. reverse pass in parallel over checkpoint segments: entry table, four sweeps per entry,
-    //    then one composition per pixel.  The table's capacity bounds every pixel as a disk hit.
-    const int64_t entries = info[1];
-    if ((rc = adjoint_rev_buffers(c, pix, entries > 0 ? entries : 1))) return rc;
-    AdjBackArgs B;
-    B.frame = c->table.frames();
-    B.n_pix = pix;
-    B.n_acc = c->d_adj_nacc;
-    B.inten = c->d_adj_inten;
-    B.cap = cap;
-    B.hs = c->d_adj_hs;
-    B.ckpt = c->d_adj_ckpt;
-    B.grad = c->d_adj_grad;
-    B.words = c->d_words;
-    B.base = c->d_adj_base;
-    B.cls = reinterpret_cast<unsigned long long*>(c->d_adj_info + 4);
-    B.tab = c->d_adj_tab;
-    B.max_entries = c->adj_entries;
-    B.out = c->d_adj_out;
-    hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
-    KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    const int64_t pblocks = std::min<int64_t>((pix + 255) / 256, (int64_t)c->n_cu * 8);
-    const int64_t sblocks = std::max<int64_t>(1, (4 * entries + 255) / 256);
-    hipLaunchKernelGGL(kt_adjoint_classes_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
-    hipLaunchKernelGGL(kt_adjoint_table_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
-    if (chart == KT_CHART_KS) {
-        hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_KS>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
-        hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_KS>, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
-    } else {
-        hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_BL>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
-        hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_BL>, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
-    }
-    KT_HIP(c, hipGetLastError());
-    KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    KT_HIP(c, hipStreamSynchronize(c->stream));
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     S.backward_ms = ms;
     unsigned long long words[W_COUNT];
     KT_HIP(c, hipMemcpy(words, c->d_words, sizeof(words), hipMemcpyDeviceToHost));
     S.rays_differentiated = (int64_t)words[0];
     S.seed_failures = (int64_t)words[1];
-    S.reverse_entries = (int64_t)words[2];
+    S.reverse_entries = (int64_t)(words[2] + words[5]);
     S.steps_differentiated = (int64_t)words[3];
     if (grad_out) {
         if (w) KT_HIP(c, hipMemcpyAsync(c->d_adj_w, w, pix * 8, hipMemcpyHostToDevice, c->stream));
-        if (!c->d_adj_part && hipMalloc(&c->d_adj_part, kSumBlocks * AP_N * 8) != hipSuccess)
-            return fail(c, KT_E_ALLOC, "adjoint partial sums");
         hipLaunchKernelGGL(kt_adjoint_sum_kernel, dim3(kSumBlocks), dim3(256), 0, c->stream, c->d_adj_grad,
                            w ? c->d_adj_w : nullptr, pix, c->d_adj_part);
         hipLaunchKernelGGL(kt_adjoint_sum_final_kernel, dim3(1), dim3(64), 0, c->stream, c->d_adj_part, c->d_adj_sum);
