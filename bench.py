@@ -143,7 +143,9 @@ def c4_run(kt, steps):
 def c5_run(ctx, kt, reps):
     """Differentiable render: 256x256 Kerr-Newman a=0.7 Q=0.2 KS theta=60 DOPRI5; one adjoint call =
     one forward pass (step sizes, checkpoints every 16 steps, at the record stride the context keeps
-    from its earlier calls) + reverse pass.  The untimed first call sizes that stride."""
+    from its earlier calls) + reverse pass.  The untimed first call sizes that stride.  The rays
+    still running after 128 steps finish on a second stream during the reverse pass, so
+    backward_ms holds their time too."""
     fr = kt.make_frame(256, 256, a=0.7, Q=0.2, Lambda=0.0, theta=60.0, r_obs=40.0, fov=30.0, disk_out=12.0)
     o = kt.default_opts()
     o.chart, o.integrator, o.rtol, o.atol = kt.KT_CHART_KS, kt.KT_INTEGRATOR_DOPRI5, RTOL, ATOL
@@ -160,6 +162,7 @@ def c5_run(ctx, kt, reps):
             "backward_ms": bw_ms, "rays_differentiated": st["rays_differentiated"],
             "steps_recorded": st["steps"], "checkpoints": st["checkpoints"], "record_bytes": st["record_bytes"],
             "steps_differentiated": st["steps_differentiated"], "reverse_entries": st["reverse_entries"],
+            "rays_parked": st["rays_parked"],
             "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
                          "frac": ach / PEAK_VALU_TFLOPS,
                          "kernel": "reverse pass (kt_adjoint_segments_kernel<KS> + table + composition)",

@@ -271,10 +271,12 @@ int kt_hip_render_intensity(kt_hip_ctx* ctx, const kt_frame* frame, const kt_opt
                             kt_stats* stats);
 
 typedef struct kt_adjoint_stats {
-    double forward_ms;        /* forward pass: step counts, intensities, step sizes, checkpoints */
-    double record_ms;         /* the forward pass again, when a ray outgrew the record stride
+    double forward_ms;        /* forward pass to 128 accepted steps: step counts, intensities,
+                                 step sizes, checkpoints; rays still running then are parked */
+    double record_ms;         /* that first part again, when a ray outgrew the record stride
                                  (0 otherwise)                                                 */
-    double backward_ms;       /* reverse pass                                                  */
+    double backward_ms;       /* reverse pass, with the parked rays' forward steps finished
+                                 beside its first half on a second stream                      */
     int64_t steps;            /* accepted steps recorded                                       */
     int64_t checkpoints;      /* state checkpoints stored (one per 16 accepted steps)          */
     int64_t record_bytes;     /* device bytes of the step sizes and checkpoints                */
@@ -284,6 +286,7 @@ typedef struct kt_adjoint_stats {
     int64_t reverse_entries;  /* reverse-pass entries: checkpoint segments (four sweeps each,
                                  one per basis direction of the adjoint at the segment end)
                                  and one event entry per disk pixel                            */
+    int64_t rays_parked;      /* rays still running after the forward pass's first part        */
 } kt_adjoint_stats;
 
 /* Hand-written adjoint of the adaptive integrator + disk shading.  Gradients of the per-pixel

@@ -76,7 +76,8 @@ class KtAdjointStats(ctypes.Structure):
     _fields_ = [("forward_ms", ctypes.c_double), ("record_ms", ctypes.c_double), ("backward_ms", ctypes.c_double),
                 ("steps", ctypes.c_int64), ("checkpoints", ctypes.c_int64), ("record_bytes", ctypes.c_int64),
                 ("rays_differentiated", ctypes.c_int64), ("seed_failures", ctypes.c_int64),
-                ("steps_differentiated", ctypes.c_int64), ("reverse_entries", ctypes.c_int64)]
+                ("steps_differentiated", ctypes.c_int64), ("reverse_entries", ctypes.c_int64),
+                ("rays_parked", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

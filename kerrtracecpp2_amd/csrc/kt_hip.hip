@@ -2866,6 +2866,7 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     S.rays_differentiated = (int64_t)words[0];
     S.seed_failures = (int64_t)words[1];
     S.reverse_entries = (int64_t)(words[2] + words[5]);
+    S.rays_parked = hA[4];
     S.steps_differentiated = (int64_t)words[3];
     if (grad_out) {
         if (w) KT_HIP(c, hipMemcpyAsync(c->d_adj_w, w, pix * 8, hipMemcpyHostToDevice, c->stream));

@@ -60,6 +60,9 @@ def test_adjoint_records_and_counts(gpu_ctx, base):
     assert st["seed_failures"] == 0
     assert st["rays_differentiated"] == int((I > 0).sum()) > 10000
     assert st["checkpoints"] * 16 >= st["steps"] and st["record_bytes"] == st["steps"] * 8 + st["checkpoints"] * 48
+    # the forward pass parks the rays that pass 128 accepted steps: the frame has such rays, and
+    # they cannot hold more than the steps past 128 that were recorded
+    assert 0 < st["rays_parked"] and st["rays_parked"] * 129 <= st["steps"]
     I2, _ = gpu_ctx.render_intensity(_frame(), _opts())
     assert np.array_equal(I2, I)                  # the intensity pass and the adjoint's forward agree
 
