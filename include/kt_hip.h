@@ -273,8 +273,8 @@ int kt_hip_render_intensity(kt_hip_ctx* ctx, const kt_frame* frame, const kt_opt
 typedef struct kt_adjoint_stats {
     double forward_ms;        /* forward pass to 128 accepted steps: step counts, intensities,
                                  step sizes, checkpoints; rays still running then are parked */
-    double record_ms;         /* that first part again, when a ray outgrew the record stride
-                                 (0 otherwise)                                                 */
+    double record_ms;         /* that first part again, when a ray outgrew the record stride or
+                                 the reverse pass the context's entry buffer (0 otherwise)     */
     double backward_ms;       /* reverse pass, with the parked rays' forward steps finished
                                  beside its first half on a second stream                      */
     int64_t steps;            /* accepted steps recorded                                       */

@@ -327,6 +327,8 @@ __global__ void kt_setup_final_kernel(FrameDev* frames, const LaunchDesc* desc, 
 // multiple of the checkpoint interval): at C5 that is the first part of the pass for most of the
 // frame, and the parked rays, a few percent, are its long tail.
 constexpr int32_t kAdjPause = 128;
+// pinned host words per context: W_COUNT render statistics, then the adjoint's read-backs
+constexpr int kHostWords = 64;
 
 struct AdjRecord {
     int32_t* n_acc;          // accepted steps of the ray (written when it ends; -1 while parked)
@@ -683,12 +685,13 @@ void kt_adjoint_forward_kernel(TraceArgs A) {
 // The parked rays of the adjoint's forward pass, one lane each, packed into as few waves as
 // they fill: each runs to its end with the forward pass's own step (records continue at the
 // ray's step count).  These waves set the end of the forward pass, so they take the highest
-// issue priority; the reverse pass meanwhile runs on the other SIMDs.  A.rec.pause = 0.
+// issue priority; the reverse pass meanwhile runs on the other SIMDs.  A.rec.pause = 0.  The
+// grid covers every pixel; the lanes past the parked count (*A.rec.n_list) return at once.
 template <int CHART>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
-void kt_adjoint_resume_kernel(TraceArgs A, int64_t n) {
+void kt_adjoint_resume_kernel(TraceArgs A) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (int64_t)*A.rec.n_list) return;
     __builtin_amdgcn_s_setprio(3);
     const FrameDev F = A.frames[0];
     const Knds<double> P = F.P;
@@ -1168,7 +1171,7 @@ struct kt_hip_ctx {
     RayInit* d_init = nullptr; size_t cap_init = 0;     // per-pixel initial states
     uint8_t* d_bg = nullptr; int bg_w = 0, bg_h = 0; size_t cap_bg = 0;
     unsigned long long* d_words = nullptr;
-    unsigned long long* h_words = nullptr;  // pinned: the statistics copied back on the stream
+    unsigned long long* h_words = nullptr;  // pinned, kHostWords: the statistics copied back on the stream
     uint8_t* d_rgb = nullptr; size_t cap_rgb = 0;       // camera-spp resolved RGB8
     double* d_tab = nullptr; double tab_gamma = -1.0;  // camera-spp linear table + thresholds
     // autotuner results per (chart, integrator, precision, bundles)
@@ -1329,7 +1332,7 @@ int kt_hip_create(int device, kt_hip_ctx** out) {
     for (int i = 0; i < 2; ++i)
         if (hipEventCreateWithFlags(&c->ev2[i], hipEventDisableTiming) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
     if (hipMalloc(&c->d_words, W_COUNT * sizeof(unsigned long long)) != hipSuccess ||
-        hipHostMalloc(&c->h_words, W_COUNT * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&c->h_words, kHostWords * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         kt_hip_destroy(c);
         return KT_E_ALLOC;
     }
@@ -2736,10 +2739,11 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     kt_adjoint_stats S;
     std::memset(&S, 0, sizeof(S));
     // The forward pass runs in two parts.  The first takes every ray up to kAdjPause accepted
-    // steps; a ray still running then is parked.  The parked rays (a few percent, the longest)
-    // are finished by the resume kernel on the second stream, while the reverse pass sweeps the
-    // pixels whose rays have ended; the parked pixels are swept after that.  A ray longer than
-    // the record stride makes the host enlarge it and run the call again.
+    // steps; a ray still running then is parked.  The parked rays (the longest) are finished by
+    // the resume kernel on the second stream, while the reverse pass sweeps the pixels whose rays
+    // have ended; the parked pixels are swept after that.  The host synchronises once per pass,
+    // at the end: a ray longer than the record stride, or more reverse-pass entries than the
+    // context holds, makes it enlarge the buffers and run the call again.
     int32_t cap = c->adj_next_cap;
     // info: 4 counts and the entry table's class counters for each of the two reverse passes
     constexpr int kInfo = 4 + 2 * kAdjClasses;
@@ -2752,10 +2756,19 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     int64_t* infoA = c->d_adj_info;
     int64_t* infoB = c->d_adj_info + kInfo;
     auto cls_of = [](int64_t* info) { return reinterpret_cast<unsigned long long*>(info + 4); };
-    int64_t hA[5] = {0, 0, 0, 0, 0}, hB[4] = {0, 0, 0, 0};
+    // pinned read-backs: the first part's counts and parked rays, the whole pass's counts, the
+    // statistics words and the gradient
+    int64_t* hA = reinterpret_cast<int64_t*>(c->h_words) + 8;
+    int64_t* hB = reinterpret_cast<int64_t*>(c->h_words) + 16;
+    unsigned long long* hw = c->h_words + 24;
+    double* hg = reinterpret_cast<double*>(c->h_words + 32);
+    static_assert(W_COUNT <= 8 && AP_N <= 8 && 40 <= kHostWords, "adjoint read-back layout");
     AdjBackArgs B;
-    for (int pass = 0; pass < 2; ++pass) {
+    // reverse-pass entries: the context's capacity, at least two per pixel on its first call
+    int64_t entries = std::max<int64_t>(c->adj_entries, 2 * pix);
+    for (int pass = 0; pass < 3; ++pass) {
         if ((rc = adjoint_buffers(c, pix, cap))) return rc;
+        if ((rc = adjoint_rev_buffers(c, pix, entries))) return rc;
         const int32_t pause = kAdjPause < cap ? kAdjPause : 0;
         AdjRecord rec;
         rec.n_acc = c->d_adj_nacc;
@@ -2773,23 +2786,17 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_adj_words2, (int)W_COUNT);
         rc = adjoint_forward(c, &o, chart, rays, rec);
         if (rc) return rc;
+        // the first part's counts (the pixels whose rays ended in it), read before the resume
+        // kernel changes the parked pixels' counts
         hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc, pix, infoA,
                            cls_of(infoA));
         KT_HIP(c, hipGetLastError());
         KT_HIP(c, hipMemcpyAsync(hA, infoA, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
         KT_HIP(c, hipMemcpyAsync(&hA[4], c->d_adj_words2, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
-        KT_HIP(c, hipStreamSynchronize(c->stream));
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-        if (pass == 0) S.forward_ms = ms; else S.record_ms = ms;
-        const int64_t parked = hA[4];
-        // reverse-pass capacity: the ended pixels' entries, and at most cap/16 + 1 per parked pixel
-        const int64_t entries = std::max<int64_t>({hA[1], parked * (cap / kCkptEvery + 1), 1});
-        if ((rc = adjoint_rev_buffers(c, pix, entries))) return rc;
-        // 2. the parked rays on the second stream, then their counts
+        // 2. the parked rays on the second stream, then the counts of the whole pass
         KT_HIP(c, hipEventRecord(c->ev2[0], c->stream));
         KT_HIP(c, hipStreamWaitEvent(c->stream2, c->ev2[0], 0));
-        if (parked > 0) {
+        {
             TraceArgs A;
             std::memset(&A, 0, sizeof(A));
             A.frames = c->table.frames();
@@ -2798,15 +2805,16 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
             A.rec.pause = 0;
             A.rec.end_phase = 2;
             A.O = run_opts(&o);
-            const unsigned rblocks = (unsigned)((parked + kBlock - 1) / kBlock);
+            const unsigned rblocks = (unsigned)((pix + kBlock - 1) / kBlock);
             if (chart == KT_CHART_KS)
-                hipLaunchKernelGGL(kt_adjoint_resume_kernel<CHART_KS>, dim3(rblocks), dim3(kBlock), 0, c->stream2, A, parked);
+                hipLaunchKernelGGL(kt_adjoint_resume_kernel<CHART_KS>, dim3(rblocks), dim3(kBlock), 0, c->stream2, A);
             else
-                hipLaunchKernelGGL(kt_adjoint_resume_kernel<CHART_BL>, dim3(rblocks), dim3(kBlock), 0, c->stream2, A, parked);
+                hipLaunchKernelGGL(kt_adjoint_resume_kernel<CHART_BL>, dim3(rblocks), dim3(kBlock), 0, c->stream2, A);
         }
         hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream2, c->d_adj_nacc, pix, infoB,
                            cls_of(infoB));
         KT_HIP(c, hipGetLastError());
+        KT_HIP(c, hipMemcpyAsync(hB, infoB, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream2));
         KT_HIP(c, hipEventRecord(c->ev2[1], c->stream2));
         // 3. the reverse pass over the pixels that ended in the first part, beside step 2
         B.frame = c->table.frames();
@@ -2824,9 +2832,9 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         B.max_entries = c->adj_entries;
         B.out = c->d_adj_out;
         hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_words, (int)W_COUNT);
-        KT_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        KT_HIP(c, hipEventRecord(c->ev[2], c->stream));
         const int64_t pblocks = std::min<int64_t>((pix + 255) / 256, (int64_t)c->n_cu * 8);
-        const int64_t sblocks = std::max<int64_t>(1, (4 * entries + 255) / 256);
+        const int64_t sblocks = std::max<int64_t>(1, (4 * c->adj_entries + 255) / 256);
         for (int part = 1; part <= 2; ++part) {
             if (part == 2) KT_HIP(c, hipStreamWaitEvent(c->stream, c->ev2[1], 0));   // the parked rays have ended
             B.want = part;
@@ -2843,41 +2851,49 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
             }
             KT_HIP(c, hipGetLastError());
         }
-        KT_HIP(c, hipEventRecord(c->ev[1], c->stream));
-        KT_HIP(c, hipMemcpyAsync(hB, infoB, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+        KT_HIP(c, hipEventRecord(c->ev[3], c->stream));
+        // 4. the statistics and the weighted gradient, read back with the counts
+        KT_HIP(c, hipMemcpyAsync(hw, c->d_words, W_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        if (grad_out) {
+            if (w) KT_HIP(c, hipMemcpyAsync(c->d_adj_w, w, pix * 8, hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL(kt_adjoint_sum_kernel, dim3(kSumBlocks), dim3(256), 0, c->stream, c->d_adj_grad,
+                               w ? c->d_adj_w : nullptr, pix, c->d_adj_part);
+            hipLaunchKernelGGL(kt_adjoint_sum_final_kernel, dim3(1), dim3(64), 0, c->stream, c->d_adj_part, c->d_adj_sum);
+            KT_HIP(c, hipGetLastError());
+            KT_HIP(c, hipMemcpyAsync(hg, c->d_adj_sum, AP_N * 8, hipMemcpyDeviceToHost, c->stream));
+        }
         KT_HIP(c, hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+        if (pass == 0) S.forward_ms = ms; else S.record_ms += ms;
+        // entries the two reverse passes needed: every ended pixel's bound, by the part it ended in
+        const int64_t need = std::max<int64_t>(hA[1], hB[1] - hA[1]);
         const int64_t nmax = hB[0];
-        if (nmax <= cap) break;
-        if (pass == 1)
-            return fail(c, KT_E_DEVICE, "adjoint records: %lld steps after enlarging to %d", (long long)nmax, cap);
+        if (nmax <= cap && need <= c->adj_entries) break;
+        if (pass == 2)
+            return fail(c, KT_E_DEVICE, "adjoint records: %lld steps, %lld entries after enlarging to %d, %lld",
+                        (long long)nmax, (long long)need, cap, (long long)c->adj_entries);
         if (nmax > (int64_t)1 << 30) return fail(c, KT_E_DEVICE, "adjoint records: %lld steps", (long long)nmax);
-        cap = (int32_t)((nmax + kCkptEvery - 1) / kCkptEvery * kCkptEvery);
-        c->adj_next_cap = cap;
+        if (nmax > cap) {
+            // a longer stride changes the entry bound too: size both from this pass's counts
+            cap = (int32_t)((nmax + kCkptEvery - 1) / kCkptEvery * kCkptEvery);
+            c->adj_next_cap = cap;
+        }
+        entries = std::max(need, c->adj_entries);
     }
     const int64_t nh = hB[2], nc = hB[3];
     S.steps = nh;
     S.checkpoints = nc;
     S.record_bytes = nh * 8 + nc * 48;
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
     S.backward_ms = ms;
-    unsigned long long words[W_COUNT];
-    KT_HIP(c, hipMemcpy(words, c->d_words, sizeof(words), hipMemcpyDeviceToHost));
-    S.rays_differentiated = (int64_t)words[0];
-    S.seed_failures = (int64_t)words[1];
-    S.reverse_entries = (int64_t)(words[2] + words[5]);
+    S.rays_differentiated = (int64_t)hw[0];
+    S.seed_failures = (int64_t)hw[1];
+    S.reverse_entries = (int64_t)(hw[2] + hw[5]);
     S.rays_parked = hA[4];
-    S.steps_differentiated = (int64_t)words[3];
-    if (grad_out) {
-        if (w) KT_HIP(c, hipMemcpyAsync(c->d_adj_w, w, pix * 8, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(kt_adjoint_sum_kernel, dim3(kSumBlocks), dim3(256), 0, c->stream, c->d_adj_grad,
-                           w ? c->d_adj_w : nullptr, pix, c->d_adj_part);
-        hipLaunchKernelGGL(kt_adjoint_sum_final_kernel, dim3(1), dim3(64), 0, c->stream, c->d_adj_part, c->d_adj_sum);
-        KT_HIP(c, hipGetLastError());
-        // the context stream does not synchronise with the null stream: copy on it and wait
-        KT_HIP(c, hipMemcpyAsync(grad_out, c->d_adj_sum, AP_N * 8, hipMemcpyDeviceToHost, c->stream));
-        KT_HIP(c, hipStreamSynchronize(c->stream));
-    }
+    S.steps_differentiated = (int64_t)hw[3];
+    if (grad_out) std::memcpy(grad_out, hg, AP_N * 8);
     if (dI_out) KT_HIP(c, hipMemcpy(dI_out, c->d_adj_grad, pix * 8 * AP_N, hipMemcpyDeviceToHost));
     if (I_out) KT_HIP(c, hipMemcpy(I_out, c->d_adj_inten, pix * 8, hipMemcpyDeviceToHost));
     if (ast) *ast = S;
