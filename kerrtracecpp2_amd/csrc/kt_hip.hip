@@ -720,6 +720,8 @@ struct AdjBackArgs {
                                    // segment's entry, its event entry (-1 when it has none)
     unsigned long long* cls;       // kAdjClasses class counts (classes kernel), then kAdjClasses cursors
     unsigned long long* n_entries; // this pass's entries (the table kernel counts them)
+    const unsigned long long* first; // the first reverse pass's entry count, where this pass's
+                                   // entries start (nullptr for the first pass)
     const int8_t* phase;           // per pixel: the forward pass in which its ray ended
     int32_t want;                  // the pixels of this reverse pass: phase[p] == want
     int32_t* tab;                  // per entry: pixel, segment (-1: the event entry)
@@ -764,7 +766,7 @@ __global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
     const unsigned long long* hist = B.cls;
     unsigned long long* cur = B.cls + kAdjClasses;
     if (t == 0) {
-        unsigned long long o = 0;
+        unsigned long long o = B.first ? *B.first : 0ull;
         for (int c = 0; c < kAdjClasses; ++c) { off[c] = o; o += hist[c]; }
     }
     unsigned long long steps = 0, entries = 0;
@@ -787,7 +789,8 @@ __global__ __launch_bounds__(256) void kt_adjoint_table_kernel(AdjBackArgs B) {
         __syncthreads();
         if (t < kAdjClasses) base[t] = cnt[t] ? atomicAdd(&cur[t], (unsigned long long)cnt[t]) : 0ull;
         __syncthreads();
-        if (p >= B.n_pix) continue;
+        // the other reverse pass's pixels keep their bases: the two passes may run side by side
+        if (p >= B.n_pix || B.phase[p] != B.want) continue;
         int64_t* bp = B.base + 3*p;
         const int64_t jf = nseg >= 2 ? (int64_t)(off[0] + base[0] + rf) : -1;
         const int64_t jl = nseg >= 1 ? (int64_t)(off[cl] + base[cl] + rl) : -1;
@@ -893,9 +896,10 @@ void kt_adjoint_segments_kernel(AdjBackArgs B) {
     LdsStates st;
     st.slot = states[threadIdx.x >> 2];
     st.writer = (threadIdx.x & 3) == 0;
-    const int64_t used = (int64_t)*B.n_entries;
-    const int64_t lanes = 4 * (used < B.max_entries ? used : B.max_entries);
-    for (int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; L < lanes; L += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j0 = B.first ? (int64_t)*B.first : 0;
+    const int64_t end = j0 + (int64_t)*B.n_entries;
+    const int64_t lanes = 4 * (end < B.max_entries ? end : B.max_entries);
+    for (int64_t L = 4*j0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; L < lanes; L += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = L >> 2;
         const int d = (int)(L & 3);
         const int64_t p = B.tab[2*j];
@@ -1159,7 +1163,7 @@ struct kt_hip_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // the adjoint's second stream (default priority): parked rays beside the reverse pass
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev2[2] = {nullptr, nullptr};
+    hipEvent_t ev2[3] = {nullptr, nullptr, nullptr};
     std::string err;
     char name[256] = {0};
     int n_cu = 0;
@@ -1329,7 +1333,7 @@ int kt_hip_create(int device, kt_hip_ctx** out) {
     for (int i = 0; i < 4; ++i)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
     if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 3; ++i)
         if (hipEventCreateWithFlags(&c->ev2[i], hipEventDisableTiming) != hipSuccess) { kt_hip_destroy(c); return KT_E_DEVICE; }
     if (hipMalloc(&c->d_words, W_COUNT * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(&c->h_words, kHostWords * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
@@ -1365,7 +1369,7 @@ void kt_hip_destroy(kt_hip_ctx* c) {
     (void)hipFree(c->d_adj_park); (void)hipFree(c->d_adj_phase); (void)hipFree(c->d_adj_list);
     (void)hipFree(c->d_adj_words2);
     for (int i = 0; i < 4; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
-    for (int i = 0; i < 2; ++i) if (c->ev2[i]) (void)hipEventDestroy(c->ev2[i]);
+    for (int i = 0; i < 3; ++i) if (c->ev2[i]) (void)hipEventDestroy(c->ev2[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
@@ -2756,14 +2760,13 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
     int64_t* infoA = c->d_adj_info;
     int64_t* infoB = c->d_adj_info + kInfo;
     auto cls_of = [](int64_t* info) { return reinterpret_cast<unsigned long long*>(info + 4); };
-    // pinned read-backs: the first part's counts and parked rays, the whole pass's counts, the
-    // statistics words and the gradient
+    // pinned read-backs: the parked rays, the whole pass's counts, the statistics words and the
+    // gradient
     int64_t* hA = reinterpret_cast<int64_t*>(c->h_words) + 8;
     int64_t* hB = reinterpret_cast<int64_t*>(c->h_words) + 16;
     unsigned long long* hw = c->h_words + 24;
     double* hg = reinterpret_cast<double*>(c->h_words + 32);
     static_assert(W_COUNT <= 8 && AP_N <= 8 && 40 <= kHostWords, "adjoint read-back layout");
-    AdjBackArgs B;
     // reverse-pass entries: the context's capacity, at least two per pixel on its first call
     int64_t entries = std::max<int64_t>(c->adj_entries, 2 * pix);
     for (int pass = 0; pass < 3; ++pass) {
@@ -2786,12 +2789,10 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_adj_words2, (int)W_COUNT);
         rc = adjoint_forward(c, &o, chart, rays, rec);
         if (rc) return rc;
-        // the first part's counts (the pixels whose rays ended in it), read before the resume
-        // kernel changes the parked pixels' counts
+        // the first part's class counters zeroed, and the parked count read back
         hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc, pix, infoA,
                            cls_of(infoA));
         KT_HIP(c, hipGetLastError());
-        KT_HIP(c, hipMemcpyAsync(hA, infoA, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
         KT_HIP(c, hipMemcpyAsync(&hA[4], c->d_adj_words2, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
         // 2. the parked rays on the second stream, then the counts of the whole pass
         KT_HIP(c, hipEventRecord(c->ev2[0], c->stream));
@@ -2815,8 +2816,10 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
                            cls_of(infoB));
         KT_HIP(c, hipGetLastError());
         KT_HIP(c, hipMemcpyAsync(hB, infoB, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream2));
-        KT_HIP(c, hipEventRecord(c->ev2[1], c->stream2));
-        // 3. the reverse pass over the pixels that ended in the first part, beside step 2
+        // 3. the reverse pass over the pixels that ended in the first part, beside step 2; then,
+        // on the second stream once the parked rays have ended, the reverse pass over theirs.  The
+        // second pass's entries follow the first's in the entry table, so the two may overlap.
+        AdjBackArgs B;
         B.frame = c->table.frames();
         B.n_pix = pix;
         B.n_acc = c->d_adj_nacc;
@@ -2836,21 +2839,27 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         const int64_t pblocks = std::min<int64_t>((pix + 255) / 256, (int64_t)c->n_cu * 8);
         const int64_t sblocks = std::max<int64_t>(1, (4 * c->adj_entries + 255) / 256);
         for (int part = 1; part <= 2; ++part) {
-            if (part == 2) KT_HIP(c, hipStreamWaitEvent(c->stream, c->ev2[1], 0));   // the parked rays have ended
+            hipStream_t s = part == 1 ? c->stream : c->stream2;
+            // the second pass starts once the first one's entries are counted (its table kernel)
+            if (part == 2) KT_HIP(c, hipStreamWaitEvent(s, c->ev2[2], 0));
             B.want = part;
             B.cls = cls_of(part == 1 ? infoA : infoB);
             B.n_entries = c->d_words + (part == 1 ? 2 : 5);
-            hipLaunchKernelGGL(kt_adjoint_classes_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
-            hipLaunchKernelGGL(kt_adjoint_table_kernel, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+            B.first = part == 1 ? nullptr : c->d_words + 2;
+            hipLaunchKernelGGL(kt_adjoint_classes_kernel, dim3((unsigned)pblocks), dim3(256), 0, s, B);
+            hipLaunchKernelGGL(kt_adjoint_table_kernel, dim3((unsigned)pblocks), dim3(256), 0, s, B);
+            if (part == 1) KT_HIP(c, hipEventRecord(c->ev2[2], s));
             if (chart == KT_CHART_KS) {
-                hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_KS>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
-                hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_KS>, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+                hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_KS>, dim3((unsigned)sblocks), dim3(256), 0, s, B);
+                hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_KS>, dim3((unsigned)pblocks), dim3(256), 0, s, B);
             } else {
-                hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_BL>, dim3((unsigned)sblocks), dim3(256), 0, c->stream, B);
-                hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_BL>, dim3((unsigned)pblocks), dim3(256), 0, c->stream, B);
+                hipLaunchKernelGGL(kt_adjoint_segments_kernel<CHART_BL>, dim3((unsigned)sblocks), dim3(256), 0, s, B);
+                hipLaunchKernelGGL(kt_adjoint_compose_kernel<CHART_BL>, dim3((unsigned)pblocks), dim3(256), 0, s, B);
             }
             KT_HIP(c, hipGetLastError());
         }
+        KT_HIP(c, hipEventRecord(c->ev2[1], c->stream2));
+        KT_HIP(c, hipStreamWaitEvent(c->stream, c->ev2[1], 0));   // the second stream's work has ended
         KT_HIP(c, hipEventRecord(c->ev[3], c->stream));
         // 4. the statistics and the weighted gradient, read back with the counts
         KT_HIP(c, hipMemcpyAsync(hw, c->d_words, W_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
@@ -2866,8 +2875,8 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
         if (pass == 0) S.forward_ms = ms; else S.record_ms += ms;
-        // entries the two reverse passes needed: every ended pixel's bound, by the part it ended in
-        const int64_t need = std::max<int64_t>(hA[1], hB[1] - hA[1]);
+        // entries the two reverse passes needed side by side: every ended pixel's bound
+        const int64_t need = hB[1];
         const int64_t nmax = hB[0];
         if (nmax <= cap && need <= c->adj_entries) break;
         if (pass == 2)
