@@ -2785,15 +2785,13 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         rec.phase = c->d_adj_phase;
         rec.list = c->d_adj_list;
         rec.n_list = c->d_adj_words2;
-        // 1. the first part of the forward pass, to the pause
+        // 1. the first part of the forward pass, to the pause (the parked-ray counter and the first
+        // reverse pass's class counters zeroed before it)
         hipLaunchKernelGGL(kt_fill_kernel, dim3(1), dim3(64), 0, c->stream, c->d_adj_words2, (int)W_COUNT);
+        hipLaunchKernelGGL(kt_fill_kernel, dim3((2 * kAdjClasses + 63) / 64), dim3(64), 0, c->stream, cls_of(infoA),
+                           2 * kAdjClasses);
         rc = adjoint_forward(c, &o, chart, rays, rec);
         if (rc) return rc;
-        // the first part's class counters zeroed, and the parked count read back
-        hipLaunchKernelGGL(kt_adjoint_count_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_adj_nacc, pix, infoA,
-                           cls_of(infoA));
-        KT_HIP(c, hipGetLastError());
-        KT_HIP(c, hipMemcpyAsync(&hA[4], c->d_adj_words2, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
         // 2. the parked rays on the second stream, then the counts of the whole pass
         KT_HIP(c, hipEventRecord(c->ev2[0], c->stream));
         KT_HIP(c, hipStreamWaitEvent(c->stream2, c->ev2[0], 0));
@@ -2863,6 +2861,7 @@ int kt_hip_render_adjoint(kt_hip_ctx* c, const kt_frame* f, const kt_opts* oin, 
         KT_HIP(c, hipEventRecord(c->ev[3], c->stream));
         // 4. the statistics and the weighted gradient, read back with the counts
         KT_HIP(c, hipMemcpyAsync(hw, c->d_words, W_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        KT_HIP(c, hipMemcpyAsync(&hA[4], c->d_adj_words2, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
         if (grad_out) {
             if (w) KT_HIP(c, hipMemcpyAsync(c->d_adj_w, w, pix * 8, hipMemcpyHostToDevice, c->stream));
             hipLaunchKernelGGL(kt_adjoint_sum_kernel, dim3(kSumBlocks), dim3(256), 0, c->stream, c->d_adj_grad,
