@@ -324,8 +324,8 @@ __global__ void kt_setup_final_kernel(FrameDev* frames, const LaunchDesc* desc, 
 // sizes are hs[p cap ..], its checkpoints ckpt[6 (p cap/16) ..]; a ray longer than cap steps
 // stops recording (its count tells the host, which enlarges cap and runs the pass again)
 // The adjoint's forward pass parks the rays still running after this many accepted steps (a
-// multiple of the checkpoint interval): at C5 that is the first part of the pass for most of the
-// frame, and the parked rays, a few percent, are its long tail.
+// multiple of the checkpoint interval): for a 256x256 disk image of Kerr-Newman a=0.7 at r_obs = 40
+// that is the whole pass for 85 % of the rays, and the parked 15 % are its long tail.
 constexpr int32_t kAdjPause = 128;
 // pinned host words per context: W_COUNT render statistics, then the adjoint's read-backs
 constexpr int kHostWords = 64;
@@ -884,8 +884,8 @@ struct LdsStates {
     }
 };
 
-// One wave per SIMD, at the compiler's own register budget (no spills): 1.75 ms for the C5
-// reverse pass against 2.25 ms at two waves per SIMD, where the step adjoint's stage arrays spill
+// One wave per SIMD, at the compiler's own register budget (no spills): 1.75 ms for the 256x256
+// Kerr-Newman reverse pass against 2.25 ms at two waves per SIMD, where the step adjoint's stage arrays spill
 // about 250 registers.  (The earlier adjoint of a per-stage rotation had fewer live values and
 // went the other way: 1.88 ms at two waves per SIMD, 2.26 ms at one.)
 template <int CHART>

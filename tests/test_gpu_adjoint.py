@@ -20,7 +20,7 @@ from tests import _hostcheck as hc
 
 pytestmark = pytest.mark.gpu
 
-C5 = dict(a=0.7, Q=0.2, Lambda=0.0, theta=60.0, r_obs=40.0)
+KN = dict(a=0.7, Q=0.2, Lambda=0.0, theta=60.0, r_obs=40.0)
 W = H = 256
 NAMES = ["a", "Q", "Lambda", "theta", "r_obs"]
 
@@ -32,7 +32,7 @@ def _opts():
 
 
 def _frame(**kw):
-    p = dict(C5)
+    p = dict(KN)
     p.update(kw)
     return kt.make_frame(W, H, fov=30.0, disk_out=12.0, **p)
 
@@ -50,7 +50,7 @@ def _interior(I):
 @pytest.fixture(scope="module")
 def base(gpu_ctx):
     out = gpu_ctx.render_adjoint(_frame(), _opts(), want_pixel_grads=True)
-    print("C5 adjoint stats", out["stats"], flush=True)
+    print("256x256 adjoint stats", out["stats"], flush=True)
     return out
 
 
@@ -82,7 +82,7 @@ def test_adjoint_record_stride_kept_between_calls(base):
             assert np.array_equal(out["dI_dp"], base["dI_dp"])
             assert np.array_equal(out["intensity"], base["intensity"])
             assert out["stats"]["steps"] == base["stats"]["steps"]
-        big = kt.make_frame(320, 288, fov=30.0, disk_out=12.0, **C5)
+        big = kt.make_frame(320, 288, fov=30.0, disk_out=12.0, **KN)
         grown = ctx.render_adjoint(big, _opts(), want_pixel_grads=True)
     finally:
         ctx.close()
@@ -131,8 +131,8 @@ def test_per_pixel_adjoint_vs_central_differences(gpu_ctx, base):
         if k == "Lambda":
             continue     # Lambda = 0 switches the KS initial covector form (reference quirk): no two-sided FD
         h = 1e-6
-        Ip, _ = gpu_ctx.render_intensity(_frame(**{k: C5[k] + h}), o)
-        Im, _ = gpu_ctx.render_intensity(_frame(**{k: C5[k] - h}), o)
+        Ip, _ = gpu_ctx.render_intensity(_frame(**{k: KN[k] + h}), o)
+        Im, _ = gpu_ctx.render_intensity(_frame(**{k: KN[k] - h}), o)
         fd = (Ip - Im) / (2 * h)
         m = mask & (Ip > 0) & (Im > 0)
         err = np.abs(G[..., q] - fd)[m]
@@ -146,7 +146,7 @@ def test_per_pixel_adjoint_vs_central_differences(gpu_ctx, base):
 def _loss_setup(gpu_ctx):
     rng = np.random.default_rng(2025)
     d = rng.uniform(-1, 1, 3) * np.array([0.02, 0.02, 2.0])
-    target, _ = gpu_ctx.render_intensity(_frame(a=C5["a"] + d[0], Q=C5["Q"] + d[1], theta=C5["theta"] + d[2]),
+    target, _ = gpu_ctx.render_intensity(_frame(a=KN["a"] + d[0], Q=KN["Q"] + d[1], theta=KN["theta"] + d[2]),
                                          _opts())
     I0, _ = gpu_ctx.render_intensity(_frame(), _opts())
     return target, _interior(I0)
@@ -173,7 +173,7 @@ def test_c5_loss_gradient_vs_central_differences(gpu_ctx):
         if k == "Lambda":
             continue
         h = 1e-6
-        I = {m: gpu_ctx.render_intensity(_frame(**{k: C5[k] + m * h}), _opts())[0] for m in (-2, -1, 1, 2)}
+        I = {m: gpu_ctx.render_intensity(_frame(**{k: KN[k] + m * h}), _opts())[0] for m in (-2, -1, 1, 2)}
         d1 = (I[1] - I[-1]) / (2 * h)
         d2 = (I[2] - I[-2]) / (4 * h)
         jump = mask & (np.abs(d1 - d2) > 1e-3 * np.abs(d2) + 1e-6 * np.abs(d2[mask]).max())
@@ -182,7 +182,7 @@ def test_c5_loss_gradient_vs_central_differences(gpu_ctx):
         g = float((w * G[..., q])[keep].sum())
         rel[k] = abs(g - fd) / abs(fd)
         dropped[k] = int(jump.sum())
-    print("C5 loss gradient: rel err vs FD", rel, "pixels with a jump in the stencil", dropped,
+    print("256x256 loss gradient: rel err vs FD", rel, "pixels with a jump in the stencil", dropped,
           "of", int(mask.sum()), flush=True)
     assert max(rel.values()) <= 1e-3
     assert max(dropped.values()) <= 1e-3 * mask.sum()
@@ -196,7 +196,7 @@ def test_torch_autograd_function(gpu_ctx):
     from kerrtracecpp2_amd.autograd import KerrIntensity
     target, mask = _loss_setup(gpu_ctx)
     spec = dict(width=W, height=H, fov=30.0, disk_out=12.0)
-    p = torch.tensor([C5[k] for k in NAMES], dtype=torch.float64, requires_grad=True)
+    p = torch.tensor([KN[k] for k in NAMES], dtype=torch.float64, requires_grad=True)
     img = KerrIntensity.apply(p, gpu_ctx, spec, _opts())
     tt, mm = torch.from_numpy(target), torch.from_numpy(mask.astype(np.float64))
     loss = (((img - tt) ** 2) * mm).sum()

@@ -6,7 +6,7 @@
   background change ............ the plan re-captures and matches the direct render
   autotuner .................... returns a timed candidate; FP64 results do not depend on the
                                  launch configuration
-  C4 (64 seeded frames, 24.2 M rays, KS DOPRI5 mixed) in one graph launch: per-pixel parity
+  the uneven batch (64 seeded frames, 24.2 M rays, KS DOPRI5 mixed) in one graph launch: per-pixel parity
                                  with the oracle on sampled pixels of 6 frames, outside the
                                  photon-ring band of the GPU FP64 image
 """
@@ -129,7 +129,7 @@ def test_c4_batched_graph_vs_oracle(gpu_ctx):
     try:
         out = plan.render(frames, want_rgba=False)
         tune = plan.tuning()
-        print("C4 graph render", out["stats"]["trace_ms"], "ms, tuning", tune, flush=True)
+        print("64-frame batch graph render", out["stats"]["trace_ms"], "ms, tuning", tune, flush=True)
         again = plan.render(frames, want_rgba=False)
     finally:
         plan.close()
@@ -162,7 +162,7 @@ def test_c4_batched_graph_vs_oracle(gpu_ctx):
         d = keep & (got["outcome"] == 1) & (ref["outcome"] == 1)
         if d.any():
             worst_dg = max(worst_dg, float(np.abs(got["redshift"][d] - ref["redshift"][d]).max()))
-    print("C4 sample fate agreement", agree, "max |dg|", worst_dg, flush=True)
+    print("64-frame batch sample fate agreement", agree, "max |dg|", worst_dg, flush=True)
     assert min(agree) >= 0.999
     assert worst_dg <= 1e-4
     # a second launch of the same graph reproduces the frame scalars; per-pixel results of the

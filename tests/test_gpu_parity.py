@@ -358,14 +358,14 @@ def test_c3_full_size_bundles_vs_oracle_sample(gpu_ctx):
     o.quirks = int(g["quirks"])
     assert not (o.quirks & kt.KT_QUIRK_BUNDLE_ROOT_SKIP)
     out = gpu_ctx.render([kt.make_frame(W, H, a=0.99, theta=89, r_obs=40, disk_out=12)], o, want_rgba=False)
-    print("C3 bundles stats", out["stats"], flush=True)
+    print("1920x1080 bundles stats", out["stats"], flush=True)
     full = out["geo"]
     band = photon_ring_band(full, W, H).reshape(H, W)[g["py"], g["px"]]
     got = full.reshape(H, W)[g["py"], g["px"]]
     ref = g["pixels"]
     keep = ~band
     fate = (got["outcome"] == ref["outcome"])
-    print("C3 fate agreement all %.6f off-band %.6f (%d off-band)" % (fate.mean(), fate[keep].mean(), keep.sum()),
+    print("1920x1080 bundles fate agreement all %.6f off-band %.6f (%d off-band)" % (fate.mean(), fate[keep].mean(), keep.sum()),
           flush=True)
     assert keep.sum() >= 14000
     assert fate[keep].mean() >= 0.999
@@ -374,7 +374,7 @@ def test_c3_full_size_bundles_vs_oracle_sample(gpu_ctx):
     dr = np.abs(got["r"][d] - ref["r"][d])
     dg = np.abs(got["redshift"][d] - ref["redshift"][d])
     rel = np.abs(got["magnif"][d] / ref["magnif"][d] - 1.0)
-    print("C3 disk max|dr| %.3g max|dg| %.3g magnif rel q50/q95/q99 %s" %
+    print("1920x1080 bundles disk max|dr| %.3g max|dg| %.3g magnif rel q50/q95/q99 %s" %
           (dr.max(), dg.max(), np.quantile(rel, [0.5, 0.95, 0.99])), flush=True)
     assert dr.max() < 2e-3
     assert dg.max() < 1e-3 and np.quantile(dg, 0.999) < 2.5e-4
@@ -384,7 +384,7 @@ def test_c3_full_size_bundles_vs_oracle_sample(gpu_ctx):
     assert e.sum() >= 4000
     dth = np.abs(got["theta_esc"][e] - ref["theta_esc"][e])
     dph = np.abs(got["phi_esc"][e] - ref["phi_esc"][e])
-    print("C3 escape max|dtheta| %.3g max|dphi| %.3g" % (dth.max(), dph.max()), flush=True)
+    print("1920x1080 bundles escape max|dtheta| %.3g max|dphi| %.3g" % (dth.max(), dph.max()), flush=True)
     assert dth.max() < 2e-3 and dph.max() < 5e-3
     hz = keep & (ref["outcome"] == 0) & (ref["theta_esc"] == 0)     # horizon: the empty bundle result
     assert hz.sum() >= 2000
